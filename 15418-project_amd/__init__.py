@@ -1,0 +1,327 @@
+"""Python bindings (ctypes) over the C ABI of the MI355X GpJSON engine.
+
+The product is the HIP library `_build/libgpj_hip.so` (include/gpj_capi.h) and
+the C++ host API built on it.  This module is thin glue for tests and
+benchmarks: it loads the in-tree library, maps status codes onto the
+reference's error classes and wraps device buffers.  There is no fallback:
+if the HIP library is missing or no GPU is present, device calls raise.
+
+Load it with `load_package()` from __graft_entry__ (the directory name is not
+a Python identifier), e.g.
+    gpj = __graft_entry__.load_package()
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "_build", "libgpj_hip.so")
+REPO_ROOT = os.path.dirname(PKG_DIR)
+CAPI_HEADER = os.path.join(REPO_ROOT, "include", "gpj_capi.h")
+
+OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_DEVICE = -2
+ERR_OUT_OF_MEMORY = -3
+ERR_QUERY_EXECUTION = -4
+ERR_MALFORMED_QUERY = -5
+ERR_UNSUPPORTED_QUERY = -6
+ERR_CAPACITY = -7
+ERR_OUT_OF_RANGE = -8
+COMPILE_EXTENSIONS = 1
+MAX_QUERY_IR_BYTES = 4096
+MAX_QUERY_LEVELS = 16
+MAX_INDEX_LEVELS = 22
+
+
+class GpJSONError(Exception):
+    """Base of the reference's error tree (include/gpjson/error/error.hpp:9-17)."""
+
+
+class ImplementationError(GpJSONError):
+    pass
+
+
+class DeviceError(GpJSONError):
+    """CudaError in the reference (src/gpjson/cuda/cuda.cpp:19-29)."""
+
+
+class MalformedQueryError(GpJSONError):
+    pass
+
+
+class UnsupportedQueryError(GpJSONError):
+    pass
+
+
+class QueryExecutionError(GpJSONError):
+    pass
+
+
+class CapacityError(GpJSONError):
+    pass
+
+
+class IndexT(ctypes.Structure):
+    _fields_ = [
+        ("d_newline", ctypes.c_void_p),
+        ("d_num_lines", ctypes.c_void_p),
+        ("num_lines", ctypes.c_int64),
+        ("d_string", ctypes.c_void_p),
+        ("d_leveled", ctypes.c_void_p),
+        ("level_size", ctypes.c_int64),
+        ("num_levels", ctypes.c_int32),
+        ("owns_memory", ctypes.c_int32),
+    ]
+
+
+class IndexOptsT(ctypes.Structure):
+    _fields_ = [
+        ("grid_size", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("reduction_grid_size", ctypes.c_int32),
+        ("reduction_block_size", ctypes.c_int32),
+        ("newline_capacity", ctypes.c_int64),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, i64, i32, u32 = (ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64,
+                                  ctypes.c_int32, ctypes.c_uint32)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.gpj_last_error.restype = ctypes.c_char_p
+        L.gpj_device_available.restype = ctypes.c_int
+        L.gpj_alloc.argtypes = [ctypes.POINTER(vp), u64]
+        L.gpj_free.argtypes = [vp]
+        L.gpj_h2d.argtypes = [vp, vp, u64, vp]
+        L.gpj_d2h.argtypes = [vp, vp, u64, vp]
+        L.gpj_stream_sync.argtypes = [vp]
+        L.gpj_index_build.argtypes = [vp, u64, i32, ctypes.POINTER(IndexOptsT),
+                                      ctypes.POINTER(IndexT), vp]
+        L.gpj_index_free.argtypes = [ctypes.POINTER(IndexT)]
+        L.gpj_index_scratch_bytes.argtypes = [u64]
+        L.gpj_index_scratch_bytes.restype = u64
+        L.gpj_index_build_into.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp, vp]
+        L.gpj_debug_escape_quote.argtypes = [vp, u64, vp, vp, vp]
+        L.gpj_query_run.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp, vp]
+        L.gpj_query_compile.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint8), u32,
+                                        u32p, u32p, u32p, u32]
+        for name in ("gpj_alloc", "gpj_free", "gpj_h2d", "gpj_d2h", "gpj_stream_sync",
+                     "gpj_device_sync", "gpj_index_build", "gpj_index_free",
+                     "gpj_index_build_into", "gpj_debug_escape_quote", "gpj_query_run",
+                     "gpj_query_compile"):
+            getattr(L, name).restype = i32
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status == OK:
+        return
+    msg = lib().gpj_last_error().decode("utf-8", "replace")
+    if what:
+        msg = f"{what}: {msg}"
+    cls = {
+        ERR_MALFORMED_QUERY: MalformedQueryError,
+        ERR_UNSUPPORTED_QUERY: UnsupportedQueryError,
+        ERR_QUERY_EXECUTION: QueryExecutionError,
+        ERR_DEVICE: DeviceError,
+        ERR_OUT_OF_MEMORY: DeviceError,
+        ERR_CAPACITY: CapacityError,
+        ERR_OUT_OF_RANGE: OverflowError,
+        ERR_INVALID_ARGUMENT: ValueError,
+    }.get(status, GpJSONError)
+    raise cls(msg)
+
+
+def device_available() -> bool:
+    return bool(lib().gpj_device_available())
+
+
+def exported_symbols_declared_in_header() -> list[str]:
+    """Function names declared in include/gpj_capi.h."""
+    import re
+    text = open(CAPI_HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(gpj_[a-z_0-9]+)\s*\(",
+                                 text, flags=re.M)))
+
+
+# ---------------------------------------------------------------------------
+# compiler
+# ---------------------------------------------------------------------------
+def compile_query(query: str, extensions: bool = False):
+    """(ir_bytes, max_depth, num_slots); raises MalformedQueryError etc."""
+    buf = (ctypes.c_uint8 * 65536)()
+    n, d, s = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    st = lib().gpj_query_compile(query.encode("latin-1"), buf, 65536, ctypes.byref(n),
+                                 ctypes.byref(d), ctypes.byref(s),
+                                 COMPILE_EXTENSIONS if extensions else 0)
+    check(st)
+    return bytes(buf[: n.value]), d.value, s.value
+
+
+# ---------------------------------------------------------------------------
+# device memory
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """Owning device allocation (DeviceArray, include/gpjson/cuda/cuda.hpp:10-40)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().gpj_alloc(ctypes.byref(p), self.nbytes), "alloc")
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_host(cls, arr, stream=None) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        b = cls(max(a.nbytes, 1))
+        if a.nbytes:
+            check(lib().gpj_h2d(b.ptr, a.ctypes.data, a.nbytes, stream), "h2d")
+            check(lib().gpj_stream_sync(stream))
+        return b
+
+    def to_host(self, dtype, count: int | None = None, stream=None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        cnt = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(cnt, dtype=dt)
+        if cnt:
+            check(lib().gpj_d2h(out.ctypes.data, self.ptr, cnt * dt.itemsize, stream), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().gpj_free(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def d2h(ptr: int, dtype, count: int, stream=None) -> np.ndarray:
+    dt = np.dtype(dtype)
+    out = np.empty(int(count), dtype=dt)
+    if count:
+        check(lib().gpj_d2h(out.ctypes.data, ptr, out.nbytes, stream), "d2h")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# index + query
+# ---------------------------------------------------------------------------
+class BuiltIndex:
+    """Owning wrapper of gpj_index (BuiltIndices, include/gpjson/index/index.hpp:83-109)."""
+
+    def __init__(self, raw: IndexT, n: int):
+        self.raw = raw
+        self.n = n
+
+    @property
+    def num_lines(self) -> int:
+        return int(self.raw.num_lines)
+
+    @property
+    def level_size(self) -> int:
+        return int(self.raw.level_size)
+
+    @property
+    def num_levels(self) -> int:
+        return int(self.raw.num_levels)
+
+    def newline(self) -> np.ndarray:
+        return d2h(self.raw.d_newline, np.int64, self.num_lines)
+
+    def string(self) -> np.ndarray:
+        return d2h(self.raw.d_string, np.uint64, self.level_size)
+
+    def leveled(self) -> np.ndarray:
+        w = self.level_size
+        return d2h(self.raw.d_leveled, np.uint64, w * self.num_levels).reshape(self.num_levels, w)
+
+    def free(self):
+        if self.raw.owns_memory:
+            lib().gpj_index_free(ctypes.byref(self.raw))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def build_index(dev: DeviceBuffer | int, n: int, levels: int, newline_capacity: int = 0,
+                stream=None) -> BuiltIndex:
+    ptr = dev.ptr if isinstance(dev, DeviceBuffer) else int(dev)
+    raw = IndexT()
+    opts = IndexOptsT(0, 0, 0, 0, int(newline_capacity))
+    check(lib().gpj_index_build(ptr, int(n), int(levels), ctypes.byref(opts), ctypes.byref(raw),
+                                stream), "index build")
+    return BuiltIndex(raw, n)
+
+
+def run_query(dev: DeviceBuffer | int, n: int, index: BuiltIndex, ir: bytes, num_results: int,
+              stream=None) -> np.ndarray:
+    """Spans int64 [num_lines, num_results, 2] (-1 = no match), partition-local."""
+    ptr = dev.ptr if isinstance(dev, DeviceBuffer) else int(dev)
+    lines = index.num_lines
+    d_ir = DeviceBuffer.from_host(np.frombuffer(ir, dtype=np.uint8), stream)
+    spans = DeviceBuffer(max(lines * num_results * 16, 16))
+    check(lib().gpj_query_run(ptr, int(n), ctypes.byref(index.raw), d_ir.ptr, len(ir),
+                              int(num_results), lines, spans.ptr, stream), "query")
+    check(lib().gpj_stream_sync(stream))
+    out = spans.to_host(np.int64, lines * num_results * 2, stream)
+    return out.reshape(lines, num_results, 2)
+
+
+def debug_escape_quote(dev: DeviceBuffer | int, n: int, stream=None):
+    ptr = dev.ptr if isinstance(dev, DeviceBuffer) else int(dev)
+    w = (n + 63) // 64
+    e, q = DeviceBuffer(max(w * 8, 8)), DeviceBuffer(max(w * 8, 8))
+    check(lib().gpj_debug_escape_quote(ptr, int(n), e.ptr, q.ptr, stream), "escape/quote")
+    check(lib().gpj_stream_sync(stream))
+    return e.to_host(np.uint64, w), q.to_host(np.uint64, w)
+
+
+# ---------------------------------------------------------------------------
+# synthetic data (csrc/gpj_gen.c)
+# ---------------------------------------------------------------------------
+GEN_PATH = os.path.join(PKG_DIR, "_build", "libgpj_gen.so")
+_gen = None
+
+SCHEMA_NDJSON_2_4K = 0   # 64 MiB plain config: records of 2-4 KiB, query $.a.b
+SCHEMA_BIG_ARRAY = 1     # one top-level array document
+SCHEMA_NDJSON_1K = 2     # throughput config: ~1 KiB records, depth <= 4
+SCHEMA_TWITTER = 3       # twitter-like records
+
+
+def generate(schema: int, target: int, seed: int, threads: int | None = None) -> np.ndarray:
+    """Seeded synthetic JSON as a uint8 array.  NDJSON schemas: `target` is the
+    byte bound; schema 1: `target` is the element count."""
+    global _gen
+    if _gen is None:
+        _gen = ctypes.CDLL(GEN_PATH)
+        _gen.gpj_gen.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_uint64]
+        _gen.gpj_gen.restype = ctypes.c_uint64
+    threads = threads or min(16, os.cpu_count() or 4)
+    cap = target + (1 << 20) if schema != 1 else target * 1024 + (1 << 20)
+    out = np.empty(cap, dtype=np.uint8)
+    n = _gen.gpj_gen(schema, target, seed, threads, out.ctypes.data, cap)
+    if n == 0:
+        raise RuntimeError("generator buffer too small")
+    return out[:n]

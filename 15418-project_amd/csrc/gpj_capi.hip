@@ -1,0 +1,325 @@
+// gpj_capi.hip -- implementation of the C ABI declared in include/gpj_capi.h.
+//
+// Host side of the HIP path: allocation, stream-ordered launches and status
+// mapping.  No C++ exception crosses the ABI; failures return a negative
+// gpj_status and leave a thread-local message for gpj_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "gpj_capi.h"
+#include "gpj_compiler.h"
+#include "gpj_kernels.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+gpj_status fail(gpj_status s, const std::string &msg) {
+  g_error = msg;
+  return s;
+}
+
+gpj_status hip_fail(hipError_t e, const char *ctx) {
+  return fail(e == hipErrorOutOfMemory ? GPJ_ERR_OUT_OF_MEMORY : GPJ_ERR_DEVICE,
+              std::string(ctx) + ": " + hipGetErrorString(e));
+}
+
+#define GPJ_HIP(call, ctx)                                                     \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) return hip_fail(e_, ctx);                            \
+  } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+uint64_t tiles_for(uint64_t n) { return div_up(div_up(n, 64), gpj::kTileWords); }
+
+// Scratch layout of the index build: tile descriptors, then the tile counter.
+uint64_t scratch_bytes(uint64_t n) {
+  return tiles_for(n) * 8 + 256;
+}
+
+__global__ void empty_index_kernel(int64_t *newline, int64_t cap, int64_t *num_lines) {
+  if (cap > 0) newline[0] = 0;
+  *num_lines = 1;
+}
+
+template <int LMAX>
+gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int levels,
+                        uint64_t *d_string, uint64_t *d_leveled, int64_t *d_newline,
+                        int64_t cap, int64_t *d_num_lines, uint64_t *desc,
+                        uint32_t *counter, uint64_t tiles, hipStream_t st) {
+  const int aligned = (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
+  hipLaunchKernelGGL(gpj::index_kernel<LMAX>, dim3((uint32_t)tiles), dim3(gpj::kTileWords),
+                     0, st, d_in, n, words, levels, aligned, d_string, d_leveled,
+                     d_newline, cap, d_num_lines, desc, counter, tiles);
+  GPJ_HIP(hipGetLastError(), "index_kernel launch");
+  return GPJ_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *gpj_last_error(void) { return g_error.c_str(); }
+
+int gpj_device_available(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return count > 0;
+}
+
+gpj_status gpj_alloc(void **d_ptr, uint64_t bytes) {
+  if (!d_ptr) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_alloc: null output pointer");
+  *d_ptr = nullptr;
+  if (bytes == 0) return GPJ_OK;
+  GPJ_HIP(hipMalloc(d_ptr, bytes), "hipMalloc");
+  return GPJ_OK;
+}
+
+gpj_status gpj_free(void *d_ptr) {
+  if (d_ptr) GPJ_HIP(hipFree(d_ptr), "hipFree");
+  return GPJ_OK;
+}
+
+gpj_status gpj_h2d(void *d_dst, const void *h_src, uint64_t bytes, void *stream) {
+  if (bytes == 0) return GPJ_OK;
+  if (!d_dst || !h_src) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_h2d: null pointer");
+  GPJ_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, as_stream(stream)),
+          "hipMemcpyAsync H2D");
+  return GPJ_OK;
+}
+
+gpj_status gpj_d2h(void *h_dst, const void *d_src, uint64_t bytes, void *stream) {
+  if (bytes == 0) return GPJ_OK;
+  if (!h_dst || !d_src) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_d2h: null pointer");
+  GPJ_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, as_stream(stream)),
+          "hipMemcpyAsync D2H");
+  GPJ_HIP(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
+  return GPJ_OK;
+}
+
+gpj_status gpj_stream_sync(void *stream) {
+  GPJ_HIP(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
+  return GPJ_OK;
+}
+
+gpj_status gpj_device_sync(void) {
+  GPJ_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  return GPJ_OK;
+}
+
+uint64_t gpj_index_scratch_bytes(uint64_t n) { return scratch_bytes(n); }
+
+gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
+                                uint64_t *d_string, uint64_t *d_leveled,
+                                int64_t *d_newline, int64_t newline_capacity,
+                                int64_t *d_num_lines, void *d_scratch,
+                                void *stream) {
+  if (levels < 0 || levels > GPJ_MAX_INDEX_LEVELS)
+    return fail(GPJ_ERR_INVALID_ARGUMENT,
+                "index levels must be in [0, " + std::to_string(GPJ_MAX_INDEX_LEVELS) + "]");
+  if (!d_num_lines || (newline_capacity > 0 && !d_newline))
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_build_into: null output");
+  hipStream_t st = as_stream(stream);
+  if (n == 0) {
+    hipLaunchKernelGGL(empty_index_kernel, dim3(1), dim3(1), 0, st, d_newline,
+                       newline_capacity, d_num_lines);
+    GPJ_HIP(hipGetLastError(), "empty_index_kernel launch");
+    return GPJ_OK;
+  }
+  if (!d_in || !d_string || (levels > 0 && !d_leveled) || !d_scratch)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_build_into: null buffer");
+  const uint64_t words = div_up(n, 64);
+  const uint64_t tiles = tiles_for(n);
+  if (tiles > 0xFFFFFFFFull)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "input too large for one index build");
+  uint64_t *desc = static_cast<uint64_t *>(d_scratch);
+  uint32_t *counter = reinterpret_cast<uint32_t *>(desc + tiles);
+  const uint64_t clear_blocks = std::min<uint64_t>(div_up(tiles, 256), 2048);
+  hipLaunchKernelGGL(gpj::index_clear_kernel, dim3((uint32_t)clear_blocks), dim3(256), 0, st,
+                     desc, tiles, counter);
+  GPJ_HIP(hipGetLastError(), "index_clear_kernel launch");
+#define GPJ_DISPATCH(L)                                                            \
+  return launch_index<L>(d_in, n, words, levels, d_string, d_leveled, d_newline,    \
+                         newline_capacity, d_num_lines, desc, counter, tiles, st)
+  if (levels <= 2) GPJ_DISPATCH(2);
+  if (levels <= 4) GPJ_DISPATCH(4);
+  if (levels <= 8) GPJ_DISPATCH(8);
+  if (levels <= 16) GPJ_DISPATCH(16);
+  GPJ_DISPATCH(22);
+#undef GPJ_DISPATCH
+}
+
+gpj_status gpj_index_free(gpj_index *index) {
+  if (!index) return GPJ_OK;
+  if (index->owns_memory) {
+    (void)hipFree(index->d_newline);
+    (void)hipFree(index->d_num_lines);
+    (void)hipFree(index->d_string);
+    (void)hipFree(index->d_leveled);
+  }
+  std::memset(index, 0, sizeof(*index));
+  return GPJ_OK;
+}
+
+gpj_status gpj_index_build(const uint8_t *d_in, uint64_t n, int32_t levels,
+                           const gpj_index_opts *opts, gpj_index *out,
+                           void *stream) {
+  if (!out) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_build: null output");
+  std::memset(out, 0, sizeof(*out));
+  if (levels < 0 || levels > GPJ_MAX_INDEX_LEVELS)
+    return fail(GPJ_ERR_INVALID_ARGUMENT,
+                "index levels must be in [0, " + std::to_string(GPJ_MAX_INDEX_LEVELS) + "]");
+  if (n > 0 && !d_in)
+    return fail(GPJ_ERR_INVALID_ARGUMENT,
+                "Index building requires a GPU-resident partition buffer");
+  hipStream_t st = as_stream(stream);
+  const uint64_t words = div_up(n, 64);
+  int64_t cap = (opts && opts->newline_capacity > 0) ? opts->newline_capacity
+                                                     : (int64_t)(n / 64 + 2);
+  void *scratch = nullptr;
+  gpj_index r;
+  std::memset(&r, 0, sizeof(r));
+  r.owns_memory = 1;
+  r.level_size = (int64_t)words;
+  r.num_levels = levels;
+  auto cleanup = [&]() {
+    (void)hipFree(scratch);
+    gpj_index_free(&r);
+  };
+  hipError_t e;
+#define GPJ_TRY(call, ctx)                                                     \
+  do {                                                                         \
+    e = (call);                                                                \
+    if (e != hipSuccess) {                                                     \
+      gpj_status s_ = hip_fail(e, ctx);                                        \
+      cleanup();                                                               \
+      return s_;                                                               \
+    }                                                                          \
+  } while (0)
+  GPJ_TRY(hipMalloc(&r.d_num_lines, 8), "hipMalloc num_lines");
+  GPJ_TRY(hipMalloc(&r.d_newline, (size_t)cap * 8), "hipMalloc newline index");
+  GPJ_TRY(hipMalloc(&r.d_string, words ? words * 8 : 8), "hipMalloc string index");
+  GPJ_TRY(hipMalloc(&r.d_leveled, words && levels ? words * 8 * (uint64_t)levels : 8),
+          "hipMalloc leveled bitmap index");
+  GPJ_TRY(hipMalloc(&scratch, scratch_bytes(n)), "hipMalloc index scratch");
+  gpj_status s = gpj_index_build_into(d_in, n, levels, r.d_string, r.d_leveled,
+                                      r.d_newline, cap, r.d_num_lines, scratch, stream);
+  if (s != GPJ_OK) {
+    cleanup();
+    return s;
+  }
+  int64_t num_lines = 0;
+  GPJ_TRY(hipMemcpyAsync(&num_lines, r.d_num_lines, 8, hipMemcpyDeviceToHost, st),
+          "copy num_lines");
+  GPJ_TRY(hipStreamSynchronize(st), "index build");
+  if (n > 0) {
+    uint32_t failed = 0;
+    const uint32_t *flag =
+        reinterpret_cast<const uint32_t *>(static_cast<uint64_t *>(scratch) + tiles_for(n)) + 1;
+    GPJ_TRY(hipMemcpy(&failed, flag, 4, hipMemcpyDeviceToHost), "copy index status");
+    if (failed) {
+      cleanup();
+      return fail(GPJ_ERR_DEVICE, "index build: tile look-back timed out");
+    }
+  }
+  if (num_lines > cap) {
+    // exact second pass: count per tile, scan, write
+    const uint64_t tiles = tiles_for(n);
+    const int aligned = (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
+    (void)hipFree(r.d_newline);
+    r.d_newline = nullptr;
+    GPJ_TRY(hipMalloc(&r.d_newline, (size_t)num_lines * 8), "hipMalloc newline index");
+    uint64_t *counts = nullptr;
+    GPJ_TRY(hipMalloc(&counts, tiles * 8), "hipMalloc newline scratch");
+    hipLaunchKernelGGL(gpj::newline_count_kernel, dim3((uint32_t)tiles), dim3(gpj::kTileWords),
+                       0, st, d_in, n, aligned, counts);
+    hipLaunchKernelGGL(gpj::newline_scan_kernel, dim3(1), dim3(1024), 0, st, counts, tiles,
+                       r.d_num_lines);
+    hipLaunchKernelGGL(gpj::newline_write_kernel, dim3((uint32_t)tiles), dim3(gpj::kTileWords),
+                       0, st, d_in, n, aligned, counts, r.d_newline);
+    e = hipStreamSynchronize(st);
+    (void)hipFree(counts);
+    GPJ_TRY(e, "newline index pass");
+  }
+#undef GPJ_TRY
+  (void)hipFree(scratch);
+  r.num_lines = num_lines;
+  *out = r;
+  return GPJ_OK;
+}
+
+gpj_status gpj_debug_escape_quote(const uint8_t *d_in, uint64_t n, uint64_t *d_escape,
+                                  uint64_t *d_quote, void *stream) {
+  if (n == 0) return GPJ_OK;
+  if (!d_in || !d_escape || !d_quote)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_debug_escape_quote: null pointer");
+  const uint64_t words = div_up(n, 64);
+  hipLaunchKernelGGL(gpj::escape_quote_kernel, dim3((uint32_t)div_up(words, 256)), dim3(256),
+                     0, as_stream(stream), d_in, n, words, d_escape, d_quote);
+  GPJ_HIP(hipGetLastError(), "escape_quote_kernel launch");
+  return GPJ_OK;
+}
+
+gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                         const uint8_t *d_ir, uint32_t ir_len, int32_t num_results,
+                         int64_t max_lines, int64_t *d_spans, void *stream) {
+  if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_query_run: null index");
+  if (ir_len == 0 || ir_len > GPJ_MAX_QUERY_IR_BYTES)
+    return fail(GPJ_ERR_QUERY_EXECUTION, "Compiled query IR exceeds constant memory buffer");
+  if (num_results < 1)
+    return fail(GPJ_ERR_QUERY_EXECUTION, "query needs at least one result slot");
+  if (max_lines <= 0 || n == 0) return GPJ_OK;
+  if (!d_in || !d_ir || !d_spans || !index->d_newline || !index->d_num_lines ||
+      !index->d_string || !index->d_leveled)
+    return fail(GPJ_ERR_QUERY_EXECUTION,
+                "Query execution requires GPU-resident newline, string, and bitmap indices");
+  const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_lines, 256), 1u << 20);
+  hipLaunchKernelGGL(gpj::query_kernel, dim3((uint32_t)blocks), dim3(256), 0, as_stream(stream),
+                     d_in, (int64_t)n, index->d_newline, index->d_num_lines, max_lines,
+                     index->d_string, index->d_leveled, index->level_size,
+                     index->num_levels, d_ir, ir_len,
+                     num_results, d_spans);
+  GPJ_HIP(hipGetLastError(), "query_kernel launch");
+  return GPJ_OK;
+}
+
+gpj_status gpj_query_compile(const char *query, uint8_t *ir, uint32_t ir_cap,
+                             uint32_t *ir_len, uint32_t *max_depth, uint32_t *num_slots,
+                             uint32_t flags) {
+  if (!query || !ir_len || !max_depth || !num_slots)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_query_compile: null pointer");
+  gpj::CompiledBytecode out;
+  std::string err;
+  const gpj::CompileStatus s =
+      gpj::compile_query(query, (flags & GPJ_COMPILE_EXTENSIONS) != 0, &out, &err);
+  switch (s) {
+  case gpj::CompileStatus::kOk: break;
+  case gpj::CompileStatus::kMalformed: return fail(GPJ_ERR_MALFORMED_QUERY, err);
+  case gpj::CompileStatus::kUnsupported: return fail(GPJ_ERR_UNSUPPORTED_QUERY, err);
+  case gpj::CompileStatus::kInvalidArgument: return fail(GPJ_ERR_INVALID_ARGUMENT, err);
+  case gpj::CompileStatus::kOutOfRange: return fail(GPJ_ERR_OUT_OF_RANGE, err);
+  }
+  *ir_len = (uint32_t)out.bytes.size();
+  *max_depth = (uint32_t)out.max_depth;
+  *num_slots = (uint32_t)out.num_slots;
+  if (out.bytes.size() > ir_cap)
+    return fail(GPJ_ERR_CAPACITY, "compiled query does not fit the IR buffer");
+  if (!out.bytes.empty()) {
+    if (!ir) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_query_compile: null IR buffer");
+    std::memcpy(ir, out.bytes.data(), out.bytes.size());
+  }
+  return GPJ_OK;
+}
+
+} // extern "C"

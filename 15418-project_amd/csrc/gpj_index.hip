@@ -1,0 +1,603 @@
+// gpj_index.hip -- structural-index kernels for gfx950 (MI355X).
+//
+// One pass over the input builds the three indices the reference builds with
+// thirteen launches (src/gpjson/index/orig_index_builder.cu:296-395):
+//   * string mask   uint64[W]      (string-index.cu:3-31 semantics)
+//   * leveled rows  uint64[L][W]   (leveled-bitmaps-index.cu:9-73 semantics)
+//   * newline index int64[1+#nl]   (newline-index.cu:5-21 semantics)
+// W = ceil(n/64).  Work unit: one 64-byte "word" per lane, 256 words
+// (16 KiB) per workgroup tile.  Each lane classifies its 64 bytes with
+// byte-parallel SWAR compares and packs the per-class flags into 64-bit masks
+// with v_dot4_u32_u8.  The three carried states (escape parity, in-string
+// parity, nesting depth) plus the newline count cross tiles through a
+// decoupled look-back: each tile publishes, before it knows its carry-in,
+// a 4-entry transfer table indexed by (escape-in, string-in), then resolves
+// its exclusive prefix by walking predecessors' descriptors.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gpj_kernels.h"
+
+namespace gpj {
+
+namespace {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr uint64_t kEven = 0x5555555555555555ull; // bit positions 0,2,4,...
+constexpr uint64_t kOdd = 0xAAAAAAAAAAAAAAAAull;  // bit positions 1,3,5,...
+
+__device__ __forceinline__ uint64_t low_mask(int bits) {
+  return bits >= 64 ? ~0ull : ((1ull << bits) - 1ull);
+}
+// Bits 0..b inclusive (b may be 64 = all).
+__device__ __forceinline__ uint64_t mask_upto(int b) {
+  return b >= 63 ? ~0ull : ((2ull << b) - 1ull);
+}
+
+__device__ __forceinline__ uint64_t prefix_xor64(uint64_t x) {
+  x ^= x << 1;
+  x ^= x << 2;
+  x ^= x << 4;
+  x ^= x << 8;
+  x ^= x << 16;
+  x ^= x << 32;
+  return x;
+}
+
+// ---- byte classification ---------------------------------------------------
+// For a 7-bit byte b and ASCII c: ((b ^ (0x7F ^ c)) + 1) has bit 7 set iff
+// b == c, with no carry out of the byte.  `nh` keeps only bit 7 of bytes
+// whose original value is < 0x80.
+__device__ __forceinline__ uint32_t eq_bytes(uint32_t lo7, uint32_t c) {
+  return (lo7 ^ (0x7F7F7F7Fu ^ (c * 0x01010101u))) + 0x01010101u;
+}
+
+// Two dwords (8 bytes) of 0x80/0x00 flags -> mask byte, returned << 7.
+__device__ __forceinline__ uint32_t pack2(uint32_t f0, uint32_t f1) {
+  return __builtin_amdgcn_udot4(
+      f1, 0x80402010u, __builtin_amdgcn_udot4(f0, 0x08040201u, 0u, false),
+      false);
+}
+// Eight dwords (32 bytes) of flags -> 32-bit mask in byte order.
+__device__ __forceinline__ uint32_t pack8(const uint32_t *f) {
+  return (pack2(f[0], f[1]) >> 7) | (pack2(f[2], f[3]) << 1) |
+         (pack2(f[4], f[5]) << 9) | (pack2(f[6], f[7]) << 17);
+}
+__device__ __forceinline__ uint64_t pack16(const uint32_t *f) {
+  return (uint64_t)pack8(f) | ((uint64_t)pack8(f + 8) << 32);
+}
+
+struct WordClasses {
+  uint64_t quote, backslash, newline, open, close, sep;
+};
+
+__device__ __forceinline__ void load_word(const uint8_t *in, uint64_t n,
+                                          uint64_t pos, bool aligned,
+                                          uint32_t (&x)[16]) {
+  if (pos + 64 <= n && aligned) {
+    const u32x4 *p = reinterpret_cast<const u32x4 *>(in + pos);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const u32x4 v = __builtin_nontemporal_load(p + i);
+      x[4 * i + 0] = v.x;
+      x[4 * i + 1] = v.y;
+      x[4 * i + 2] = v.z;
+      x[4 * i + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        uint64_t q = pos + 4 * i + b;
+        uint32_t c = q < n ? in[q] : 0u;
+        v |= c << (8 * b);
+      }
+      x[i] = v;
+    }
+  }
+}
+
+__device__ __forceinline__ WordClasses classify(const uint32_t (&x)[16]) {
+  uint32_t fq[16], fb[16], fn[16], fo[16], fc[16], fs[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const uint32_t v = x[i];
+    const uint32_t nh = ~v & 0x80808080u;
+    const uint32_t lo = v & 0x7F7F7F7Fu;
+    const uint32_t lo20 = lo | 0x20202020u; // folds '[' onto '{', ']' onto '}'
+    fq[i] = eq_bytes(lo, '"') & nh;
+    fb[i] = eq_bytes(lo, '\\') & nh;
+    fn[i] = eq_bytes(lo, '\n') & nh;
+    fo[i] = eq_bytes(lo20, '{') & nh;
+    fc[i] = eq_bytes(lo20, '}') & nh;
+    fs[i] = (eq_bytes(lo, ':') | eq_bytes(lo, ',')) & nh;
+  }
+  WordClasses w;
+  w.quote = pack16(fq);
+  w.backslash = pack16(fb);
+  w.newline = pack16(fn);
+  w.open = pack16(fo);
+  w.close = pack16(fc);
+  w.sep = pack16(fs);
+  return w;
+}
+
+// Escaped-byte mask of one word for escape carry-in 0: bit i set iff the
+// backslash run ending right before byte i (inside the word) has odd length.
+// Runs starting at even positions escape their odd positions and vice versa;
+// adding the run's start bit to B sweeps the run and marks the byte after it.
+__device__ __forceinline__ uint64_t escaped_from_zero(uint64_t b) {
+  const uint64_t starts = b & ~(b << 1);
+  const uint64_t from_even = (b + (starts & kEven)) ^ b;
+  const uint64_t from_odd = (b + (starts & kOdd)) ^ b;
+  return (from_even & kOdd) | (from_odd & kEven);
+}
+
+// ---- wave / block primitives ----------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+__device__ __forceinline__ uint32_t count_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= (uint32_t)d) v += o;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_reduce_add64(uint64_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// ---- tile descriptors -------------------------------------------------------
+// Aggregate (flag 1): a transfer table over (esc_in, str_in) in {0,1}^2.
+//   entry e = esc_out | str_out << 1 | depth_delta(8 bits) << 2; 10 bits each
+//   bits 0..39: entries at (esc_in*2+str_in)*10; bits 40..61: newline count.
+// Inclusive (flag 2): esc bit 0, str bit 1, depth bits 2..9 (int8),
+//   newline count bits 10..61.
+constexpr uint64_t kFlagShift = 62;
+constexpr uint64_t kFlagAggregate = 1;
+constexpr uint64_t kFlagInclusive = 2;
+constexpr uint32_t kMaxSpins = 1u << 24;
+
+struct Carry {
+  uint32_t esc, str;
+  int32_t depth; // int8 semantics, kept in [-128,127]
+  uint64_t nl;
+};
+
+__device__ __forceinline__ uint32_t tbl_entry(uint64_t tbl, uint32_t esc,
+                                              uint32_t str) {
+  return (uint32_t)(tbl >> ((esc * 2 + str) * 10)) & 0x3FFu;
+}
+// (f o g)(s) = f(g(s)): apply g first.
+__device__ __forceinline__ uint64_t tbl_compose(uint64_t f, uint64_t g) {
+  uint64_t r = 0;
+#pragma unroll
+  for (uint32_t s = 0; s < 4; ++s) {
+    const uint32_t ge = tbl_entry(g, s >> 1, s & 1);
+    const uint32_t fe = tbl_entry(f, ge & 1, (ge >> 1) & 1);
+    const uint32_t dd = ((ge >> 2) + (fe >> 2)) & 0xFFu;
+    const uint32_t e = (fe & 3u) | (dd << 2);
+    r |= (uint64_t)e << (s * 10);
+  }
+  return r;
+}
+__device__ __forceinline__ Carry tbl_apply(uint64_t tbl, uint64_t nl_add,
+                                           Carry c) {
+  const uint32_t e = tbl_entry(tbl, c.esc, c.str);
+  Carry o;
+  o.esc = e & 1u;
+  o.str = (e >> 1) & 1u;
+  o.depth = (int32_t)(int8_t)(uint8_t)((uint32_t)c.depth + (e >> 2));
+  o.nl = c.nl + nl_add;
+  return o;
+}
+__device__ __forceinline__ uint64_t encode_inclusive(Carry c) {
+  return (uint64_t)c.esc | ((uint64_t)c.str << 1) |
+         ((uint64_t)((uint32_t)c.depth & 0xFFu) << 2) |
+         ((c.nl & ((1ull << 52) - 1)) << 10) | (kFlagInclusive << kFlagShift);
+}
+__device__ __forceinline__ Carry decode_inclusive(uint64_t d) {
+  Carry c;
+  c.esc = (uint32_t)d & 1u;
+  c.str = (uint32_t)(d >> 1) & 1u;
+  c.depth = (int32_t)(int8_t)(uint8_t)(d >> 2);
+  c.nl = (d >> 10) & ((1ull << 52) - 1);
+  return c;
+}
+
+__device__ __forceinline__ uint64_t desc_load(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void desc_store(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Add a segment's structural bits to the row of `depth`.
+template <int LMAX>
+__device__ __forceinline__ void add_row(uint64_t (&rows)[LMAX], int depth,
+                                        uint64_t bits) {
+#pragma unroll
+  for (int k = 0; k < LMAX; ++k) rows[k] |= (k == depth) ? bits : 0ull;
+}
+
+} // namespace
+
+// Clears the tile descriptors and the tile counter (kept out of
+// hipMemsetAsync so a captured graph replays it correctly).
+__global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
+                                   uint32_t *counter) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = i; k < count; k += stride) desc[k] = 0;
+  if (i == 0) {
+    counter[0] = 0; // tile counter
+    counter[1] = 0; // look-back failure flag
+  }
+}
+
+// The fused single-pass index kernel.  One workgroup per 16 KiB tile; tile
+// ids come from an atomic counter so every predecessor a tile waits on has
+// already started.
+template <int LMAX>
+__global__ __launch_bounds__(kTileWords) void index_kernel(
+    const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
+    int aligned, uint64_t *__restrict__ string_out,
+    uint64_t *__restrict__ leveled_out, int64_t *__restrict__ newline_out,
+    int64_t newline_cap, int64_t *__restrict__ num_lines_out,
+    uint64_t *__restrict__ desc, uint32_t *__restrict__ tile_counter,
+    uint64_t num_tiles) {
+  uint32_t *const error_flag = tile_counter + 1;
+  __shared__ uint32_t s_tile;
+  __shared__ uint8_t s_bsinfo[kTileWords];  // bit0 trailing parity, bit1 all-backslash
+  __shared__ uint8_t s_lead[kTileWords];    // leading backslash run, bit7 = quote at its end
+  __shared__ uint8_t s_cin[kTileWords];     // escape carry-in per word (tile carry-in 0)
+  __shared__ uint32_t s_wpar[kTileWords / 64];
+  __shared__ uint64_t s_wsum[kTileWords / 64];
+  __shared__ uint32_t s_wscan[kTileWords / 64];
+  __shared__ uint32_t s_first;              // k | lead_k << 16 | g << 24
+  __shared__ uint32_t s_tail_esc;           // tile escape out (X=0) | all-bs << 1
+  __shared__ Carry s_carry;
+
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & 63u;
+  const uint32_t wv = t >> 6;
+
+  if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  const uint64_t w = tile * kTileWords + t;
+  const uint64_t pos = w * 64;
+
+  // 1. classify this lane's 64 bytes
+  WordClasses m = {0, 0, 0, 0, 0, 0};
+  if (pos < n) {
+    uint32_t x[16];
+    load_word(in, n, pos, aligned != 0, x);
+    m = classify(x);
+  }
+
+  // 2. escapes, assuming the tile starts with escape carry 0
+  const uint64_t B = m.backslash;
+  const bool all_bs = (B == ~0ull);
+  const uint64_t e0 = escaped_from_zero(B);
+  const int lead = (B == ~0ull) ? 64 : __builtin_ctzll(~B);
+  const uint32_t trail = (uint32_t)((~B == 0ull) ? 0 : (__builtin_clzll(~B) & 1));
+  s_bsinfo[t] = (uint8_t)(trail | (all_bs ? 2u : 0u));
+  s_lead[t] = (uint8_t)(lead | ((lead < 64 && ((m.quote >> lead) & 1ull)) ? 0x80 : 0));
+  const int any_all_bs = __syncthreads_or(all_bs ? 1 : 0);
+  if (any_all_bs) {
+    if (t == 0) {
+      uint32_t c = 0;
+      for (uint32_t k = 0; k < kTileWords; ++k) {
+        s_cin[k] = (uint8_t)c;
+        const uint8_t info = s_bsinfo[k];
+        c = (info & 2u) ? c : (info & 1u);
+      }
+      s_tail_esc = c;
+      uint32_t k = 0;
+      while (k < kTileWords && (s_lead[k] & 0x7F) >= 64) ++k;
+      if (k == kTileWords) {
+        s_tail_esc = 2u; // every word is backslashes: identity transfer
+        s_first = (uint32_t)kTileWords;
+      } else {
+        s_first = k | ((uint32_t)(s_lead[k] & 0x7F) << 16) |
+                  ((uint32_t)(s_lead[k] >> 7) << 24);
+      }
+    }
+  } else if (t == 0) {
+    s_tail_esc = s_bsinfo[kTileWords - 1] & 1u;
+    s_first = 0u | ((uint32_t)(s_lead[0] & 0x7F) << 16) |
+              ((uint32_t)(s_lead[0] >> 7) << 24);
+  }
+  __syncthreads();
+  const uint32_t cin = any_all_bs ? s_cin[t] : (t ? (s_bsinfo[t - 1] & 1u) : 0u);
+  const uint64_t escaped = cin ? (e0 ^ mask_upto(lead)) : e0;
+
+  // 3. string mask for string carry-in 0
+  const uint64_t q = m.quote & ~escaped;
+  const uint64_t px = prefix_xor64(q);
+  const uint64_t par_ballot = __ballot((uint32_t)(px >> 63));
+  const uint32_t wave_par_before = count_below(par_ballot) & 1u;
+  if (lane == 0) s_wpar[wv] = (uint32_t)__popcll(par_ballot) & 1u;
+
+  // 4. per-word depth deltas for both string polarities, newline counts
+  __syncthreads();
+  uint32_t sin = wave_par_before;
+  uint32_t tile_par = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kTileWords / 64; ++k) {
+    if (k < wv) sin ^= s_wpar[k];
+    tile_par ^= s_wpar[k];
+  }
+  const uint64_t s0 = px ^ (sin ? ~0ull : 0ull);
+  {
+    const int d0 = __popcll(m.open & ~s0) - __popcll(m.close & ~s0);
+    const int tot = __popcll(m.open) - __popcll(m.close);
+    const uint64_t packed = (uint64_t)(d0 + 64) | ((uint64_t)(tot + 64) << 16) |
+                            ((uint64_t)__popcll(m.newline) << 32);
+    const uint64_t r = wave_reduce_add64(packed);
+    if (lane == 0) s_wsum[wv] = r;
+  }
+  __syncthreads();
+
+  // 5. publish the tile aggregate and resolve the exclusive prefix (thread 0)
+  if (t == 0) {
+    uint64_t sum = 0;
+    for (uint32_t k = 0; k < kTileWords / 64; ++k) sum += s_wsum[k];
+    const uint32_t D0 = (uint32_t)((sum & 0xFFFFu) - 64u * kTileWords) & 0xFFu;
+    const uint32_t TOT = (uint32_t)(((sum >> 16) & 0xFFFFu) - 64u * kTileWords) & 0xFFu;
+    const uint64_t NL = sum >> 32;
+    const uint32_t g = (s_first >> 24) & 1u;
+    const uint32_t tail = s_tail_esc;
+    uint64_t tbl = 0;
+    for (uint32_t X = 0; X < 2; ++X) {
+      for (uint32_t Y = 0; Y < 2; ++Y) {
+        const uint32_t eff = Y ^ (X & g);
+        const uint32_t esc_out = (tail & 2u) ? X : (tail & 1u);
+        const uint32_t str_out = Y ^ tile_par ^ (X & g);
+        const uint32_t dd = eff ? ((TOT - D0) & 0xFFu) : D0;
+        const uint64_t e = esc_out | (str_out << 1) | (dd << 2);
+        tbl |= e << ((X * 2 + Y) * 10);
+      }
+    }
+    Carry excl;
+    if (tile == 0) {
+      excl.esc = 0;
+      excl.str = 0;
+      excl.depth = -1;
+      excl.nl = 0;
+    } else {
+      desc_store(&desc[tile], tbl | (NL << 40) | (kFlagAggregate << kFlagShift));
+      // identity transfer: (X,Y) -> (X,Y), delta 0
+      uint64_t acc = 0;
+      for (uint32_t s = 0; s < 4; ++s) acc |= (uint64_t)((s >> 1) | ((s & 1u) << 1)) << (s * 10);
+      uint64_t acc_nl = 0;
+      int64_t p = (int64_t)tile - 1;
+      uint32_t spins = 0;
+      for (;;) {
+        uint64_t d = desc_load(&desc[p]);
+        while ((d >> kFlagShift) == 0) {
+          // Bounded wait: a predecessor always publishes within microseconds;
+          // if it never does, flag the build as failed instead of hanging.
+          if (++spins > kMaxSpins) {
+            __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            d = kFlagInclusive << kFlagShift;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          d = desc_load(&desc[p]);
+        }
+        if ((d >> kFlagShift) == kFlagInclusive) {
+          excl = tbl_apply(acc, acc_nl, decode_inclusive(d));
+          break;
+        }
+        acc = tbl_compose(acc, d & ((1ull << 40) - 1));
+        acc_nl += (d >> 40) & ((1ull << 22) - 1);
+        --p;
+      }
+    }
+    const Carry incl = tbl_apply(tbl, NL, excl);
+    desc_store(&desc[tile], encode_inclusive(incl));
+    if (tile == num_tiles - 1) *num_lines_out = (int64_t)(1 + incl.nl);
+    s_carry = excl;
+  }
+  __syncthreads();
+  const Carry c = s_carry;
+
+  // 6. final string mask: string carry-in, plus the flip of everything from
+  //    the tile's first non-backslash byte on when an escape carry-in of 1
+  //    un-counts a quote there
+  const uint32_t k_first = s_first & 0xFFFFu;
+  const uint32_t lead_k = (s_first >> 16) & 0xFFu;
+  const uint32_t g = (s_first >> 24) & 1u;
+  uint64_t flip = c.str ? ~0ull : 0ull;
+  if (c.esc & g) {
+    if (t > k_first) flip ^= ~0ull;
+    else if (t == k_first) flip ^= ~0ull << lead_k;
+  }
+  const uint64_t S = s0 ^ flip;
+  const uint64_t O = m.open & ~S;
+  const uint64_t C = m.close & ~S;
+  const uint64_t P = m.sep & ~S;
+
+  // 7. block exclusive scan of (depth delta, newline count)
+  const int d = __popcll(O) - __popcll(C);
+  const uint32_t nlc = (uint32_t)__popcll(m.newline);
+  const uint32_t v = ((uint32_t)(d + 64)) | (nlc << 16);
+  const uint32_t incl_v = wave_incl_scan(v);
+  if (lane == 63) s_wscan[wv] = incl_v;
+  __syncthreads();
+  uint32_t excl_v = incl_v - v;
+  for (uint32_t k = 0; k < wv; ++k) excl_v += s_wscan[k];
+  const int depth_in =
+      (int)(int8_t)(uint8_t)((uint32_t)c.depth + (excl_v & 0xFFFFu) - 64u * t);
+  const uint64_t nl_base = 1 + c.nl + (excl_v >> 16);
+
+  if (w >= words) return;
+
+  // 8. leveled rows: walk the depth changes of this word
+  uint64_t rows[LMAX];
+#pragma unroll
+  for (int k = 0; k < LMAX; ++k) rows[k] = 0;
+  {
+    const uint64_t E = O | C | P;
+    uint64_t chg = O | C;
+    int depth = depth_in;
+    int seg_start = 0;
+    while (chg) {
+      const int p = __builtin_ctzll(chg);
+      chg &= chg - 1;
+      const bool is_open = (O >> p) & 1ull;
+      const int b = is_open ? p : p + 1;
+      const uint64_t seg = low_mask(b) & ~low_mask(seg_start);
+      add_row<LMAX>(rows, depth, E & seg);
+      depth = (int)(int8_t)(uint8_t)(depth + (is_open ? 1 : -1));
+      seg_start = b;
+    }
+    add_row<LMAX>(rows, depth, E & ~low_mask(seg_start));
+  }
+  string_out[w] = S;
+#pragma unroll
+  for (int k = 0; k < LMAX; ++k)
+    if (k < levels) leveled_out[(uint64_t)k * words + w] = rows[k];
+
+  // 9. newline positions
+  if (tile == 0 && t == 0 && newline_cap > 0) newline_out[0] = 0;
+  uint64_t nlm = m.newline;
+  uint64_t o = nl_base;
+  while (nlm) {
+    const int b = __builtin_ctzll(nlm);
+    nlm &= nlm - 1;
+    if ((int64_t)o < newline_cap) newline_out[o] = (int64_t)(pos + (uint64_t)b);
+    ++o;
+  }
+}
+
+template __global__ void index_kernel<2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
+template __global__ void index_kernel<4>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
+template __global__ void index_kernel<8>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
+template __global__ void index_kernel<16>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
+template __global__ void index_kernel<22>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
+
+// ---- exact newline index (fallback when the capacity hint was too small) ---
+
+// Per-tile newline counts (256 words per block).
+__global__ __launch_bounds__(kTileWords) void newline_count_kernel(
+    const uint8_t *__restrict__ in, uint64_t n, int aligned,
+    uint64_t *__restrict__ tile_counts) {
+  __shared__ uint64_t s_sum[kTileWords / 64];
+  const uint64_t pos = ((uint64_t)blockIdx.x * kTileWords + threadIdx.x) * 64;
+  uint64_t c = 0;
+  if (pos < n) {
+    uint32_t x[16];
+    load_word(in, n, pos, aligned != 0, x);
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      c += __popc(eq_bytes(x[i] & 0x7F7F7F7Fu, '\n') & ~x[i] & 0x80808080u);
+  }
+  c = wave_reduce_add64(c);
+  if ((threadIdx.x & 63u) == 0) s_sum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t s = 0;
+    for (int k = 0; k < kTileWords / 64; ++k) s += s_sum[k];
+    tile_counts[blockIdx.x] = s;
+  }
+}
+
+// Exclusive scan of tile counts in one workgroup; writes the total line count.
+__global__ __launch_bounds__(1024) void newline_scan_kernel(
+    uint64_t *__restrict__ tile_counts, uint64_t tiles,
+    int64_t *__restrict__ num_lines_out) {
+  __shared__ uint64_t s_part[1024];
+  const uint32_t t = threadIdx.x;
+  const uint64_t per = (tiles + 1023) / 1024;
+  const uint64_t lo = t * per, hi = (lo + per < tiles) ? lo + per : tiles;
+  uint64_t s = 0;
+  for (uint64_t i = lo; i < hi; ++i) s += tile_counts[i];
+  s_part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    uint64_t run = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const uint64_t v = s_part[i];
+      s_part[i] = run;
+      run += v;
+    }
+    *num_lines_out = (int64_t)(1 + run);
+  }
+  __syncthreads();
+  uint64_t run = s_part[t];
+  for (uint64_t i = lo; i < hi; ++i) {
+    const uint64_t v = tile_counts[i];
+    tile_counts[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kTileWords) void newline_write_kernel(
+    const uint8_t *__restrict__ in, uint64_t n, int aligned,
+    const uint64_t *__restrict__ tile_offsets, int64_t *__restrict__ newline_out) {
+  __shared__ uint32_t s_wscan[kTileWords / 64];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  const uint64_t pos = ((uint64_t)blockIdx.x * kTileWords + t) * 64;
+  uint64_t nlm = 0;
+  if (pos < n) {
+    uint32_t x[16], f[16];
+    load_word(in, n, pos, aligned != 0, x);
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      f[i] = eq_bytes(x[i] & 0x7F7F7F7Fu, '\n') & ~x[i] & 0x80808080u;
+    nlm = pack16(f);
+  }
+  const uint32_t c = (uint32_t)__popcll(nlm);
+  const uint32_t incl = wave_incl_scan(c);
+  if (lane == 63) s_wscan[wv] = incl;
+  __syncthreads();
+  uint64_t o = incl - c;
+  for (uint32_t k = 0; k < wv; ++k) o += s_wscan[k];
+  o += 1 + tile_offsets[blockIdx.x];
+  if (blockIdx.x == 0 && t == 0) newline_out[0] = 0;
+  while (nlm) {
+    const int b = __builtin_ctzll(nlm);
+    nlm &= nlm - 1;
+    newline_out[o++] = (int64_t)(pos + (uint64_t)b);
+  }
+}
+
+// ---- debug: escape and quote bitmaps (true carry semantics) ----------------
+__global__ __launch_bounds__(256) void escape_quote_kernel(
+    const uint8_t *__restrict__ in, uint64_t n, uint64_t words,
+    uint64_t *__restrict__ escape_out, uint64_t *__restrict__ quote_out) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= words) return;
+  const uint64_t pos = w * 64;
+  uint32_t x[16];
+  load_word(in, n, pos, false, x);
+  const WordClasses m = classify(x);
+  // carry-in: parity of the backslash run ending right before this word
+  uint32_t cin = 0;
+  for (uint64_t p = pos; p > 0 && in[p - 1] == '\\'; --p) cin ^= 1u;
+  const uint64_t B = m.backslash;
+  const int lead = (B == ~0ull) ? 64 : __builtin_ctzll(~B);
+  uint64_t e = escaped_from_zero(B);
+  if (cin) e ^= mask_upto(lead);
+  const uint64_t valid = (pos + 64 <= n) ? ~0ull : low_mask((int)(n - pos));
+  escape_out[w] = e & valid;
+  quote_out[w] = m.quote & ~e & valid;
+}
+
+} // namespace gpj

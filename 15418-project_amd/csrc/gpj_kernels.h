@@ -1,0 +1,38 @@
+// gpj_kernels.h -- kernel declarations shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpj {
+
+constexpr int kTileWords = 256;              // words (64 B each) per index tile
+constexpr uint64_t kTileBytes = 64ull * kTileWords;
+
+__global__ void index_clear_kernel(uint64_t *desc, uint64_t count, uint32_t *counter);
+
+template <int LMAX>
+__global__ void index_kernel(const uint8_t *in, uint64_t n, uint64_t words,
+                             int levels, int aligned, uint64_t *string_out,
+                             uint64_t *leveled_out, int64_t *newline_out,
+                             int64_t newline_cap, int64_t *num_lines_out,
+                             uint64_t *desc, uint32_t *tile_counter,
+                             uint64_t num_tiles);
+
+__global__ void newline_count_kernel(const uint8_t *in, uint64_t n, int aligned,
+                                     uint64_t *tile_counts);
+__global__ void newline_scan_kernel(uint64_t *tile_counts, uint64_t tiles,
+                                    int64_t *num_lines_out);
+__global__ void newline_write_kernel(const uint8_t *in, uint64_t n, int aligned,
+                                     const uint64_t *tile_offsets,
+                                     int64_t *newline_out);
+__global__ void escape_quote_kernel(const uint8_t *in, uint64_t n, uint64_t words,
+                                    uint64_t *escape_out, uint64_t *quote_out);
+
+__global__ void query_kernel(const uint8_t *file, int64_t n,
+                             const int64_t *newline, const int64_t *num_lines,
+                             int64_t max_lines, const uint64_t *string,
+                             const uint64_t *leveled, int64_t level_size,
+                             int32_t num_levels, const uint8_t *ir, uint32_t ir_len,
+                             int32_t num_results, int64_t *result);
+
+} // namespace gpj

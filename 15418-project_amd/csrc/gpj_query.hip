@@ -1,0 +1,303 @@
+// gpj_query.hip -- JSONPath bytecode interpreter over the structural index.
+//
+// Semantics follow the reference query kernel
+// (src/gpjson/query/kernels/orig/query.cu:144-454): each line between two
+// newline-index entries is one record, trimmed to its first '{' and last '}';
+// the bytecode walks the leveled rows and the string mask; STORE_RESULT
+// writes {start, end} of the current value (spaces trimmed), -1 = no match.
+// Two deliberate differences, both outside what the reference defines:
+// bytes outside [0, n) read as 0 (the reference reads out of bounds there),
+// and an unknown opcode or running off the bytecode ends the record.
+// The bytecode comes in through a device pointer and is staged in LDS, so
+// concurrent launches with different queries are independent.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gpj_kernels.h"
+
+namespace gpj {
+
+namespace {
+
+enum : uint8_t {
+  kEnd = 0, kStoreResult = 1, kMoveUp = 2, kMoveDown = 3, kMoveToKey = 4,
+  kMoveToIndex = 5, kMoveToIndexReverse = 6, kMarkPosition = 7,
+  kResetPosition = 8, kExpressionStringEquals = 9
+};
+constexpr int kMaxLevels = 16;
+constexpr uint32_t kMaxIr = 4096;
+
+struct Ctx {
+  const uint8_t *file;
+  int64_t n;
+  const uint64_t *string;
+  const uint64_t *leveled;
+  int64_t level_size;
+  int num_levels;
+  const uint8_t *ir; // LDS copy
+  uint32_t ir_len;
+};
+
+__device__ __forceinline__ uint8_t byte_at(const Ctx &c, int64_t p) {
+  return (p >= 0 && p < c.n) ? c.file[p] : (uint8_t)0;
+}
+__device__ __forceinline__ uint8_t ir_at(const Ctx &c, int64_t p) {
+  return (p >= 0 && p < (int64_t)c.ir_len) ? c.ir[p] : (uint8_t)0;
+}
+__device__ __forceinline__ uint64_t through(int b) {
+  return b >= 63 ? ~0ull : ((1ull << (b + 1)) - 1);
+}
+
+// Last set bit of `bm` in [lo, idx], or -1.
+__device__ int64_t last_bit_in(const uint64_t *bm, int64_t lo, int64_t idx) {
+  if (idx < lo || idx < 0) return -1;
+  int64_t wi = idx >> 6, wb = wi << 6;
+  uint64_t word = bm[wi] & through((int)(idx - wb));
+  while (wb + 63 >= lo) {
+    uint64_t cand = word;
+    const int64_t first = lo - wb;
+    if (first > 0) cand &= ~through((int)(first - 1));
+    if (cand) return wb + 63 - __builtin_clzll(cand);
+    if (wi == 0) break;
+    --wi;
+    wb -= 64;
+    if (wb + 63 < lo) break;
+    word = bm[wi];
+  }
+  return -1;
+}
+
+// First structural bit of `level` in [idx, level_end], else level_end.
+__device__ int64_t next_structural(const Ctx &c, int64_t level_end, int64_t idx,
+                                   int level) {
+  if (idx >= level_end) return level_end;
+  if (level < 0 || level >= c.num_levels) return level_end;
+  if (idx < 0) idx = 0;
+  const uint64_t *row = c.leveled + (uint64_t)c.level_size * (uint64_t)level;
+  int64_t wi = idx >> 6, wb = wi << 6;
+  uint64_t word = row[wi] & (~0ull << (idx - wb));
+  while (wb <= level_end) {
+    uint64_t cand = word;
+    const int64_t last = level_end - wb;
+    if (last < 63) cand &= through((int)last);
+    if (cand) return wb + __builtin_ctzll(cand);
+    ++wi;
+    wb += 64;
+    if (wb > level_end || wi >= c.level_size) break;
+    word = row[wi];
+  }
+  return level_end;
+}
+
+__device__ int64_t prev_structural(const Ctx &c, int64_t level_start,
+                                   int64_t idx, int level) {
+  if (idx <= level_start) return level_start;
+  if (level < 0 || level >= c.num_levels) return level_start;
+  const uint64_t *row = c.leveled + (uint64_t)c.level_size * (uint64_t)level;
+  const int64_t s = last_bit_in(row, level_start, idx);
+  return s == -1 ? level_start : s;
+}
+
+__device__ __forceinline__ int32_t read_varint(const Ctx &c, int64_t &qp) {
+  uint32_t value = 0, byte = 0;
+  int shift = 0;
+  for (;;) {
+    byte = ir_at(c, qp++);
+    if (!(byte & 0x80u)) break;
+    value |= (byte & 0x7Fu) << shift;
+    shift += 7;
+  }
+  return (int32_t)(value | (byte << shift));
+}
+
+__device__ void run_record(const Ctx &c, int64_t line_start, int64_t line_end,
+                           int32_t num_results, int64_t *res) {
+  while (line_end > line_start && byte_at(c, line_end) != '}') --line_end;
+  while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
+  if (line_start == line_end) return;
+
+  int64_t idx = line_start;
+  int lvl = 0;
+  int64_t qp = 0;
+  int64_t level_end[kMaxLevels];
+  int64_t curr_index[kMaxLevels];
+  int64_t marked_pos[kMaxLevels];
+  int marked_lvl[kMaxLevels];
+  level_end[0] = line_end;
+#pragma unroll
+  for (int i = 1; i < kMaxLevels; ++i) level_end[i] = -1;
+#pragma unroll
+  for (int i = 0; i < kMaxLevels; ++i) curr_index[i] = -1;
+  int mi = -1;
+  int nres = 0;
+
+  for (;;) {
+    if (qp >= (int64_t)c.ir_len) return;
+    const uint8_t op = c.ir[qp++];
+    switch (op) {
+    case kEnd:
+      return;
+    case kStoreResult: {
+      if (nres >= num_results) return;
+      int64_t end_str = level_end[lvl];
+      while (byte_at(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
+      while (end_str > idx && byte_at(c, end_str - 1) == ' ') --end_str;
+      res[nres * 2] = idx;
+      res[nres * 2 + 1] = end_str;
+      ++nres;
+      break;
+    }
+    case kMoveUp:
+      if (lvl <= 0) return;
+      idx = level_end[lvl];
+      level_end[lvl] = -1;
+      --lvl;
+      break;
+    case kMoveDown:
+      if (lvl + 1 >= kMaxLevels) return;
+      ++lvl;
+      if (level_end[lvl] == -1) {
+        level_end[lvl] = next_structural(c, level_end[lvl - 1], idx + 1, lvl - 1);
+        while (byte_at(c, idx) == ' ') ++idx;
+      }
+      break;
+    case kMoveToKey: {
+      const int32_t key_len = read_varint(c, qp);
+      const int64_t key_pos = qp;
+      qp += key_len;
+      if (byte_at(c, idx) != '{') return;
+      for (;;) {
+        ++idx;
+        idx = next_structural(c, level_end[lvl], idx, lvl);
+        if (byte_at(c, idx) != ':') break;
+        const int64_t string_end = last_bit_in(c.string, line_start + 1, idx - 1);
+        const int64_t string_start = string_end - key_len;
+        bool match = string_start >= line_start && byte_at(c, string_start) == '"';
+        for (int32_t i = 0; match && i < key_len; ++i)
+          match = ir_at(c, key_pos + i) == byte_at(c, string_start + i + 1);
+        if (match) break;
+        ++idx;
+        idx = next_structural(c, level_end[lvl], idx, lvl);
+        if (byte_at(c, idx) == '}') return;
+      }
+      ++idx;
+      break;
+    }
+    case kMoveToIndex: {
+      const int32_t target = read_varint(c, qp);
+      const uint8_t ch = byte_at(c, idx);
+      if (ch != '[' && ch != ',' && ch != ']') return;
+      if (ch == '[') curr_index[lvl] = 0;
+      else ++curr_index[lvl];
+      for (;;) {
+        if (curr_index[lvl] < target) {
+          ++idx;
+          idx = next_structural(c, level_end[lvl], idx, lvl);
+          if (byte_at(c, idx) == ',') { ++curr_index[lvl]; continue; }
+          return;
+        }
+        if (curr_index[lvl] > target) {
+          --idx;
+          idx = prev_structural(c, 0, idx, lvl);
+          --curr_index[lvl];
+          continue;
+        }
+        break;
+      }
+      if (byte_at(c, idx) == ']') return;
+      ++idx;
+      break;
+    }
+    case kMoveToIndexReverse: {
+      const int32_t target = read_varint(c, qp);
+      idx = level_end[lvl] - 1;
+      while (idx >= 0 && byte_at(c, idx) == ' ') --idx;
+      const uint8_t ch = byte_at(c, idx);
+      if (ch != ']' && ch != ',' && ch != '[') return;
+      if (ch == ']') curr_index[lvl] = 0;
+      else ++curr_index[lvl];
+      for (;;) {
+        if (curr_index[lvl] < target) {
+          --idx;
+          idx = prev_structural(c, 0, idx, lvl);
+          if (byte_at(c, idx) == ',' || curr_index[lvl] + 1 == target) {
+            ++curr_index[lvl];
+            continue;
+          }
+          return;
+        }
+        if (curr_index[lvl] > target) {
+          ++idx;
+          idx = next_structural(c, level_end[lvl], idx, lvl);
+          --curr_index[lvl];
+          continue;
+        }
+        break;
+      }
+      ++idx;
+      break;
+    }
+    case kMarkPosition:
+      if (mi + 1 >= kMaxLevels) return;
+      ++mi;
+      marked_pos[mi] = idx;
+      marked_lvl[mi] = lvl;
+      break;
+    case kResetPosition:
+      if (mi < 0) return;
+      idx = marked_pos[mi];
+      lvl = marked_lvl[mi];
+      --mi;
+      for (int i = lvl + 1; i < kMaxLevels; ++i) {
+        if (level_end[i] == -1) break;
+        level_end[i] = -1;
+      }
+      break;
+    case kExpressionStringEquals: {
+      const int32_t key_len = read_varint(c, qp);
+      const int64_t key_pos = qp;
+      qp += key_len;
+      while (byte_at(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
+      int64_t string_end = level_end[lvl] - 1;
+      while (idx < string_end && byte_at(c, string_end) == ' ') --string_end;
+      if (string_end - idx + 1 != key_len) return;
+      for (int32_t i = 0; i < key_len; ++i)
+        if (ir_at(c, key_pos + i) != byte_at(c, idx + i)) return;
+      break;
+    }
+    default:
+      return;
+    }
+  }
+}
+
+} // namespace
+
+__global__ __launch_bounds__(256) void query_kernel(
+    const uint8_t *__restrict__ file, int64_t n,
+    const int64_t *__restrict__ newline, const int64_t *__restrict__ num_lines,
+    int64_t max_lines, const uint64_t *__restrict__ string,
+    const uint64_t *__restrict__ leveled, int64_t level_size, int32_t num_levels,
+    const uint8_t *__restrict__ ir, uint32_t ir_len, int32_t num_results,
+    int64_t *__restrict__ result) {
+  __shared__ uint8_t s_ir[kMaxIr];
+  const uint32_t len = ir_len < kMaxIr ? ir_len : kMaxIr;
+  for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) s_ir[i] = ir[i];
+  __syncthreads();
+  const Ctx c{file, n, string, leveled, level_size, num_levels, s_ir, len};
+  const int64_t total = *num_lines;
+  const int64_t lines = total < max_lines ? total : max_lines;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; line < lines;
+       line += stride) {
+    int64_t *res = result + line * 2 * (int64_t)num_results;
+    for (int i = 0; i < 2 * num_results; ++i) res[i] = -1;
+    const int64_t line_start = newline[line];
+    int64_t line_end = (line + 1 < total) ? newline[line + 1] : n - 1;
+    if (line_start < 0 || line_start >= n || line_end < 0) continue;
+    if (line_end >= n) line_end = n - 1;
+    run_record(c, line_start, line_end, num_results, res);
+  }
+}
+
+} // namespace gpj

@@ -1,0 +1,142 @@
+/*
+ * gpj_capi.h -- C ABI of the MI355X GpJSON engine (libgpj_hip.so).
+ *
+ * Plain pointers and sizes, no C++ types, no exceptions across the boundary.
+ * Every entry point returns a gpj_status; on error gpj_last_error() returns a
+ * thread-local message.  Device pointers are HIP device allocations; `stream`
+ * is a hipStream_t (NULL = the null stream).  All work is stream-ordered.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef GPJ_CAPI_H
+#define GPJ_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t gpj_status;
+
+enum {
+  GPJ_OK = 0,
+  GPJ_ERR_INVALID_ARGUMENT = -1, /* ImplementationError */
+  GPJ_ERR_DEVICE = -2,           /* CudaError (src/gpjson/cuda/cuda.cpp:19-29) */
+  GPJ_ERR_OUT_OF_MEMORY = -3,    /* CudaError from cudaMalloc */
+  GPJ_ERR_QUERY_EXECUTION = -4,  /* QueryExecutionError (include/gpjson/query/error.hpp:17-20) */
+  GPJ_ERR_MALFORMED_QUERY = -5,  /* MalformedQueryError (include/gpjson/query/error.hpp:7-10) */
+  GPJ_ERR_UNSUPPORTED_QUERY = -6,/* UnsupportedQueryError (include/gpjson/query/error.hpp:12-15) */
+  GPJ_ERR_CAPACITY = -7,         /* a caller-provided buffer is too small */
+  GPJ_ERR_OUT_OF_RANGE = -8      /* std::out_of_range (an index literal overflows) */
+};
+
+/* Limits the query path shares with the reference
+ * (include/gpjson/query/kernels/orig.hpp:9-10, leveled-bitmaps-index.cu:6). */
+#define GPJ_MAX_QUERY_LEVELS 16
+#define GPJ_MAX_QUERY_IR_BYTES 4096
+#define GPJ_MAX_INDEX_LEVELS 22
+
+/* Thread-local text of the last error on this thread ("" if none). */
+const char *gpj_last_error(void);
+
+/* ---- device runtime: replaces DeviceArray / check / device_available
+ *      (include/gpjson/cuda/cuda.hpp:10-45, src/gpjson/cuda/cuda.cpp:33-125) */
+int gpj_device_available(void);
+gpj_status gpj_alloc(void **d_ptr, uint64_t bytes);
+gpj_status gpj_free(void *d_ptr);
+gpj_status gpj_h2d(void *d_dst, const void *h_src, uint64_t bytes, void *stream);
+gpj_status gpj_d2h(void *h_dst, const void *d_src, uint64_t bytes, void *stream);
+gpj_status gpj_stream_sync(void *stream);
+gpj_status gpj_device_sync(void);
+
+/* ---- structural index: replaces IndexBuilder::build
+ *      (include/gpjson/index/index_builder.hpp:55-63; kernels of
+ *      src/gpjson/index/orig_index_builder.cu:224-395) */
+
+typedef struct {
+  /* Launch geometry of the reference options struct
+   * (include/gpjson/index/index_builder.hpp:34-42).  Accepted and recorded;
+   * results never depend on them. */
+  int32_t grid_size;
+  int32_t block_size;
+  int32_t reduction_grid_size;
+  int32_t reduction_block_size;
+  /* Newline-index capacity hint in entries; 0 = default (n/64 + 2).  When the
+   * input holds more lines, the build falls back to an exact second pass. */
+  int64_t newline_capacity;
+} gpj_index_opts;
+
+/* Device-resident indices of one partition (include/gpjson/index/index.hpp:83-109). */
+typedef struct {
+  int64_t *d_newline;   /* int64[num_lines]: 0 then every '\n' offset */
+  int64_t *d_num_lines; /* device copy of num_lines (int64) */
+  int64_t num_lines;    /* 1 + count('\n') */
+  uint64_t *d_string;   /* uint64[level_size] string mask */
+  uint64_t *d_leveled;  /* uint64[num_levels][level_size], level-major */
+  int64_t level_size;   /* ceil(n / 64) */
+  int32_t num_levels;
+  int32_t owns_memory;  /* set by gpj_index_build; gpj_index_free releases */
+} gpj_index;
+
+/* Allocate and build all three indices for n bytes at d_in.  Synchronizes
+ * `stream` once to learn num_lines, as the reference does
+ * (orig_index_builder.cu:325-330). */
+gpj_status gpj_index_build(const uint8_t *d_in, uint64_t n, int32_t levels,
+                           const gpj_index_opts *opts, gpj_index *out,
+                           void *stream);
+gpj_status gpj_index_free(gpj_index *index);
+
+/* Bytes of device scratch gpj_index_build_into needs for n input bytes. */
+uint64_t gpj_index_scratch_bytes(uint64_t n);
+
+/* Stream-ordered build into caller-owned buffers; allocates nothing and never
+ * synchronizes.  d_string: level_size words, d_leveled: levels*level_size
+ * words, d_newline: newline_capacity entries, d_num_lines: 1 int64, d_scratch:
+ * gpj_index_scratch_bytes(n) bytes.  If the input has more than
+ * newline_capacity lines, *d_num_lines still receives the true count and only
+ * the first newline_capacity entries are written; the caller checks. */
+gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
+                                uint64_t *d_string, uint64_t *d_leveled,
+                                int64_t *d_newline, int64_t newline_capacity,
+                                int64_t *d_num_lines, void *d_scratch,
+                                void *stream);
+
+/* Debug: escape and unescaped-quote bitmaps (reference kernels
+ * src/gpjson/index/kernels/orig/uncombined/escape-index.cu and
+ * src/gpjson/index/kernels/orig/quote-index.cu), ceil(n/64) words each. */
+gpj_status gpj_debug_escape_quote(const uint8_t *d_in, uint64_t n,
+                                  uint64_t *d_escape, uint64_t *d_quote,
+                                  void *stream);
+
+/* ---- query: replaces the query kernel launch of QueryExecutor::execute_batch
+ *      (src/gpjson/query/kernels/orig/query_executor_host.cu:179-237,
+ *      src/gpjson/query/kernels/orig/query.cu:144-454) */
+
+/* Run one compiled query (IR already in device memory at d_ir, ir_len bytes)
+ * over every line of the index.  d_spans receives int64
+ * [max_lines][num_results][2] partition-local {start, end} pairs, -1 for no
+ * match; lines beyond *index->d_num_lines are not touched.  max_lines bounds
+ * the grid (use index->num_lines when known).  Stream-ordered; allocates
+ * nothing. */
+gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                         const uint8_t *d_ir, uint32_t ir_len,
+                         int32_t num_results, int64_t max_lines,
+                         int64_t *d_spans, void *stream);
+
+/* ---- query compiler: replaces QueryCompiler::compile
+ *      (src/gpjson/query/query_compiler.cpp:467-477).  Host only.
+ * Writes the bytecode into ir (capacity ir_cap) and its length, the query's
+ * max depth and its number of result slots.  flags: GPJ_COMPILE_EXTENSIONS
+ * enables the `[*]` wildcard, which the reference rejects. */
+#define GPJ_COMPILE_EXTENSIONS 1u
+gpj_status gpj_query_compile(const char *query, uint8_t *ir, uint32_t ir_cap,
+                             uint32_t *ir_len, uint32_t *max_depth,
+                             uint32_t *num_slots, uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPJ_CAPI_H */

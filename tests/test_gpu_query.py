@@ -1,0 +1,94 @@
+"""GPU parity of the bytecode query kernel: every result span equals the
+CPU oracle's restatement of the reference interpreter (query.cu:144-454)."""
+import numpy as np
+import pytest
+
+import jsongen
+
+pytestmark = pytest.mark.gpu
+
+BOOKSTORE = (b'{"store":{"book":[{"category":"reference","author":"Nigel Rees","title":'
+             b'"Sayings of the Century","price":"8.95"},{"category":"fiction","author":'
+             b'"Herman Melville","title":"Moby Dick","isbn":"0-553-21311-3","price":"8.99"},'
+             b'{"category":"fiction","author":"J.R.R. Tolkien","title":"The Lord of the Rings",'
+             b'"isbn":"0-395-19395-8","price":"22.99"}],"bicycle":{"color":"red","price":'
+             b'"19.95"}},"expensive":"10"}')
+
+QUERIES = [
+    "$.a", "$.b", "$.a.b", "$.a.c", "$.id", "$.name", "$.tags[0]", "$.tags[1]", "$.tags[-1]",
+    "$.items[2]", "$.a[0:2]", "$.a[1,3]", "$.arr[:3]", "$.a[-2:]", "$.a[-1]", "$['a']['b']",
+    "$.a[?(@ == \"x\")]", "$.h[?(@.k == \"v\")].id", "$.user.lang", "$.e[0].f", "$.x.y.a.b",
+    "$.nested.arr[1].x", "$.k[0][1]", "$.a.b.c.d", "$", "$.ab", "$.abc",
+]
+
+
+def run_both(gpj, orc, data: bytes, query: str, levels=None):
+    ir, depth, slots = gpj.compile_query(query)
+    lv_count = max(depth, 1) if levels is None else levels
+    dev = gpj.DeviceBuffer.from_host(np.frombuffer(data, dtype=np.uint8))
+    idx = gpj.build_index(dev, len(data), lv_count)
+    got = gpj.run_query(dev, len(data), idx, ir, slots)
+    want = orc.query(data, orc.build_index(data, lv_count), ir, slots)
+    return got, want
+
+
+def texts(data: bytes, spans):
+    out = []
+    for line in spans:
+        vals = [data[a:b] for a, b in line if a >= 0 and b >= 0]
+        if vals:
+            out.append(vals)
+    return out
+
+
+def test_bookstore_kats(gpu, orc):
+    # tests/query/query_executor_test.cpp:155-245
+    got, want = run_both(gpu, orc, BOOKSTORE, "$.expensive")
+    np.testing.assert_array_equal(got, want)
+    assert texts(BOOKSTORE, got) == [[b'"10"']]
+    got, want = run_both(gpu, orc, BOOKSTORE, "$.store.book[1].author")
+    np.testing.assert_array_equal(got, want)
+    assert texts(BOOKSTORE, got) == [[b'"Herman Melville"']]
+    got, want = run_both(gpu, orc, BOOKSTORE, "$.store.magazine")
+    np.testing.assert_array_equal(got, want)
+    assert texts(BOOKSTORE, got) == []
+
+
+def test_engine_kats(gpu, orc):
+    # tests/engine_test.cpp:95-128 (single partition form)
+    data = b'{"name":"Ada","scores":[1,2],"lang":"en"}\n{"name":"Bob","scores":[3,4],"lang":"fr"}\n'
+    got, want = run_both(gpu, orc, data, "$.name")
+    np.testing.assert_array_equal(got, want)
+    assert texts(data, got) == [[b'"Ada"'], [b'"Bob"']]
+    got, want = run_both(gpu, orc, data, "$.scores[1]")
+    np.testing.assert_array_equal(got, want)
+    assert texts(data, got) == [[b"2"], [b"4"]]
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_random_records_all_queries(gpu, orc, seed):
+    data = jsongen.ndjson(seed, 1 << 20, max_depth=6, spaces=(seed == 12))
+    for q in QUERIES:
+        print(f"[query] seed={seed} q={q}", flush=True)
+        got, want = run_both(gpu, orc, data, q)
+        if not np.array_equal(got, want):
+            bad = np.argwhere((got != want).any(axis=(1, 2)))[:5].ravel().tolist()
+            raise AssertionError(f"{q}: lines {bad} differ: got {got[bad].tolist()} "
+                                 f"want {want[bad].tolist()}")
+
+
+def test_extra_levels_do_not_change_results(gpu, orc):
+    data = jsongen.ndjson(13, 200_000, max_depth=6)
+    for q in ("$.a.b", "$.tags[1]"):
+        base, _ = run_both(gpu, orc, data, q)
+        more, want = run_both(gpu, orc, data, q, levels=12)
+        np.testing.assert_array_equal(more, want)
+        np.testing.assert_array_equal(base, more)
+
+
+def test_byte_soup_queries_do_not_fault(gpu, orc):
+    # arbitrary bytes: spans must still match the oracle's guarded semantics
+    data = jsongen.byte_soup(21, 300_000, special_weight=0.3)
+    for q in ("$.a", "$.a[2]", "$.a[-1]", "$.a.b[?(@ == \"x\")]"):
+        got, want = run_both(gpu, orc, data, q)
+        np.testing.assert_array_equal(got, want)
