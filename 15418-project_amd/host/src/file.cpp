@@ -1,0 +1,141 @@
+// File readers and partitioning (reference: src/gpjson/file/file_reader.cpp,
+// src/gpjson/file/file.cpp).
+#include "gpjson/file/error.hpp"
+#include "gpjson/file/file.hpp"
+#include "gpjson/file/file_reader.hpp"
+
+#include <cerrno>
+#include <cstring>
+#include <fcntl.h>
+#include <filesystem>
+#include <fstream>
+#include <sstream>
+#include <sys/mman.h>
+#include <unistd.h>
+
+namespace gpjson::file {
+
+FilePartition::FilePartition(size_t partition_id, size_t global_start_offset,
+                             size_t global_end_offset, const void *data)
+    : partition_id_(partition_id), global_start_offset_(global_start_offset),
+      global_end_offset_(global_end_offset), host_data_(data) {}
+
+bool FilePartition::device_loaded() const {
+  return size_bytes() == 0 || device_data_.data() != nullptr;
+}
+
+void FilePartition::load_to_device() {
+  if (device_loaded()) return;
+  device::DeviceArray d(size_bytes());
+  d.copy_from_host(host_data_, size_bytes());
+  device_data_ = std::move(d);
+}
+
+std::vector<FilePartition> partition_bytes(const std::string &file_path,
+                                           const std::byte *base, size_t size,
+                                           size_t partition_size_bytes) {
+  std::vector<FilePartition> parts;
+  if (partition_size_bytes == 0 || size == 0 || size <= partition_size_bytes) {
+    parts.emplace_back(0, 0, size, base);
+    return parts;
+  }
+  size_t start = 0, id = 0;
+  while (size - start > partition_size_bytes) {
+    // scan back from start+size for the last newline of this partition
+    const size_t target = start + partition_size_bytes;
+    size_t cut = target;
+    bool found = false;
+    for (;;) {
+      if (base[cut] == std::byte{'\n'}) {
+        found = true;
+        break;
+      }
+      if (cut == start) break;
+      --cut;
+    }
+    if (!found) {
+      std::ostringstream m;
+      m << "Cannot partition file '" << file_path << "' using partition size "
+        << partition_size_bytes << ": no newline found before offset " << target << ".";
+      throw error::file::PartitionError(m.str());
+    }
+    const size_t next = cut + 1;
+    if (next >= size) break;
+    parts.emplace_back(id++, start, next - 1, base + start);
+    start = next;
+  }
+  parts.emplace_back(id, start, size, base + start);
+  return parts;
+}
+
+namespace {
+size_t file_size_or_throw(const std::string &path) {
+  std::error_code ec;
+  const auto sz = std::filesystem::file_size(std::filesystem::path(path), ec);
+  if (ec)
+    throw error::file::FileOpenError("Failed to get size for file '" + path + "': " + ec.message());
+  return static_cast<size_t>(sz);
+}
+} // namespace
+
+CopiedFileReader::CopiedFileReader(std::string file_path) : file_path_(std::move(file_path)) {
+  metadata_.file_path = file_path_;
+}
+
+void CopiedFileReader::create_partitions(size_t partition_size_bytes) {
+  metadata_.file_size_bytes = file_size_or_throw(file_path_);
+  bytes_.assign(metadata_.file_size_bytes, std::byte{0});
+  std::ifstream in(file_path_, std::ios::binary);
+  if (!in) throw error::file::FileOpenError("Failed to open file '" + file_path_ + "' for reading.");
+  if (metadata_.file_size_bytes > 0) {
+    in.read(reinterpret_cast<char *>(bytes_.data()),
+            static_cast<std::streamsize>(metadata_.file_size_bytes));
+    if (!in || static_cast<size_t>(in.gcount()) != metadata_.file_size_bytes)
+      throw error::file::FileReadError("Failed to read all bytes from file '" + file_path_ + "'.");
+  }
+  partitions_ = partition_bytes(file_path_, bytes_.empty() ? nullptr : bytes_.data(),
+                                metadata_.file_size_bytes, partition_size_bytes);
+  metadata_.num_partitions = partitions_.size();
+}
+
+MmapFileReader::MmapFileReader(std::string file_path) : file_path_(std::move(file_path)) {
+  metadata_.file_path = file_path_;
+}
+
+MmapFileReader::~MmapFileReader() { unmap(); }
+
+void MmapFileReader::map() {
+  if (fd_ != -1 || mapped_size_ != 0) return;
+  metadata_.file_size_bytes = file_size_or_throw(file_path_);
+  fd_ = open(file_path_.c_str(), O_RDONLY);
+  if (fd_ == -1)
+    throw error::file::FileOpenError("Failed to open file '" + file_path_ +
+                                     "' for reading: " + std::strerror(errno) + ".");
+  if (metadata_.file_size_bytes == 0) return;
+  void *m = mmap(nullptr, metadata_.file_size_bytes, PROT_READ, MAP_PRIVATE, fd_, 0);
+  if (m == MAP_FAILED) {
+    const std::string e = std::strerror(errno);
+    unmap();
+    throw error::file::FileReadError("Failed to mmap file '" + file_path_ + "': " + e + ".");
+  }
+  mapped_ = static_cast<const std::byte *>(m);
+  mapped_size_ = metadata_.file_size_bytes;
+}
+
+void MmapFileReader::unmap() noexcept {
+  if (mapped_ && mapped_size_) munmap(const_cast<std::byte *>(mapped_), mapped_size_);
+  mapped_ = nullptr;
+  mapped_size_ = 0;
+  if (fd_ != -1) {
+    close(fd_);
+    fd_ = -1;
+  }
+}
+
+void MmapFileReader::create_partitions(size_t partition_size_bytes) {
+  map();
+  partitions_ = partition_bytes(file_path_, mapped_, metadata_.file_size_bytes, partition_size_bytes);
+  metadata_.num_partitions = partitions_.size();
+}
+
+} // namespace gpjson::file

@@ -1,0 +1,102 @@
+// QueryExecutor::execute_batch over gpj_query_run
+// (reference: src/gpjson/query/kernels/orig/query_executor_host.cu:80-241).
+#include "gpjson/query/query_executor.hpp"
+
+#include "gpj_capi.h"
+#include "gpjson/engine.hpp"
+#include "gpjson/query/error.hpp"
+
+#include <string>
+#include <vector>
+
+namespace gpjson::query {
+
+namespace {
+
+void validate(const CompiledQuery &q, const index::BuiltIndices &idx) {
+  if (q.max_depth() > static_cast<size_t>(idx.get_leveled_bitmap_index().num_levels()))
+    throw error::query::QueryExecutionError(
+        "Query requires more bitmap levels than the index provides");
+  if (q.max_depth() >= static_cast<size_t>(kMaxQueryLevels))
+    throw error::query::QueryExecutionError("Query exceeds the maximum executor depth");
+  if (q.ir_bytes().size() > kMaxQueryIrBytes)
+    throw error::query::QueryExecutionError("Compiled query IR exceeds the 4096-byte IR buffer");
+}
+
+} // namespace
+
+QueryExecutor::QueryExecutor(const gpjson::EngineOptions &options)
+    : grid_size_(options.query_executor_options.grid_size),
+      block_size_(options.query_executor_options.block_size) {}
+
+BatchQueryResult QueryExecutor::execute_batch(const BatchCompiledQuery &compiled,
+                                              const file::FilePartition &partition,
+                                              const index::BuiltIndices &idx) const {
+  BatchQueryResult batch(compiled.size());
+  for (size_t q = 0; q < compiled.size(); ++q)
+    batch.set_query_text(q, compiled.queries()[q].query_text());
+  if (compiled.size() == 0 || partition.size_bytes() == 0 || partition.host_bytes() == nullptr)
+    return batch;
+  if (partition.device_bytes() == nullptr)
+    throw error::query::QueryExecutionError(
+        "Query execution requires a GPU-resident partition buffer");
+  if (!device::device_available())
+    throw error::query::QueryExecutionError("HIP device unavailable for query execution");
+  const int64_t num_lines = idx.get_newline_index().num_lines();
+  if (num_lines <= 0) return batch;
+  if (!idx.get_newline_index().data() || !idx.get_string_index().data() ||
+      !idx.get_leveled_bitmap_index().data())
+    throw error::query::QueryExecutionError(
+        "Query execution requires GPU-resident newline, string, and bitmap indices");
+
+  gpj_index view{};
+  view.d_newline = const_cast<int64_t *>(static_cast<const int64_t *>(idx.get_newline_index().data()));
+  view.d_num_lines = const_cast<int64_t *>(idx.get_newline_index().device_num_lines());
+  view.num_lines = num_lines;
+  view.d_string = const_cast<uint64_t *>(static_cast<const uint64_t *>(idx.get_string_index().data()));
+  view.d_leveled = const_cast<uint64_t *>(
+      static_cast<const uint64_t *>(idx.get_leveled_bitmap_index().data()));
+  view.level_size = static_cast<int64_t>((partition.size_bytes() + 63) / 64);
+  view.num_levels = idx.get_leveled_bitmap_index().num_levels();
+
+  const char *host = static_cast<const char *>(partition.host_bytes());
+  const size_t base = partition.global_start_offset();
+  for (size_t q = 0; q < compiled.size(); ++q) {
+    const CompiledQuery &cq = compiled.queries()[q];
+    validate(cq, idx);
+    const int slots = static_cast<int>(cq.num_result_slots());
+    device::DeviceArray d_ir(cq.ir_bytes().size());
+    d_ir.copy_from_host(cq.ir_bytes().data(), cq.ir_bytes().size());
+    const size_t count = static_cast<size_t>(num_lines) * slots * 2;
+    device::DeviceArray d_spans(count * sizeof(int64_t));
+    device::check(gpj_query_run(static_cast<const uint8_t *>(partition.device_bytes()),
+                                partition.size_bytes(), &view, d_ir.as<uint8_t>(),
+                                static_cast<uint32_t>(cq.ir_bytes().size()), slots, num_lines,
+                                d_spans.as<int64_t>(), nullptr),
+                  "execute_query_kernel launch");
+    std::vector<int64_t> spans(count, -1);
+    d_spans.copy_to_host(spans.data(), count * sizeof(int64_t));
+    for (int64_t line = 0; line < num_lines; ++line) {
+      LineQueryResult lr;
+      for (int s = 0; s < slots; ++s) {
+        const int64_t a = spans[(static_cast<size_t>(line) * slots + s) * 2];
+        const int64_t b = spans[(static_cast<size_t>(line) * slots + s) * 2 + 1];
+        if (a < 0 || b < 0) {
+          lr.add_offset(QueryOffset{});
+          continue;
+        }
+        if (a > b || static_cast<size_t>(b) > partition.size_bytes())
+          throw error::query::QueryExecutionError("Query kernel produced an invalid result range");
+        QueryOffset off;
+        off.start = base + static_cast<size_t>(a);
+        off.end = base + static_cast<size_t>(b);
+        off.json_text = std::string(host + a, static_cast<size_t>(b - a));
+        lr.add_offset(std::move(off));
+      }
+      batch.add_line_result(q, std::move(lr));
+    }
+  }
+  return batch;
+}
+
+} // namespace gpjson::query

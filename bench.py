@@ -1,0 +1,246 @@
+"""Benchmark: JSON GB/s indexed + queried on one MI355X.
+
+One step = build the structural index of the whole input (string mask,
+leveled rows, newline index) and run every query of the workload over it, all
+stream-ordered on one HIP stream, with the input already resident in HBM.
+
+Default workload (the throughput configuration): 8 GiB of seeded synthetic
+NDJSON, ~1 KiB records of depth <= 4, four queries over one shared 3-level
+index: $.user.lang, $.e[0].f, $.h[*].t and $.h[?(@.k == "v")].id.
+
+Prints one JSON line (see the keys at the end of main()).  Timing uses HIP
+events recorded on the stream the kernels run on; the dominant kernel's
+average launch time gives the roofline figure.  The CPU baseline is the
+multi-threaded CPU oracle (a port of the reference algorithm) on a bounded
+prefix sample of the same data, rank 0 only.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__ as ge  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "index_kernel_traffic.json")
+
+WORKLOADS = {
+    # name: (schema, bytes, seed, queries, extension queries allowed)
+    "ndjson_8GiB_4q": (2, 8 << 30, 0x15418002,
+                       ["$.user.lang", "$.e[0].f", "$.h[*].t", '$.h[?(@.k == "v")].id']),
+    "ndjson_1GiB_4q": (2, 1 << 30, 0x15418002,
+                       ["$.user.lang", "$.e[0].f", "$.h[*].t", '$.h[?(@.k == "v")].id']),
+    "ndjson_64MiB_ab": (0, 64 << 20, 0x15418000, ["$.a.b"]),
+}
+
+
+class Hip:
+    """The few HIP runtime calls the timer needs (events + one stream)."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                               ctypes.c_void_p]
+
+    def _ok(self, st, what):
+        if st != 0:
+            raise RuntimeError(f"{what} failed with HIP error {st}")
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        self._ok(self.L.hipStreamCreate(ctypes.byref(s)), "hipStreamCreate")
+        return s.value
+
+    def event(self):
+        e = ctypes.c_void_p()
+        self._ok(self.L.hipEventCreate(ctypes.byref(e)), "hipEventCreate")
+        return e.value
+
+    def record(self, e, s):
+        self._ok(self.L.hipEventRecord(ctypes.c_void_p(e), ctypes.c_void_p(s)), "hipEventRecord")
+
+    def sync_event(self, e):
+        self._ok(self.L.hipEventSynchronize(ctypes.c_void_p(e)), "hipEventSynchronize")
+
+    def elapsed_ms(self, a, b) -> float:
+        ms = ctypes.c_float()
+        self._ok(self.L.hipEventElapsedTime(ctypes.byref(ms), ctypes.c_void_p(a),
+                                            ctypes.c_void_p(b)), "hipEventElapsedTime")
+        return float(ms.value)
+
+
+def cut_at_newline(data: np.ndarray, limit: int) -> np.ndarray:
+    if data.size <= limit:
+        return data
+    nl = np.flatnonzero(data[:limit] == 10)
+    return data[: int(nl[-1]) + 1] if nl.size else data[:limit]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="ndjson_8GiB_4q", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-sample-mib", type=int, default=512)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1 "
+              f"(got {args.gpus}).  The sharding design is described in DESIGN.md.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    gpj = ge.load_package()
+    gpj.lib()
+    if not gpj.device_available():
+        raise SystemExit("bench.py: no HIP device")
+    hip = Hip()
+    schema, nbytes, seed, queries = WORKLOADS[args.workload]
+
+    t0 = time.time()
+    data = gpj.generate(schema, nbytes, seed)
+    gen_s = time.time() - t0
+    n = int(data.size)
+    print(f"[bench] generated {n} bytes in {gen_s:.1f}s", file=sys.stderr, flush=True)
+
+    compiled = [gpj.compile_query(q, extensions=True) for q in queries]
+    levels = max(1, max(c[1] for c in compiled))
+    words = (n + 63) // 64
+    nl_cap = n // 256 + 2
+
+    stream = hip.stream()
+    d_in = gpj.DeviceBuffer(n)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    gpj.check(gpj.lib().gpj_h2d(d_in.ptr, data.ctypes.data, n, stream))
+    hip.record(e1, stream)
+    hip.sync_event(e1)
+    h2d_gbs = n / (hip.elapsed_ms(e0, e1) * 1e-3) / 1e9
+
+    d_string = gpj.DeviceBuffer(words * 8)
+    d_leveled = gpj.DeviceBuffer(words * 8 * levels)
+    d_newline = gpj.DeviceBuffer(nl_cap * 8)
+    d_numlines = gpj.DeviceBuffer(8)
+    d_scratch = gpj.DeviceBuffer(gpj.lib().gpj_index_scratch_bytes(n))
+    idx = gpj.IndexT(d_newline.ptr, d_numlines.ptr, nl_cap, d_string.ptr, d_leveled.ptr,
+                     words, levels, 0)
+    d_irs = [gpj.DeviceBuffer.from_host(np.frombuffer(c[0], dtype=np.uint8), stream)
+             for c in compiled]
+    d_spans = [gpj.DeviceBuffer(nl_cap * c[2] * 16) for c in compiled]
+    L = gpj.lib()
+
+    def step(ev=None):
+        if ev is not None:
+            hip.record(ev[0], stream)
+        gpj.check(L.gpj_index_build_into(d_in.ptr, n, levels, d_string.ptr, d_leveled.ptr,
+                                         d_newline.ptr, nl_cap, d_numlines.ptr, d_scratch.ptr,
+                                         stream), "index")
+        if ev is not None:
+            hip.record(ev[1], stream)
+        for qi, c in enumerate(compiled):
+            gpj.check(L.gpj_query_run(d_in.ptr, n, ctypes.byref(idx), d_irs[qi].ptr, len(c[0]),
+                                      c[2], nl_cap, d_spans[qi].ptr, stream), "query")
+            if ev is not None:
+                hip.record(ev[2 + qi], stream)
+
+    for _ in range(args.warmup):
+        step()
+    gpj.check(L.gpj_stream_sync(stream))
+    num_lines = int(d_numlines.to_host(np.int64, 1)[0])
+    if num_lines > nl_cap:
+        raise SystemExit("bench.py: newline capacity too small for this workload")
+
+    evs = [[hip.event() for _ in range(2 + len(compiled))] for _ in range(args.steps)]
+    t_start, t_end = hip.event(), hip.event()
+    gpj.check(L.gpj_stream_sync(stream))
+    hip.record(t_start, stream)
+    for k in range(args.steps):
+        step(evs[k])
+    hip.record(t_end, stream)
+    hip.sync_event(t_end)
+    gpj.check(L.gpj_stream_sync(stream))
+    total_ms = hip.elapsed_ms(t_start, t_end)
+    ms_per_step = total_ms / args.steps
+    index_ms = float(np.mean([hip.elapsed_ms(e[0], e[1]) for e in evs]))
+    query_ms = [float(np.mean([hip.elapsed_ms(e[1 + qi], e[2 + qi]) for e in evs]))
+                for qi in range(len(compiled))]
+
+    # roofline of the dominant kernel (index): algorithmic bytes per launch
+    records = num_lines
+    algo_index = n + n / 8 + levels * n / 8 + 8 * (records + 1)
+    achieved = algo_index / (index_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        with open(PROFILE_TRAFFIC) as f:
+            prof = json.load(f)
+        if prof.get("workload") == args.workload:
+            traffic = prof.get("hbm_bytes_per_launch")
+
+    # match counts (spans of slot 0 >= 0) for the record
+    matches = []
+    for qi, c in enumerate(compiled):
+        sp = d_spans[qi].to_host(np.int64, num_lines * c[2] * 2).reshape(num_lines, c[2], 2)
+        matches.append(int((sp[:, :, 0] >= 0).sum()))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = ge.load_oracle()
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = cut_at_newline(data, args.cpu_sample_mib << 20)
+        cpu_compiled = [c for c, q in zip(compiled, queries) if "[*]" not in q]
+        t0 = time.time()
+        ix = orc.build_index(sample, levels, threads)
+        for c in cpu_compiled:
+            orc.query(sample, ix, c[0], c[2], threads)
+        cpu_s = time.time() - t0
+        cpu = {"value": round(sample.size / cpu_s / 1e9, 4), "unit": "GB/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {sample.size} bytes of the same data (cut at a newline), "
+                         f"index + {len(cpu_compiled)} queries "
+                         f"({', '.join(q for q in queries if '[*]' not in q)}); "
+                         f"multi-threaded CPU oracle, {cpu_s:.2f}s wall"}
+
+    out = {
+        "metric": "JSON GB/s indexed+queried (fraction of HBM peak); p50/p99 request latency",
+        "value": round(n / (ms_per_step * 1e-3) / 1e9, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "bytes": n, "records": num_lines - 1,
+                   "queries": queries, "levels": levels, "matches": matches,
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "gpj::index_kernel",
+                     "kernel_ms": round(index_ms, 4), "algorithmic_bytes": int(algo_index)},
+        "cpu_baseline": cpu,
+        "kernels_ms": {"index": round(index_ms, 4),
+                       **{f"query[{q}]": round(t, 4) for q, t in zip(queries, query_ms)}},
+        "h2d_GBps_pageable": round(h2d_gbs, 2),
+        "end_to_end_hbm_frac": round(
+            (algo_index + sum(8 * (records + 1) + n / 8 + 8 * m for m in matches))
+            / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
