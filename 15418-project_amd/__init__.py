@@ -116,12 +116,14 @@ def lib():
         L.gpj_index_build_into.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp, vp]
         L.gpj_debug_escape_quote.argtypes = [vp, u64, vp, vp, vp]
         L.gpj_query_run.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp, vp]
+        L.gpj_query_run_ext.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp,
+                                        vp, vp]
         L.gpj_query_compile.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint8), u32,
                                         u32p, u32p, u32p, u32]
         for name in ("gpj_alloc", "gpj_free", "gpj_h2d", "gpj_d2h", "gpj_stream_sync",
                      "gpj_device_sync", "gpj_index_build", "gpj_index_free",
                      "gpj_index_build_into", "gpj_debug_escape_quote", "gpj_query_run",
-                     "gpj_query_compile"):
+                     "gpj_query_run_ext", "gpj_query_compile"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -286,6 +288,28 @@ def run_query(dev: DeviceBuffer | int, n: int, index: BuiltIndex, ir: bytes, num
     check(lib().gpj_stream_sync(stream))
     out = spans.to_host(np.int64, lines * num_results * 2, stream)
     return out.reshape(lines, num_results, 2)
+
+
+def run_query_ext(dev: DeviceBuffer | int, n: int, index: BuiltIndex, ir: bytes,
+                  capacity: int = 8, stream=None):
+    """Extension bytecode: (spans int64 [lines, capacity, 2], counts int64 [lines]).
+    Re-runs with the exact capacity when a line has more results."""
+    ptr = dev.ptr if isinstance(dev, DeviceBuffer) else int(dev)
+    lines = index.num_lines
+    d_ir = DeviceBuffer.from_host(np.frombuffer(ir, dtype=np.uint8), stream)
+    while True:
+        spans = DeviceBuffer(max(lines * capacity * 16, 16))
+        counts = DeviceBuffer(max(lines * 8, 8))
+        check(lib().gpj_query_run_ext(ptr, int(n), ctypes.byref(index.raw), d_ir.ptr, len(ir),
+                                      int(capacity), lines, spans.ptr, counts.ptr, stream),
+              "query")
+        check(lib().gpj_stream_sync(stream))
+        c = counts.to_host(np.int64, lines, stream)
+        need = int(c.max()) if c.size else 0
+        if need <= capacity:
+            sp = spans.to_host(np.int64, lines * capacity * 2, stream)
+            return sp.reshape(lines, capacity, 2), c
+        capacity = need
 
 
 def debug_escape_quote(dev: DeviceBuffer | int, n: int, stream=None):

@@ -271,27 +271,42 @@ gpj_status gpj_debug_escape_quote(const uint8_t *d_in, uint64_t n, uint64_t *d_e
   return GPJ_OK;
 }
 
-gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,
-                         const uint8_t *d_ir, uint32_t ir_len, int32_t num_results,
-                         int64_t max_lines, int64_t *d_spans, void *stream) {
-  if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_query_run: null index");
+namespace {
+gpj_status launch_query(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                        const uint8_t *d_ir, uint32_t ir_len, int32_t cap, int64_t max_lines,
+                        int32_t variable, int64_t *d_spans, int64_t *d_counts, void *stream) {
+  if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null index");
   if (ir_len == 0 || ir_len > GPJ_MAX_QUERY_IR_BYTES)
-    return fail(GPJ_ERR_QUERY_EXECUTION, "Compiled query IR exceeds constant memory buffer");
-  if (num_results < 1)
-    return fail(GPJ_ERR_QUERY_EXECUTION, "query needs at least one result slot");
+    return fail(GPJ_ERR_QUERY_EXECUTION, "Compiled query IR exceeds the 4096-byte IR buffer");
+  if (cap < 1) return fail(GPJ_ERR_QUERY_EXECUTION, "query needs at least one result slot");
   if (max_lines <= 0 || n == 0) return GPJ_OK;
   if (!d_in || !d_ir || !d_spans || !index->d_newline || !index->d_num_lines ||
-      !index->d_string || !index->d_leveled)
+      !index->d_string || !index->d_leveled || (variable && !d_counts))
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "Query execution requires GPU-resident newline, string, and bitmap indices");
   const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_lines, 256), 1u << 20);
   hipLaunchKernelGGL(gpj::query_kernel, dim3((uint32_t)blocks), dim3(256), 0, as_stream(stream),
                      d_in, (int64_t)n, index->d_newline, index->d_num_lines, max_lines,
-                     index->d_string, index->d_leveled, index->level_size,
-                     index->num_levels, d_ir, ir_len,
-                     num_results, d_spans);
+                     index->d_string, index->d_leveled, index->level_size, index->num_levels,
+                     d_ir, ir_len, cap, variable, d_spans, d_counts);
   GPJ_HIP(hipGetLastError(), "query_kernel launch");
   return GPJ_OK;
+}
+} // namespace
+
+gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                         const uint8_t *d_ir, uint32_t ir_len, int32_t num_results,
+                         int64_t max_lines, int64_t *d_spans, void *stream) {
+  return launch_query(d_in, n, index, d_ir, ir_len, num_results, max_lines, 0, d_spans, nullptr,
+                      stream);
+}
+
+gpj_status gpj_query_run_ext(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                             const uint8_t *d_ir, uint32_t ir_len, int32_t capacity,
+                             int64_t max_lines, int64_t *d_spans, int64_t *d_counts,
+                             void *stream) {
+  return launch_query(d_in, n, index, d_ir, ir_len, capacity, max_lines, 1, d_spans, d_counts,
+                      stream);
 }
 
 gpj_status gpj_query_compile(const char *query, uint8_t *ir, uint32_t ir_cap,

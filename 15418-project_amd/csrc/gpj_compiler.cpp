@@ -24,7 +24,10 @@ namespace {
 enum Op : uint8_t {
   kEnd = 0, kStoreResult = 1, kMoveUp = 2, kMoveDown = 3, kMoveToKey = 4,
   kMoveToIndex = 5, kMoveToIndexReverse = 6, kMarkPosition = 7,
-  kResetPosition = 8, kExpressionStringEquals = 9
+  kResetPosition = 8, kExpressionStringEquals = 9,
+  // extension opcodes (emitted only with GPJ_COMPILE_EXTENSIONS)
+  kMoveToEach = 10,      // iterate the elements of the current array
+  kJumpIfNotArray = 11   // <4-byte varint forward offset>
 };
 
 struct CompileFailure {
@@ -59,6 +62,20 @@ public:
     op(kStoreResult);
     ++stores;
   }
+  // Fixed-width (4-byte) LEB128 so a forward jump can be patched later.
+  size_t jump_placeholder() {
+    op(kJumpIfNotArray);
+    const size_t at = bytes.size();
+    bytes.insert(bytes.end(), {0x80, 0x80, 0x80, 0x00});
+    return at;
+  }
+  void patch_jump(size_t at) {
+    const uint32_t off = static_cast<uint32_t>(bytes.size() - (at + 4));
+    bytes[at] = 0x80 | (off & 0x7F);
+    bytes[at + 1] = 0x80 | ((off >> 7) & 0x7F);
+    bytes[at + 2] = 0x80 | ((off >> 14) & 0x7F);
+    bytes[at + 3] = (off >> 21) & 0x7F;
+  }
   std::vector<uint8_t> bytes;
   size_t level = 0;
   size_t stores = 0;
@@ -73,7 +90,66 @@ public:
     expression();
     out_.store();
     out_.op(kEnd);
-    return {out_.bytes, depth_, out_.stores};
+    // Extension bytecode reports the leveled rows it actually reads; the
+    // reference-mode figure keeps the reference's own count.
+    const size_t depth = ext_ ? rows_needed(out_.bytes) : depth_;
+    return {out_.bytes, depth, out_.stores};
+  }
+
+  // Highest leveled row any path of the bytecode reads, plus one.
+  static size_t rows_needed(const std::vector<uint8_t> &ir) {
+    size_t need = 0;
+    walk(ir, 0, 0, std::vector<size_t>{}, &need);
+    return need;
+  }
+  static uint64_t read_varint(const std::vector<uint8_t> &ir, size_t &pos) {
+    uint64_t v = 0;
+    int shift = 0;
+    while (pos < ir.size()) {
+      const uint8_t b = ir[pos++];
+      v |= static_cast<uint64_t>(b & 0x7F) << shift;
+      if (!(b & 0x80)) break;
+      shift += 7;
+    }
+    return v;
+  }
+  static void walk(const std::vector<uint8_t> &ir, size_t pos, size_t lvl,
+                   std::vector<size_t> marks, size_t *need) {
+    auto touch = [&](size_t row) {
+      if (row + 1 > *need) *need = row + 1;
+    };
+    while (pos < ir.size()) {
+      switch (ir[pos++]) {
+      case kEnd: return;
+      case kStoreResult: break;
+      case kMoveUp: if (lvl > 0) --lvl; break;
+      case kMoveDown: touch(lvl); ++lvl; break;
+      case kMoveToKey: {
+        const uint64_t n = read_varint(ir, pos);
+        pos += n;
+        touch(lvl);
+        break;
+      }
+      case kMoveToIndex:
+      case kMoveToIndexReverse: read_varint(ir, pos); touch(lvl); break;
+      case kMarkPosition: marks.push_back(lvl); break;
+      case kResetPosition:
+        if (!marks.empty()) { lvl = marks.back(); marks.pop_back(); }
+        break;
+      case kExpressionStringEquals: {
+        const uint64_t n = read_varint(ir, pos);
+        pos += n;
+        break;
+      }
+      case kMoveToEach: touch(lvl); break;
+      case kJumpIfNotArray: {
+        const uint64_t off = read_varint(ir, pos);
+        walk(ir, pos + off, lvl, marks, need);
+        break;
+      }
+      default: return;
+      }
+    }
   }
 
 private:
@@ -196,8 +272,15 @@ private:
       }
     } else if (c == '*') {
       if (!ext_) fail_unsupported("Unsupported wildcard expression");
-      fail_unsupported("Unsupported wildcard expression");
+      expect('*');
+      out_.op(kMoveToEach);
+      out_.down();
+      ++depth_;
     } else if (c == '?') {
+      if (ext_) {
+        element_filter();
+        return;
+      }
       filter();
     } else {
       fail_malformed("Unexpected character in index, expected a quoted string or integer");
@@ -259,6 +342,34 @@ private:
     out_.str("\"" + value + "\"");
     out_.op(kResetPosition);
     expect(')');
+  }
+
+  // Extension form of [?(...)]: on an array, filter its elements; on any
+  // other value, test the value itself as the reference does.  Emitted as
+  //   JUMP_IF_NOT_ARRAY L; EACH; DOWN; <filter>; <rest>; STORE; END
+  //   L: <filter>; <rest>            (run() appends STORE; END)
+  void element_filter() {
+    const long cursor = cur_;
+    const size_t saved_depth = depth_;
+    const size_t saved_level = out_.level;
+    const size_t jump = out_.jump_placeholder();
+    out_.op(kMoveToEach);
+    out_.down();
+    ++depth_;
+    filter();
+    expect(']');
+    if (more()) expression();
+    out_.store();
+    out_.op(kEnd);
+    const size_t array_depth = depth_;
+    out_.patch_jump(jump);
+    cur_ = cursor;
+    depth_ = saved_depth;
+    out_.level = saved_level;
+    filter();
+    expect(']');
+    if (more()) expression();
+    if (array_depth > depth_) depth_ = array_depth;
   }
 
   void key(const std::string &name) {

@@ -33,6 +33,7 @@ __global__ void query_kernel(const uint8_t *file, int64_t n,
                              int64_t max_lines, const uint64_t *string,
                              const uint64_t *leveled, int64_t level_size,
                              int32_t num_levels, const uint8_t *ir, uint32_t ir_len,
-                             int32_t num_results, int64_t *result);
+                             int32_t num_results, int32_t variable, int64_t *result,
+                             int64_t *counts);
 
 } // namespace gpj

@@ -22,7 +22,8 @@ namespace {
 enum : uint8_t {
   kEnd = 0, kStoreResult = 1, kMoveUp = 2, kMoveDown = 3, kMoveToKey = 4,
   kMoveToIndex = 5, kMoveToIndexReverse = 6, kMarkPosition = 7,
-  kResetPosition = 8, kExpressionStringEquals = 9
+  kResetPosition = 8, kExpressionStringEquals = 9,
+  kMoveToEach = 10, kJumpIfNotArray = 11 // extension opcodes
 };
 constexpr int kMaxLevels = 16;
 constexpr uint32_t kMaxIr = 4096;
@@ -110,11 +111,26 @@ __device__ __forceinline__ int32_t read_varint(const Ctx &c, int64_t &qp) {
   return (int32_t)(value | (byte << shift));
 }
 
-__device__ void run_record(const Ctx &c, int64_t line_start, int64_t line_end,
-                           int32_t num_results, int64_t *res) {
+// One element iteration of MOVE_TO_EACH (extension): the array's level, the
+// separator before the current element, where the path restarts, and the
+// mark-stack depth to restore.
+struct Frame {
+  int lvl;
+  int64_t sep;
+  int64_t qp;
+  int mi;
+};
+constexpr int kMaxFrames = 8;
+
+// Interprets the bytecode over one record.  Fixed mode (reference): STORE i
+// writes slot i; any failure ends the record.  Variable mode (extension):
+// STORE appends (first `cap` stored, all counted) and a finished path
+// resumes the innermost MOVE_TO_EACH at its next element.
+__device__ int64_t run_record(const Ctx &c, int64_t line_start, int64_t line_end,
+                              int32_t cap, bool variable, int64_t *res) {
   while (line_end > line_start && byte_at(c, line_end) != '}') --line_end;
   while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
-  if (line_start == line_end) return;
+  if (line_start == line_end) return 0;
 
   int64_t idx = line_start;
   int lvl = 0;
@@ -123,6 +139,7 @@ __device__ void run_record(const Ctx &c, int64_t line_start, int64_t line_end,
   int64_t curr_index[kMaxLevels];
   int64_t marked_pos[kMaxLevels];
   int marked_lvl[kMaxLevels];
+  Frame fr[kMaxFrames];
   level_end[0] = line_end;
 #pragma unroll
   for (int i = 1; i < kMaxLevels; ++i) level_end[i] = -1;
@@ -130,143 +147,189 @@ __device__ void run_record(const Ctx &c, int64_t line_start, int64_t line_end,
   for (int i = 0; i < kMaxLevels; ++i) curr_index[i] = -1;
   int mi = -1;
   int nres = 0;
+  int nfr = 0;
+  int64_t count = 0;
 
   for (;;) {
-    if (qp >= (int64_t)c.ir_len) return;
-    const uint8_t op = c.ir[qp++];
-    switch (op) {
-    case kEnd:
-      return;
-    case kStoreResult: {
-      if (nres >= num_results) return;
-      int64_t end_str = level_end[lvl];
-      while (byte_at(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
-      while (end_str > idx && byte_at(c, end_str - 1) == ' ') --end_str;
-      res[nres * 2] = idx;
-      res[nres * 2 + 1] = end_str;
-      ++nres;
-      break;
-    }
-    case kMoveUp:
-      if (lvl <= 0) return;
-      idx = level_end[lvl];
-      level_end[lvl] = -1;
-      --lvl;
-      break;
-    case kMoveDown:
-      if (lvl + 1 >= kMaxLevels) return;
-      ++lvl;
-      if (level_end[lvl] == -1) {
-        level_end[lvl] = next_structural(c, level_end[lvl - 1], idx + 1, lvl - 1);
-        while (byte_at(c, idx) == ' ') ++idx;
-      }
-      break;
-    case kMoveToKey: {
-      const int32_t key_len = read_varint(c, qp);
-      const int64_t key_pos = qp;
-      qp += key_len;
-      if (byte_at(c, idx) != '{') return;
-      for (;;) {
-        ++idx;
-        idx = next_structural(c, level_end[lvl], idx, lvl);
-        if (byte_at(c, idx) != ':') break;
-        const int64_t string_end = last_bit_in(c.string, line_start + 1, idx - 1);
-        const int64_t string_start = string_end - key_len;
-        bool match = string_start >= line_start && byte_at(c, string_start) == '"';
-        for (int32_t i = 0; match && i < key_len; ++i)
-          match = ir_at(c, key_pos + i) == byte_at(c, string_start + i + 1);
-        if (match) break;
-        ++idx;
-        idx = next_structural(c, level_end[lvl], idx, lvl);
-        if (byte_at(c, idx) == '}') return;
-      }
-      ++idx;
-      break;
-    }
-    case kMoveToIndex: {
-      const int32_t target = read_varint(c, qp);
-      const uint8_t ch = byte_at(c, idx);
-      if (ch != '[' && ch != ',' && ch != ']') return;
-      if (ch == '[') curr_index[lvl] = 0;
-      else ++curr_index[lvl];
-      for (;;) {
-        if (curr_index[lvl] < target) {
-          ++idx;
-          idx = next_structural(c, level_end[lvl], idx, lvl);
-          if (byte_at(c, idx) == ',') { ++curr_index[lvl]; continue; }
-          return;
+    if (qp >= (int64_t)c.ir_len) goto stop;
+    {
+      const uint8_t op = c.ir[qp++];
+      switch (op) {
+      case kEnd:
+        goto stop;
+      case kStoreResult: {
+        if (!variable && nres >= cap) goto stop;
+        int64_t end_str = level_end[lvl];
+        while (byte_at(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
+        while (end_str > idx && byte_at(c, end_str - 1) == ' ') --end_str;
+        const int64_t slot = variable ? count : nres;
+        if (slot < cap) {
+          res[slot * 2] = idx;
+          res[slot * 2 + 1] = end_str;
         }
-        if (curr_index[lvl] > target) {
-          --idx;
-          idx = prev_structural(c, 0, idx, lvl);
-          --curr_index[lvl];
-          continue;
-        }
+        if (variable) ++count;
+        else ++nres;
         break;
       }
-      if (byte_at(c, idx) == ']') return;
-      ++idx;
-      break;
-    }
-    case kMoveToIndexReverse: {
-      const int32_t target = read_varint(c, qp);
-      idx = level_end[lvl] - 1;
-      while (idx >= 0 && byte_at(c, idx) == ' ') --idx;
-      const uint8_t ch = byte_at(c, idx);
-      if (ch != ']' && ch != ',' && ch != '[') return;
-      if (ch == ']') curr_index[lvl] = 0;
-      else ++curr_index[lvl];
-      for (;;) {
-        if (curr_index[lvl] < target) {
-          --idx;
-          idx = prev_structural(c, 0, idx, lvl);
-          if (byte_at(c, idx) == ',' || curr_index[lvl] + 1 == target) {
-            ++curr_index[lvl];
+      case kMoveUp:
+        if (lvl <= 0) goto stop;
+        idx = level_end[lvl];
+        level_end[lvl] = -1;
+        --lvl;
+        break;
+      case kMoveDown:
+        if (lvl + 1 >= kMaxLevels) goto stop;
+        ++lvl;
+        if (level_end[lvl] == -1) {
+          level_end[lvl] = next_structural(c, level_end[lvl - 1], idx + 1, lvl - 1);
+          while (byte_at(c, idx) == ' ') ++idx;
+        }
+        break;
+      case kMoveToKey: {
+        const int32_t key_len = read_varint(c, qp);
+        const int64_t key_pos = qp;
+        qp += key_len;
+        if (byte_at(c, idx) != '{') goto stop;
+        for (;;) {
+          ++idx;
+          idx = next_structural(c, level_end[lvl], idx, lvl);
+          if (byte_at(c, idx) != ':') break;
+          const int64_t string_end = last_bit_in(c.string, line_start + 1, idx - 1);
+          const int64_t string_start = string_end - key_len;
+          bool match = string_start >= line_start && byte_at(c, string_start) == '"';
+          for (int32_t i = 0; match && i < key_len; ++i)
+            match = ir_at(c, key_pos + i) == byte_at(c, string_start + i + 1);
+          if (match) break;
+          ++idx;
+          idx = next_structural(c, level_end[lvl], idx, lvl);
+          if (byte_at(c, idx) == '}') goto stop;
+        }
+        ++idx;
+        break;
+      }
+      case kMoveToIndex: {
+        const int32_t target = read_varint(c, qp);
+        const uint8_t ch = byte_at(c, idx);
+        if (ch != '[' && ch != ',' && ch != ']') goto stop;
+        if (ch == '[') curr_index[lvl] = 0;
+        else ++curr_index[lvl];
+        for (;;) {
+          if (curr_index[lvl] < target) {
+            ++idx;
+            idx = next_structural(c, level_end[lvl], idx, lvl);
+            if (byte_at(c, idx) == ',') { ++curr_index[lvl]; continue; }
+            goto stop;
+          }
+          if (curr_index[lvl] > target) {
+            --idx;
+            idx = prev_structural(c, 0, idx, lvl);
+            --curr_index[lvl];
             continue;
           }
-          return;
+          break;
         }
-        if (curr_index[lvl] > target) {
-          ++idx;
-          idx = next_structural(c, level_end[lvl], idx, lvl);
-          --curr_index[lvl];
-          continue;
-        }
+        if (byte_at(c, idx) == ']') goto stop;
+        ++idx;
         break;
       }
-      ++idx;
-      break;
-    }
-    case kMarkPosition:
-      if (mi + 1 >= kMaxLevels) return;
-      ++mi;
-      marked_pos[mi] = idx;
-      marked_lvl[mi] = lvl;
-      break;
-    case kResetPosition:
-      if (mi < 0) return;
-      idx = marked_pos[mi];
-      lvl = marked_lvl[mi];
-      --mi;
-      for (int i = lvl + 1; i < kMaxLevels; ++i) {
-        if (level_end[i] == -1) break;
-        level_end[i] = -1;
+      case kMoveToIndexReverse: {
+        const int32_t target = read_varint(c, qp);
+        idx = level_end[lvl] - 1;
+        while (idx >= 0 && byte_at(c, idx) == ' ') --idx;
+        const uint8_t ch = byte_at(c, idx);
+        if (ch != ']' && ch != ',' && ch != '[') goto stop;
+        if (ch == ']') curr_index[lvl] = 0;
+        else ++curr_index[lvl];
+        for (;;) {
+          if (curr_index[lvl] < target) {
+            --idx;
+            idx = prev_structural(c, 0, idx, lvl);
+            if (byte_at(c, idx) == ',' || curr_index[lvl] + 1 == target) {
+              ++curr_index[lvl];
+              continue;
+            }
+            goto stop;
+          }
+          if (curr_index[lvl] > target) {
+            ++idx;
+            idx = next_structural(c, level_end[lvl], idx, lvl);
+            --curr_index[lvl];
+            continue;
+          }
+          break;
+        }
+        ++idx;
+        break;
       }
-      break;
-    case kExpressionStringEquals: {
-      const int32_t key_len = read_varint(c, qp);
-      const int64_t key_pos = qp;
-      qp += key_len;
-      while (byte_at(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
-      int64_t string_end = level_end[lvl] - 1;
-      while (idx < string_end && byte_at(c, string_end) == ' ') --string_end;
-      if (string_end - idx + 1 != key_len) return;
-      for (int32_t i = 0; i < key_len; ++i)
-        if (ir_at(c, key_pos + i) != byte_at(c, idx + i)) return;
-      break;
+      case kMarkPosition:
+        if (mi + 1 >= kMaxLevels) goto stop;
+        ++mi;
+        marked_pos[mi] = idx;
+        marked_lvl[mi] = lvl;
+        break;
+      case kResetPosition:
+        if (mi < 0) goto stop;
+        idx = marked_pos[mi];
+        lvl = marked_lvl[mi];
+        --mi;
+        for (int i = lvl + 1; i < kMaxLevels; ++i) {
+          if (level_end[i] == -1) break;
+          level_end[i] = -1;
+        }
+        break;
+      case kExpressionStringEquals: {
+        const int32_t key_len = read_varint(c, qp);
+        const int64_t key_pos = qp;
+        qp += key_len;
+        while (byte_at(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
+        int64_t string_end = level_end[lvl] - 1;
+        while (idx < string_end && byte_at(c, string_end) == ' ') --string_end;
+        if (string_end - idx + 1 != key_len) goto stop;
+        bool eq = true;
+        for (int32_t i = 0; eq && i < key_len; ++i)
+          eq = ir_at(c, key_pos + i) == byte_at(c, idx + i);
+        if (!eq) goto stop;
+        break;
+      }
+      case kMoveToEach: {
+        if (!variable || byte_at(c, idx) != '[' || nfr >= kMaxFrames) goto stop;
+        const int64_t first = next_structural(c, level_end[lvl], idx + 1, lvl);
+        int64_t p = idx + 1;
+        while (p < first && byte_at(c, p) == ' ') ++p;
+        if (p == first && byte_at(c, first) == ']') goto stop; // empty array
+        fr[nfr].lvl = lvl;
+        fr[nfr].sep = idx;
+        fr[nfr].qp = qp;
+        fr[nfr].mi = mi;
+        ++nfr;
+        ++idx;
+        break;
+      }
+      case kJumpIfNotArray: {
+        const int32_t off = read_varint(c, qp);
+        if (byte_at(c, idx) != '[') qp += off;
+        break;
+      }
+      default:
+        goto stop;
+      }
     }
-    default:
-      return;
+    continue;
+  stop:
+    for (;;) {
+      if (nfr == 0) return count;
+      Frame &f = fr[nfr - 1];
+      const int64_t sep = next_structural(c, level_end[f.lvl], f.sep + 1, f.lvl);
+      if (byte_at(c, sep) == ',' && sep < level_end[f.lvl]) {
+        f.sep = sep;
+        lvl = f.lvl;
+        for (int i = lvl + 1; i < kMaxLevels; ++i) level_end[i] = -1;
+        mi = f.mi;
+        idx = sep + 1;
+        qp = f.qp;
+        break;
+      }
+      --nfr;
     }
   }
 }
@@ -279,7 +342,7 @@ __global__ __launch_bounds__(256) void query_kernel(
     int64_t max_lines, const uint64_t *__restrict__ string,
     const uint64_t *__restrict__ leveled, int64_t level_size, int32_t num_levels,
     const uint8_t *__restrict__ ir, uint32_t ir_len, int32_t num_results,
-    int64_t *__restrict__ result) {
+    int32_t variable, int64_t *__restrict__ result, int64_t *__restrict__ counts) {
   __shared__ uint8_t s_ir[kMaxIr];
   const uint32_t len = ir_len < kMaxIr ? ir_len : kMaxIr;
   for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) s_ir[i] = ir[i];
@@ -294,9 +357,12 @@ __global__ __launch_bounds__(256) void query_kernel(
     for (int i = 0; i < 2 * num_results; ++i) res[i] = -1;
     const int64_t line_start = newline[line];
     int64_t line_end = (line + 1 < total) ? newline[line + 1] : n - 1;
-    if (line_start < 0 || line_start >= n || line_end < 0) continue;
-    if (line_end >= n) line_end = n - 1;
-    run_record(c, line_start, line_end, num_results, res);
+    int64_t found = 0;
+    if (line_start >= 0 && line_start < n && line_end >= 0) {
+      if (line_end >= n) line_end = n - 1;
+      found = run_record(c, line_start, line_end, num_results, variable != 0, res);
+    }
+    if (counts) counts[line] = found;
   }
 }
 

@@ -125,11 +125,25 @@ gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index
                          int32_t num_results, int64_t max_lines,
                          int64_t *d_spans, void *stream);
 
+/* Extension mode (bytecode compiled with GPJ_COMPILE_EXTENSIONS; no
+ * counterpart in the reference, whose compiler rejects `[*]` at
+ * src/gpjson/query/query_compiler.cpp:287-290): a record may yield any number
+ * of results, in execution order.  d_spans receives the first `capacity`
+ * {start, end} pairs of each line ([max_lines][capacity][2], -1 padded) and
+ * d_counts the full count per line; a caller re-runs with a larger capacity
+ * when any count exceeds it.  Stream-ordered; allocates nothing. */
+gpj_status gpj_query_run_ext(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                             const uint8_t *d_ir, uint32_t ir_len, int32_t capacity,
+                             int64_t max_lines, int64_t *d_spans, int64_t *d_counts,
+                             void *stream);
+
 /* ---- query compiler: replaces QueryCompiler::compile
  *      (src/gpjson/query/query_compiler.cpp:467-477).  Host only.
  * Writes the bytecode into ir (capacity ir_cap) and its length, the query's
  * max depth and its number of result slots.  flags: GPJ_COMPILE_EXTENSIONS
- * enables the `[*]` wildcard, which the reference rejects. */
+ * enables the `[*]` wildcard (which the reference rejects) and element
+ * filters on arrays; such bytecode runs through gpj_query_run_ext and its
+ * max depth is the number of leveled rows it reads. */
 #define GPJ_COMPILE_EXTENSIONS 1u
 gpj_status gpj_query_compile(const char *query, uint8_t *ir, uint32_t ir_cap,
                              uint32_t *ir_len, uint32_t *max_depth,

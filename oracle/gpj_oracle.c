@@ -265,11 +265,34 @@ static int32_t read_varint(const uint8_t *ir, int64_t ir_len, int64_t *pos) {
   return (int32_t)(value | (byte << shift));
 }
 
+/* Extension opcodes (not in the reference; emitted only by the compiler's
+ * extension mode):
+ *   MOVE_TO_EACH (10): the current value must be a non-empty array; run the
+ *     rest of the bytecode once per element, in order.  Whenever a path ends
+ *     (END or any failure) the innermost iteration resumes at its next
+ *     element; the record ends when no iteration is left.
+ *   JUMP_IF_NOT_ARRAY (11) <varint off>: skip `off` bytecode bytes unless the
+ *     current value starts with '['.
+ * In variable mode STORE_RESULT appends to the record's result list (up to
+ * `cap` spans stored, all counted). */
+enum { OP_MOVE_TO_EACH = 10, OP_JUMP_IF_NOT_ARRAY = 11 };
+#define QMAX_FRAMES 8
+
+typedef struct {
+  int lvl;
+  int64_t sep;  /* '[' or ',' before the current element */
+  int64_t qp;   /* bytecode position after MOVE_TO_EACH */
+  int mi;
+} qframe;
+
 static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
                        int64_t line, const uint8_t *ir, int64_t ir_len,
-                       int num_results, int64_t *result) {
+                       int num_results, int variable, int64_t *result,
+                       int64_t *counts) {
   int64_t *res = result + line * 2 * (int64_t)num_results;
   for (int i = 0; i < 2 * num_results; ++i) res[i] = -1;
+  int64_t count = 0;
+  if (counts) counts[line] = 0;
 
   int64_t line_start = nl[line];
   int64_t line_end = (line + 1 < nl_size) ? nl[line + 1] : c->n - 1;
@@ -291,31 +314,42 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
   int nres = 0;
   int64_t curr_index[QMAX_LEVELS];
   for (int i = 0; i < QMAX_LEVELS; ++i) curr_index[i] = -1;
+  qframe fr[QMAX_FRAMES];
+  int nfr = 0;
 
   for (;;) {
-    if (qp >= ir_len) return;
+    if (qp >= ir_len) goto stop;
+    {
     uint8_t op = ir[qp++];
     switch (op) {
     case OP_END:
-      return;
+      goto stop;
     case OP_STORE_RESULT: {
-      if (nres >= num_results) return;
+      if (!variable && nres >= num_results) goto stop;
       int64_t end_str = level_end[lvl];
       while (qbyte(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
       while (end_str > idx && qbyte(c, end_str - 1) == ' ') --end_str;
-      res[nres * 2] = idx;
-      res[nres * 2 + 1] = end_str;
-      ++nres;
+      if (variable) {
+        if (count < num_results) {
+          res[count * 2] = idx;
+          res[count * 2 + 1] = end_str;
+        }
+        ++count;
+      } else {
+        res[nres * 2] = idx;
+        res[nres * 2 + 1] = end_str;
+        ++nres;
+      }
       break;
     }
     case OP_MOVE_UP:
-      if (lvl <= 0) return;
+      if (lvl <= 0) goto stop;
       idx = level_end[lvl];
       level_end[lvl] = -1;
       --lvl;
       break;
     case OP_MOVE_DOWN:
-      if (lvl + 1 >= QMAX_LEVELS) return;
+      if (lvl + 1 >= QMAX_LEVELS) goto stop;
       ++lvl;
       if (level_end[lvl] == -1) {
         level_end[lvl] = next_struct(c, level_end[lvl - 1], idx + 1, lvl - 1);
@@ -326,7 +360,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
       int32_t key_len = read_varint(ir, ir_len, &qp);
       int64_t key_pos = qp;
       qp += key_len;
-      if (qbyte(c, idx) != '{') return;
+      if (qbyte(c, idx) != '{') goto stop;
       for (;;) { /* search_key */
         ++idx;
         idx = next_struct(c, level_end[lvl], idx, lvl);
@@ -340,7 +374,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
         if (match) break;
         ++idx;
         idx = next_struct(c, level_end[lvl], idx, lvl);
-        if (qbyte(c, idx) == '}') return;
+        if (qbyte(c, idx) == '}') goto stop;
       }
       ++idx;
       break;
@@ -348,7 +382,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
     case OP_MOVE_TO_INDEX: {
       int32_t target = read_varint(ir, ir_len, &qp);
       uint8_t ch = qbyte(c, idx);
-      if (ch != '[' && ch != ',' && ch != ']') return;
+      if (ch != '[' && ch != ',' && ch != ']') goto stop;
       if (ch == '[') curr_index[lvl] = 0;
       else ++curr_index[lvl];
       for (;;) { /* search_index */
@@ -356,7 +390,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
           ++idx;
           idx = next_struct(c, level_end[lvl], idx, lvl);
           if (qbyte(c, idx) == ',') { ++curr_index[lvl]; continue; }
-          return;
+          goto stop;
         }
         if (curr_index[lvl] > target) {
           --idx;
@@ -366,7 +400,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
         }
         break;
       }
-      if (qbyte(c, idx) == ']') return;
+      if (qbyte(c, idx) == ']') goto stop;
       ++idx;
       break;
     }
@@ -375,7 +409,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
       idx = level_end[lvl] - 1;
       while (idx >= 0 && qbyte(c, idx) == ' ') --idx;
       uint8_t ch = qbyte(c, idx);
-      if (ch != ']' && ch != ',' && ch != '[') return;
+      if (ch != ']' && ch != ',' && ch != '[') goto stop;
       if (ch == ']') curr_index[lvl] = 0;
       else ++curr_index[lvl];
       for (;;) { /* search_index_reverse */
@@ -386,7 +420,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
             ++curr_index[lvl];
             continue;
           }
-          return;
+          goto stop;
         }
         if (curr_index[lvl] > target) {
           ++idx;
@@ -400,13 +434,13 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
       break;
     }
     case OP_MARK_POSITION:
-      if (mi + 1 >= QMAX_LEVELS) return;
+      if (mi + 1 >= QMAX_LEVELS) goto stop;
       ++mi;
       marked_pos[mi] = idx;
       marked_lvl[mi] = lvl;
       break;
     case OP_RESET_POSITION:
-      if (mi < 0) return;
+      if (mi < 0) goto stop;
       idx = marked_pos[mi];
       lvl = marked_lvl[mi];
       --mi;
@@ -422,13 +456,54 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
       while (qbyte(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
       int64_t string_end = level_end[lvl] - 1;
       while (idx < string_end && qbyte(c, string_end) == ' ') --string_end;
-      if (string_end - idx + 1 != key_len) return;
+      if (string_end - idx + 1 != key_len) goto stop;
       for (int32_t i = 0; i < key_len; ++i)
-        if ((key_pos + i >= ir_len ? 0 : ir[key_pos + i]) != qbyte(c, idx + i)) return;
+        if ((key_pos + i >= ir_len ? 0 : ir[key_pos + i]) != qbyte(c, idx + i)) goto stop;
+      break;
+    }
+    case OP_MOVE_TO_EACH: {
+      if (!variable || qbyte(c, idx) != '[' || nfr >= QMAX_FRAMES) goto stop;
+      int64_t first = next_struct(c, level_end[lvl], idx + 1, lvl);
+      int64_t p = idx + 1;
+      while (p < first && qbyte(c, p) == ' ') ++p;
+      if (p == first && qbyte(c, first) == ']') goto stop; /* empty array */
+      fr[nfr].lvl = lvl;
+      fr[nfr].sep = idx;
+      fr[nfr].qp = qp;
+      fr[nfr].mi = mi;
+      ++nfr;
+      ++idx;
+      break;
+    }
+    case OP_JUMP_IF_NOT_ARRAY: {
+      int32_t off = read_varint(ir, ir_len, &qp);
+      if (qbyte(c, idx) != '[') qp += off;
       break;
     }
     default:
-      return;
+      goto stop;
+    }
+    }
+    continue;
+  stop:
+    /* path finished: resume the innermost element iteration, if any */
+    for (;;) {
+      if (nfr == 0) {
+        if (counts) counts[line] = count;
+        return;
+      }
+      qframe *f = &fr[nfr - 1];
+      int64_t sep = next_struct(c, level_end[f->lvl], f->sep + 1, f->lvl);
+      if (qbyte(c, sep) == ',' && sep < level_end[f->lvl]) {
+        f->sep = sep;
+        lvl = f->lvl;
+        for (int i = lvl + 1; i < QMAX_LEVELS; ++i) level_end[i] = -1;
+        mi = f->mi;
+        idx = sep + 1;
+        qp = f->qp;
+        break;
+      }
+      --nfr;
     }
   }
 }
@@ -442,7 +517,20 @@ void gpjo_query(const uint8_t *file, int64_t n, const int64_t *newline,
                 int64_t *result) {
   qctx c = {file, n, string, leveled, level_size, num_levels};
   for (int64_t line = 0; line < num_lines; ++line)
-    query_line(&c, newline, num_lines, line, ir, ir_len, num_results, result);
+    query_line(&c, newline, num_lines, line, ir, ir_len, num_results, 0, result, NULL);
+}
+
+/* Extension mode: variable-length results.  result holds num_lines * cap * 2
+ * int64 spans (first `cap` results of each line, in execution order), counts
+ * the number of results of each line (which may exceed cap). */
+void gpjo_query_ext(const uint8_t *file, int64_t n, const int64_t *newline,
+                    int64_t num_lines, const uint64_t *string,
+                    const uint64_t *leveled, int64_t level_size, int num_levels,
+                    const uint8_t *ir, int64_t ir_len, int cap, int64_t *result,
+                    int64_t *counts) {
+  qctx c = {file, n, string, leveled, level_size, num_levels};
+  for (int64_t line = 0; line < num_lines; ++line)
+    query_line(&c, newline, num_lines, line, ir, ir_len, cap, 1, result, counts);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -596,7 +684,15 @@ uint64_t gpjo_build_index_mt(const uint8_t *file, uint64_t n, int levels,
     uint64_t s = (uint64_t)t * per * 64, e = s + per * 64;
     if (s >= n && !(n == 0 && t == 0)) break;
     if (e > n) e = n;
-    ch[t] = (mt_chunk){file, n, s, e, levels, string_out, leveled_out, newline_out};
+    memset(&ch[t], 0, sizeof ch[t]);
+    ch[t].file = file;
+    ch[t].n = n;
+    ch[t].start = s;
+    ch[t].end = e;
+    ch[t].levels = levels;
+    ch[t].string_out = string_out;
+    ch[t].leveled_out = leveled_out;
+    ch[t].newline_out = newline_out;
     ++used;
   }
   mt_run(ch, used, 0);
@@ -635,13 +731,16 @@ typedef struct {
   const uint8_t *ir;
   int64_t ir_len;
   int num_results;
+  int variable;
   int64_t *result;
+  int64_t *counts;
 } mt_query;
 
 static void *mt_query_worker(void *arg) {
   mt_query *q = (mt_query *)arg;
   for (int64_t line = q->lo; line < q->hi; ++line)
-    query_line(&q->c, q->nl, q->nl_size, line, q->ir, q->ir_len, q->num_results, q->result);
+    query_line(&q->c, q->nl, q->nl_size, line, q->ir, q->ir_len, q->num_results, q->variable,
+               q->result, q->counts);
   return NULL;
 }
 
@@ -659,7 +758,30 @@ void gpjo_query_mt(const uint8_t *file, int64_t n, const int64_t *newline,
     if (hi > num_lines) hi = num_lines;
     if (lo > hi) lo = hi;
     q[t] = (mt_query){{file, n, string, leveled, level_size, num_levels}, newline, num_lines,
-                      lo, hi, ir, ir_len, num_results, result};
+                      lo, hi, ir, ir_len, num_results, 0, result, NULL};
+    pthread_create(&tid[t], NULL, mt_query_worker, &q[t]);
+  }
+  for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+  free(tid);
+  free(q);
+}
+
+/* Extension mode on `threads` host threads. */
+void gpjo_query_ext_mt(const uint8_t *file, int64_t n, const int64_t *newline,
+                       int64_t num_lines, const uint64_t *string,
+                       const uint64_t *leveled, int64_t level_size, int num_levels,
+                       const uint8_t *ir, int64_t ir_len, int cap, int64_t *result,
+                       int64_t *counts, int threads) {
+  if (threads < 1) threads = 1;
+  mt_query *q = (mt_query *)calloc((size_t)threads, sizeof(mt_query));
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+  int64_t per = (num_lines + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    int64_t lo = t * per, hi = lo + per;
+    if (hi > num_lines) hi = num_lines;
+    if (lo > hi) lo = hi;
+    q[t] = (mt_query){{file, n, string, leveled, level_size, num_levels}, newline, num_lines,
+                      lo, hi, ir, ir_len, cap, 1, result, counts};
     pthread_create(&tid[t], NULL, mt_query_worker, &q[t]);
   }
   for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);

@@ -54,6 +54,10 @@ def lib():
         L.gpjo_query.argtypes = [u8p, i64, i64p, i64, u64p, u64p, i64, cint, u8p, i64, cint, i64p]
         L.gpjo_query_mt.argtypes = [u8p, i64, i64p, i64, u64p, u64p, i64, cint, u8p, i64, cint, i64p,
                                     cint]
+        L.gpjo_query_ext.argtypes = [u8p, i64, i64p, i64, u64p, u64p, i64, cint, u8p, i64, cint,
+                                     i64p, i64p]
+        L.gpjo_query_ext_mt.argtypes = [u8p, i64, i64p, i64, u64p, u64p, i64, cint, u8p, i64, cint,
+                                        i64p, i64p, cint]
         L.gpjo_partition.argtypes = [u8p, u64, u64, u64p, i64]
         L.gpjo_partition.restype = i64
         _lib = L
@@ -164,6 +168,29 @@ def query(data, index, ir: bytes, num_results: int, threads: int = 1) -> np.ndar
     else:
         lib().gpjo_query_mt(*args, threads)
     return res[: nl.size * num_results * 2].reshape(nl.size, num_results, 2)
+
+
+def query_ext(data, index, ir: bytes, cap: int = 8, threads: int = 1):
+    """Extension mode: (spans int64 [lines, cap, 2], counts int64 [lines]); a
+    line's results beyond `cap` are counted but not stored."""
+    f = as_bytes(data)
+    string, leveled, nl = index
+    w = words(f.size)
+    res = np.full(max(nl.size * cap * 2, 1), -1, dtype=np.int64)
+    counts = np.zeros(max(nl.size, 1), dtype=np.int64)
+    irb = np.frombuffer(bytes(ir), dtype=np.uint8).copy()
+    lv = np.asarray(leveled)
+    args = (_p(_buf(f), ctypes.c_uint8), f.size, _p(_buf(nl), ctypes.c_int64), nl.size,
+            _p(_buf(np.ascontiguousarray(string)), ctypes.c_uint64),
+            _p(_buf(np.ascontiguousarray(lv).reshape(-1)), ctypes.c_uint64), w,
+            int(lv.shape[0]) if lv.ndim == 2 else 0,
+            _p(_buf(irb), ctypes.c_uint8), irb.size, cap, _p(res, ctypes.c_int64),
+            _p(counts, ctypes.c_int64))
+    if threads <= 1:
+        lib().gpjo_query_ext(*args)
+    else:
+        lib().gpjo_query_ext_mt(*args, threads)
+    return res[: nl.size * cap * 2].reshape(nl.size, cap, 2), counts[: nl.size]
 
 
 class PartitionError(Exception):

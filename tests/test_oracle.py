@@ -319,3 +319,90 @@ def test_query_oracle_quirks(orc, gpj):
     ir, depth, slots = gpj.compile_query("$")
     res = orc.query(data, orc.build_index(data, 1), ir, slots)
     assert data[res[0, 0, 0]:res[0, 0, 1]] == b'{"a":{}'
+
+
+# ---------------------------------------------------------------------------
+# extension semantics ([*], array-element filter) vs the DOM evaluator
+# ---------------------------------------------------------------------------
+def dom_eval_ext(data: bytes, node, steps, out):
+    """Standard JSONPath over arrays: [*] visits every element in order; a
+    filter on an array keeps the matching elements, on any other value it
+    tests the value itself (the reference's behaviour).  Results in document
+    order."""
+    if not steps:
+        out.append((node.start, node.end))
+        return
+    kind, arg = steps[0]
+    if kind == "each":
+        if node.kind == "arr":
+            for child in node.children:
+                dom_eval_ext(data, child, steps[1:], out)
+        return
+    if kind == "keyeq":
+        key, val = arg
+        targets = node.children if node.kind == "arr" else [node]
+        for t in targets:
+            if t.kind != "obj":
+                continue
+            for k, child in t.children:
+                if k == key:
+                    if data[child.start:child.end] == b'"' + val + b'"':
+                        dom_eval_ext(data, t, steps[1:], out)
+                    break
+        return
+    if kind == "key":
+        if node.kind == "obj":
+            for k, child in node.children:
+                if k == arg:
+                    dom_eval_ext(data, child, steps[1:], out)
+                    return
+        return
+    raise AssertionError(kind)
+
+
+EXT_DOM_QUERIES = [
+    ("$.h[*]", [("key", b"h"), ("each", None)]),
+    ("$.h[*].t", [("key", b"h"), ("each", None), ("key", b"t")]),
+    ("$.h[*][*]", [("key", b"h"), ("each", None), ("each", None)]),
+    ('$.h[?(@.k == "v")].x', [("key", b"h"), ("keyeq", (b"k", b"v")), ("key", b"x")]),
+    ('$.h[?(@.k == "v")]', [("key", b"h"), ("keyeq", (b"k", b"v"))]),
+    ("$.a.h[*].k", [("key", b"a"), ("key", b"h"), ("each", None), ("key", b"k")]),
+]
+
+
+def test_extension_oracle_matches_dom(orc, gpj):
+    r = random.Random(5)
+    lines = []
+    for _ in range(1200):
+        ks = r.sample(["a", "h", "x", "k", "t"], r.randint(1, 4))
+        lines.append("{" + ",".join(f'"{k}":{dom_record(r, 1)}' for k in ks) + "}")
+    data = ("\n".join(lines)).encode()
+    nodes, pos = [], 0
+    for ln in lines:
+        nodes.append(parse(data, pos)[0])
+        pos += len(ln) + 1
+    checked = 0
+    for text, steps in EXT_DOM_QUERIES:
+        ir, depth, _ = gpj.compile_query(text, extensions=True)
+        spans, counts = orc.query_ext(data, orc.build_index(data, max(depth, 1)), ir, 64)
+        for li, node in enumerate(nodes):
+            out = []
+            dom_eval_ext(data, node, steps, out)
+            got = [(int(a), int(b)) for a, b in spans[li][: int(counts[li])]]
+            assert got == out, (text, lines[li], got, out)
+            checked += len(out)
+    assert checked > 500
+
+
+def test_extension_mode_keeps_reference_results(orc, gpj):
+    # bytecode without extension opcodes gives the same spans in both modes
+    data = jsongen.ndjson(8, 200_000, max_depth=5)
+    for q in ("$.a", "$.a.b", "$.tags[1]", "$.arr[0:2]"):
+        ir, depth, slots = gpj.compile_query(q)
+        idx = orc.build_index(data, max(depth, 1))
+        fixed = orc.query(data, idx, ir, slots)
+        spans, counts = orc.query_ext(data, idx, ir, slots)
+        for li in range(fixed.shape[0]):
+            got = [tuple(x) for x in spans[li][: int(counts[li])].tolist()]
+            want = [tuple(x) for x in fixed[li].tolist() if x[0] >= 0]
+            assert got == want
