@@ -115,6 +115,7 @@ def lib():
         L.gpj_index_scratch_bytes.restype = u64
         L.gpj_index_build_into.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp, vp]
         L.gpj_debug_escape_quote.argtypes = [vp, u64, vp, vp, vp]
+        L.gpj_debug_copy.argtypes = [vp, vp, u64, vp]
         L.gpj_query_run.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp, vp]
         L.gpj_query_run_ext.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp,
                                         vp, vp]
@@ -123,7 +124,7 @@ def lib():
         for name in ("gpj_alloc", "gpj_free", "gpj_h2d", "gpj_d2h", "gpj_stream_sync",
                      "gpj_device_sync", "gpj_index_build", "gpj_index_free",
                      "gpj_index_build_into", "gpj_debug_escape_quote", "gpj_query_run",
-                     "gpj_query_run_ext", "gpj_query_compile"):
+                     "gpj_query_run_ext", "gpj_query_compile", "gpj_debug_copy"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -172,6 +173,38 @@ def compile_query(query: str, extensions: bool = False):
                                  COMPILE_EXTENSIONS if extensions else 0)
     check(st)
     return bytes(buf[: n.value]), d.value, s.value
+
+
+def uses_extension_opcodes(ir: bytes) -> bool:
+    """True if the bytecode holds MOVE_TO_EACH (10) or JUMP_IF_NOT_ARRAY (11),
+    i.e. it must run through the variable-length entry point."""
+    pos = 0
+
+    def varint():
+        nonlocal pos
+        v, shift = 0, 0
+        while pos < len(ir):
+            b = ir[pos]
+            pos += 1
+            v |= (b & 0x7F) << shift
+            if not b & 0x80:
+                break
+            shift += 7
+        return v
+
+    while pos < len(ir):
+        op = ir[pos]
+        pos += 1
+        if op in (10, 11):
+            return True
+        if op in (4, 9):
+            n = varint()
+            pos += n
+        elif op in (5, 6):
+            varint()
+        elif op == 0:
+            continue
+    return False
 
 
 # ---------------------------------------------------------------------------

@@ -63,6 +63,15 @@ gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int lev
   return GPJ_OK;
 }
 
+// Plain 16-byte-per-lane grid-stride copy; the calibration kernel for HBM
+// traffic counters.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src,
+                                                   uint4 *__restrict__ dst, uint64_t n16) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+}
+
 } // namespace
 
 extern "C" {
@@ -119,6 +128,19 @@ gpj_status gpj_device_sync(void) {
 }
 
 uint64_t gpj_index_scratch_bytes(uint64_t n) { return scratch_bytes(n); }
+
+gpj_status gpj_debug_copy(void *d_dst, const void *d_src, uint64_t bytes, void *stream) {
+  if (bytes % 16 != 0 || (reinterpret_cast<uintptr_t>(d_dst) & 15u) ||
+      (reinterpret_cast<uintptr_t>(d_src) & 15u))
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_debug_copy: 16-byte aligned sizes and pointers");
+  if (bytes == 0) return GPJ_OK;
+  const uint64_t n16 = bytes / 16;
+  const uint64_t blocks = std::min<uint64_t>(div_up(n16, 256), 8192);
+  hipLaunchKernelGGL(copy_kernel, dim3((uint32_t)blocks), dim3(256), 0, as_stream(stream),
+                     static_cast<const uint4 *>(d_src), static_cast<uint4 *>(d_dst), n16);
+  GPJ_HIP(hipGetLastError(), "copy_kernel launch");
+  return GPJ_OK;
+}
 
 gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
                                 uint64_t *d_string, uint64_t *d_leveled,

@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--cpu-sample-mib", type=int, default=512)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel over MIB MiB (profiler counter scale)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -106,6 +108,14 @@ def main():
     if not gpj.device_available():
         raise SystemExit("bench.py: no HIP device")
     hip = Hip()
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        src, dst = gpj.DeviceBuffer(nb), gpj.DeviceBuffer(nb)
+        for _ in range(3):
+            gpj.check(gpj.lib().gpj_debug_copy(dst.ptr, src.ptr, nb, None))
+        gpj.check(gpj.lib().gpj_stream_sync(None))
+        print(json.dumps({"copy_calibration_bytes": nb, "launches": 3}))
+        return
     schema, nbytes, seed, queries = WORKLOADS[args.workload]
 
     t0 = time.time()
@@ -116,7 +126,7 @@ def main():
 
     compiled = [gpj.compile_query(q, extensions=True) for q in queries]
     # bytecode with extension opcodes ([*], array filters) runs in variable mode
-    variable = [any(b in (10, 11) for b in c[0]) for c in compiled]
+    variable = [gpj.uses_extension_opcodes(c[0]) for c in compiled]
     ext_cap = 8
     levels = max(1, max(c[1] for c in compiled))
     words = (n + 63) // 64
@@ -144,6 +154,7 @@ def main():
     d_spans = [gpj.DeviceBuffer(nl_cap * k * 16) for k in caps]
     d_counts = [gpj.DeviceBuffer(nl_cap * 8) if v else None for v in variable]
     L = gpj.lib()
+    grid_lines = nl_cap
 
     def step(ev=None):
         if ev is not None:
@@ -156,21 +167,26 @@ def main():
         for qi, c in enumerate(compiled):
             if variable[qi]:
                 gpj.check(L.gpj_query_run_ext(d_in.ptr, n, ctypes.byref(idx), d_irs[qi].ptr,
-                                              len(c[0]), ext_cap, nl_cap, d_spans[qi].ptr,
+                                              len(c[0]), ext_cap, grid_lines, d_spans[qi].ptr,
                                               d_counts[qi].ptr, stream), "query")
             else:
                 gpj.check(L.gpj_query_run(d_in.ptr, n, ctypes.byref(idx), d_irs[qi].ptr,
-                                          len(c[0]), c[2], nl_cap, d_spans[qi].ptr, stream),
+                                          len(c[0]), c[2], grid_lines, d_spans[qi].ptr, stream),
                           "query")
             if ev is not None:
                 hip.record(ev[2 + qi], stream)
 
-    for _ in range(args.warmup):
-        step()
+    # first build: learn the line count (the reference reads it back the same
+    # way, orig_index_builder.cu:325-330) and size the query grids to it
+    step()
     gpj.check(L.gpj_stream_sync(stream))
     num_lines = int(d_numlines.to_host(np.int64, 1)[0])
     if num_lines > nl_cap:
         raise SystemExit("bench.py: newline capacity too small for this workload")
+    grid_lines = num_lines
+    for _ in range(args.warmup):
+        step()
+    gpj.check(L.gpj_stream_sync(stream))
 
     evs = [[hip.event() for _ in range(2 + len(compiled))] for _ in range(args.steps)]
     t_start, t_end = hip.event(), hip.event()

@@ -110,6 +110,11 @@ gpj_status gpj_debug_escape_quote(const uint8_t *d_in, uint64_t n,
                                   uint64_t *d_escape, uint64_t *d_quote,
                                   void *stream);
 
+/* Calibration: a plain device-to-device copy kernel (no counterpart in the
+ * reference).  Its HBM traffic counters give the scale factor that turns
+ * profiler FETCH_SIZE / WRITE_SIZE readings into bytes.  16-byte aligned. */
+gpj_status gpj_debug_copy(void *d_dst, const void *d_src, uint64_t bytes, void *stream);
+
 /* ---- query: replaces the query kernel launch of QueryExecutor::execute_batch
  *      (src/gpjson/query/kernels/orig/query_executor_host.cu:179-237,
  *      src/gpjson/query/kernels/orig/query.cu:144-454) */
