@@ -78,6 +78,17 @@ class IndexT(ctypes.Structure):
     ]
 
 
+class QueryDescT(ctypes.Structure):
+    _fields_ = [
+        ("d_ir", ctypes.c_void_p),
+        ("ir_len", ctypes.c_uint32),
+        ("capacity", ctypes.c_int32),
+        ("variable", ctypes.c_int32),
+        ("d_spans", ctypes.c_void_p),
+        ("d_counts", ctypes.c_void_p),
+    ]
+
+
 class IndexOptsT(ctypes.Structure):
     _fields_ = [
         ("grid_size", ctypes.c_int32),
@@ -119,12 +130,15 @@ def lib():
         L.gpj_query_run.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp, vp]
         L.gpj_query_run_ext.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp,
                                         vp, vp]
+        L.gpj_query_run_batch.argtypes = [vp, u64, ctypes.POINTER(IndexT), ctypes.POINTER(QueryDescT),
+                                          i32, i64, vp]
         L.gpj_query_compile.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint8), u32,
                                         u32p, u32p, u32p, u32]
         for name in ("gpj_alloc", "gpj_free", "gpj_h2d", "gpj_d2h", "gpj_stream_sync",
                      "gpj_device_sync", "gpj_index_build", "gpj_index_free",
                      "gpj_index_build_into", "gpj_debug_escape_quote", "gpj_query_run",
-                     "gpj_query_run_ext", "gpj_query_compile", "gpj_debug_copy"):
+                     "gpj_query_run_ext", "gpj_query_run_batch", "gpj_query_compile",
+                     "gpj_debug_copy"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -340,9 +354,41 @@ def run_query_ext(dev: DeviceBuffer | int, n: int, index: BuiltIndex, ir: bytes,
         c = counts.to_host(np.int64, lines, stream)
         need = int(c.max()) if c.size else 0
         if need <= capacity:
-            sp = spans.to_host(np.int64, lines * capacity * 2, stream)
-            return sp.reshape(lines, capacity, 2), c
+            sp = spans.to_host(np.int64, lines * capacity * 2, stream).reshape(lines, capacity, 2)
+            # entries past a line's count are unspecified: present them as -1
+            sp[np.arange(capacity)[None, :] >= c[:, None]] = -1
+            return sp, c
         capacity = need
+
+
+def run_query_batch(dev: DeviceBuffer | int, n: int, index: BuiltIndex, compiled,
+                    capacity: int = 8, stream=None):
+    """Run several compiled queries in one launch.  `compiled`: list of
+    (ir, max_depth, slots).  Returns, per query, spans [lines, cap, 2] (-1 =
+    none) and, for extension bytecode, counts (else None)."""
+    ptr = dev.ptr if isinstance(dev, DeviceBuffer) else int(dev)
+    lines = index.num_lines
+    keep, descs = [], (QueryDescT * max(len(compiled), 1))()
+    for i, (ir, _d, slots) in enumerate(compiled):
+        var = uses_extension_opcodes(ir)
+        cap = capacity if var else slots
+        d_ir = DeviceBuffer.from_host(np.frombuffer(ir, dtype=np.uint8), stream)
+        sp = DeviceBuffer(max(lines * cap * 16, 16))
+        ct = DeviceBuffer(max(lines * 8, 8)) if var else None
+        keep.append((d_ir, sp, ct, cap, var))
+        descs[i] = QueryDescT(d_ir.ptr, len(ir), cap, int(var), sp.ptr, ct.ptr if ct else None)
+    check(lib().gpj_query_run_batch(ptr, int(n), ctypes.byref(index.raw), descs, len(compiled),
+                                    lines, stream), "query batch")
+    check(lib().gpj_stream_sync(stream))
+    out = []
+    for d_ir, sp, ct, cap, var in keep:
+        spans = sp.to_host(np.int64, lines * cap * 2, stream).reshape(lines, cap, 2)
+        counts = None
+        if var:
+            counts = ct.to_host(np.int64, lines, stream)
+            spans[np.arange(cap)[None, :] >= counts[:, None]] = -1
+        out.append((spans, counts))
+    return out
 
 
 def debug_escape_quote(dev: DeviceBuffer | int, n: int, stream=None):

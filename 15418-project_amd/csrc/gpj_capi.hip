@@ -293,42 +293,55 @@ gpj_status gpj_debug_escape_quote(const uint8_t *d_in, uint64_t n, uint64_t *d_e
   return GPJ_OK;
 }
 
-namespace {
-gpj_status launch_query(const uint8_t *d_in, uint64_t n, const gpj_index *index,
-                        const uint8_t *d_ir, uint32_t ir_len, int32_t cap, int64_t max_lines,
-                        int32_t variable, int64_t *d_spans, int64_t *d_counts, void *stream) {
+gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                               const gpj_query_desc *queries, int32_t num_queries,
+                               int64_t max_lines, void *stream) {
   if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null index");
-  if (ir_len == 0 || ir_len > GPJ_MAX_QUERY_IR_BYTES)
-    return fail(GPJ_ERR_QUERY_EXECUTION, "Compiled query IR exceeds the 4096-byte IR buffer");
-  if (cap < 1) return fail(GPJ_ERR_QUERY_EXECUTION, "query needs at least one result slot");
-  if (max_lines <= 0 || n == 0) return GPJ_OK;
-  if (!d_in || !d_ir || !d_spans || !index->d_newline || !index->d_num_lines ||
-      !index->d_string || !index->d_leveled || (variable && !d_counts))
+  if (num_queries < 0 || num_queries > GPJ_MAX_BATCH_QUERIES || (num_queries > 0 && !queries))
+    return fail(GPJ_ERR_INVALID_ARGUMENT,
+                "query batch holds 0.." + std::to_string(GPJ_MAX_BATCH_QUERIES) + " queries");
+  gpj::QueryBatch batch{};
+  uint32_t total_ir = 0;
+  for (int32_t q = 0; q < num_queries; ++q) {
+    const gpj_query_desc &d = queries[q];
+    if (d.ir_len == 0 || d.ir_len > GPJ_MAX_QUERY_IR_BYTES)
+      return fail(GPJ_ERR_QUERY_EXECUTION, "Compiled query IR exceeds the 4096-byte IR buffer");
+    if (d.capacity < 1) return fail(GPJ_ERR_QUERY_EXECUTION, "query needs at least one result slot");
+    if (!d.d_ir || !d.d_spans || (d.variable && !d.d_counts))
+      return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null bytecode or result buffer");
+    total_ir += d.ir_len;
+    batch.q[q] = gpj::QuerySlot{d.d_ir, d.ir_len, d.capacity, d.variable ? 1 : 0, d.d_spans,
+                                d.variable ? d.d_counts : nullptr};
+  }
+  if (total_ir > gpj::kBatchIrBytes)
+    return fail(GPJ_ERR_QUERY_EXECUTION, "query batch bytecode exceeds 8192 bytes; split it");
+  batch.count = num_queries;
+  if (max_lines <= 0 || n == 0 || num_queries == 0) return GPJ_OK;
+  if (!d_in || !index->d_newline || !index->d_num_lines || !index->d_string || !index->d_leveled)
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "Query execution requires GPU-resident newline, string, and bitmap indices");
   const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_lines, 256), 1u << 20);
-  hipLaunchKernelGGL(gpj::query_kernel, dim3((uint32_t)blocks), dim3(256), 0, as_stream(stream),
-                     d_in, (int64_t)n, index->d_newline, index->d_num_lines, max_lines,
-                     index->d_string, index->d_leveled, index->level_size, index->num_levels,
-                     d_ir, ir_len, cap, variable, d_spans, d_counts);
-  GPJ_HIP(hipGetLastError(), "query_kernel launch");
+  hipLaunchKernelGGL(gpj::query_batch_kernel, dim3((uint32_t)blocks), dim3(256), 0,
+                     as_stream(stream), d_in, (int64_t)n, index->d_newline, index->d_num_lines,
+                     max_lines, index->d_string, index->d_leveled, index->level_size,
+                     index->num_levels, batch);
+  GPJ_HIP(hipGetLastError(), "query_batch_kernel launch");
   return GPJ_OK;
 }
-} // namespace
 
 gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                          const uint8_t *d_ir, uint32_t ir_len, int32_t num_results,
                          int64_t max_lines, int64_t *d_spans, void *stream) {
-  return launch_query(d_in, n, index, d_ir, ir_len, num_results, max_lines, 0, d_spans, nullptr,
-                      stream);
+  const gpj_query_desc d{d_ir, ir_len, num_results, 0, d_spans, nullptr};
+  return gpj_query_run_batch(d_in, n, index, &d, 1, max_lines, stream);
 }
 
 gpj_status gpj_query_run_ext(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                              const uint8_t *d_ir, uint32_t ir_len, int32_t capacity,
                              int64_t max_lines, int64_t *d_spans, int64_t *d_counts,
                              void *stream) {
-  return launch_query(d_in, n, index, d_ir, ir_len, capacity, max_lines, 1, d_spans, d_counts,
-                      stream);
+  const gpj_query_desc d{d_ir, ir_len, capacity, 1, d_spans, d_counts};
+  return gpj_query_run_batch(d_in, n, index, &d, 1, max_lines, stream);
 }
 
 gpj_status gpj_query_compile(const char *query, uint8_t *ir, uint32_t ir_cap,

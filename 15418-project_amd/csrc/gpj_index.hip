@@ -170,6 +170,9 @@ constexpr uint64_t kFlagShift = 62;
 constexpr uint64_t kFlagAggregate = 1;
 constexpr uint64_t kFlagInclusive = 2;
 constexpr uint32_t kMaxSpins = 1u << 24;
+// (esc, str) -> (esc, str), depth delta 0, for all four inputs
+constexpr uint64_t kIdentityTable =
+    (0ull << 0) | (2ull << 10) | (1ull << 20) | (3ull << 30);
 
 struct Carry {
   uint32_t esc, str;
@@ -353,9 +356,10 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   }
   __syncthreads();
 
-  // 5. publish the tile aggregate and resolve the exclusive prefix (thread 0)
-  if (t == 0) {
+  // 5. publish the tile aggregate and resolve the exclusive prefix (wave 0)
+  if (wv == 0) {
     uint64_t sum = 0;
+#pragma unroll
     for (uint32_t k = 0; k < kTileWords / 64; ++k) sum += s_wsum[k];
     const uint32_t D0 = (uint32_t)((sum & 0xFFFFu) - 64u * kTileWords) & 0xFFu;
     const uint32_t TOT = (uint32_t)(((sum >> 16) & 0xFFFFu) - 64u * kTileWords) & 0xFFu;
@@ -363,7 +367,9 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
     const uint32_t g = (s_first >> 24) & 1u;
     const uint32_t tail = s_tail_esc;
     uint64_t tbl = 0;
+#pragma unroll
     for (uint32_t X = 0; X < 2; ++X) {
+#pragma unroll
       for (uint32_t Y = 0; Y < 2; ++Y) {
         const uint32_t eff = Y ^ (X & g);
         const uint32_t esc_out = (tail & 2u) ? X : (tail & 1u);
@@ -374,45 +380,67 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
       }
     }
     Carry excl;
-    if (tile == 0) {
-      excl.esc = 0;
-      excl.str = 0;
-      excl.depth = -1;
-      excl.nl = 0;
-    } else {
-      desc_store(&desc[tile], tbl | (NL << 40) | (kFlagAggregate << kFlagShift));
-      // identity transfer: (X,Y) -> (X,Y), delta 0
-      uint64_t acc = 0;
-      for (uint32_t s = 0; s < 4; ++s) acc |= (uint64_t)((s >> 1) | ((s & 1u) << 1)) << (s * 10);
+    excl.esc = 0;
+    excl.str = 0;
+    excl.depth = -1;
+    excl.nl = 0;
+    if (tile != 0) {
+      if (lane == 0) desc_store(&desc[tile], tbl | (NL << 40) | (kFlagAggregate << kFlagShift));
+      // Look back 64 predecessors at a time: lane i holds tile-1-i.  Compose
+      // the transfer tables newer-first up to the nearest inclusive prefix.
+      uint64_t acc = kIdentityTable;
       uint64_t acc_nl = 0;
-      int64_t p = (int64_t)tile - 1;
+      int64_t base = (int64_t)tile - 1;
       uint32_t spins = 0;
       for (;;) {
-        uint64_t d = desc_load(&desc[p]);
-        while ((d >> kFlagShift) == 0) {
-          // Bounded wait: a predecessor always publishes within microseconds;
-          // if it never does, flag the build as failed instead of hanging.
+        const int64_t p = base - (int64_t)lane;
+        uint64_t d = 0;
+        uint32_t first_incl = 64;
+        for (;;) {
+          d = (p >= 0) ? desc_load(&desc[p]) : (kFlagInclusive << kFlagShift);
+          const uint64_t incl = __ballot((d >> kFlagShift) == kFlagInclusive);
+          const uint64_t zero = __ballot((d >> kFlagShift) == 0);
+          first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+          const uint64_t needed = first_incl >= 63 ? ~0ull : ((2ull << first_incl) - 1ull);
+          if ((zero & needed) == 0) break;
           if (++spins > kMaxSpins) {
-            __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+              __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             d = kFlagInclusive << kFlagShift;
+            first_incl = 0;
             break;
           }
           __builtin_amdgcn_s_sleep(1);
-          d = desc_load(&desc[p]);
         }
-        if ((d >> kFlagShift) == kFlagInclusive) {
-          excl = tbl_apply(acc, acc_nl, decode_inclusive(d));
+        // tables of lanes [0, first_incl), identity beyond
+        uint64_t t = (lane < first_incl) ? (d & ((1ull << 40) - 1)) : kIdentityTable;
+        uint64_t nlv = (lane < first_incl) ? ((d >> 40) & ((1ull << 22) - 1)) : 0ull;
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+          const uint64_t ot = __shfl_down(t, off, 64);
+          const uint64_t on = __shfl_down(nlv, off, 64);
+          if ((lane & (2 * off - 1)) == 0 && lane + off < 64) {
+            t = tbl_compose(t, ot);
+            nlv += on;
+          }
+        }
+        const uint64_t window = __shfl(t, 0, 64);
+        acc = tbl_compose(acc, window);
+        acc_nl += __shfl(nlv, 0, 64);
+        if (first_incl < 64) {
+          const uint64_t inc = __shfl(d, (int)first_incl, 64);
+          excl = tbl_apply(acc, acc_nl, decode_inclusive(inc));
           break;
         }
-        acc = tbl_compose(acc, d & ((1ull << 40) - 1));
-        acc_nl += (d >> 40) & ((1ull << 22) - 1);
-        --p;
+        base -= 64;
       }
     }
-    const Carry incl = tbl_apply(tbl, NL, excl);
-    desc_store(&desc[tile], encode_inclusive(incl));
-    if (tile == num_tiles - 1) *num_lines_out = (int64_t)(1 + incl.nl);
-    s_carry = excl;
+    if (lane == 0) {
+      const Carry incl = tbl_apply(tbl, NL, excl);
+      desc_store(&desc[tile], encode_inclusive(incl));
+      if (tile == num_tiles - 1) *num_lines_out = (int64_t)(1 + incl.nl);
+      s_carry = excl;
+    }
   }
   __syncthreads();
   const Carry c = s_carry;

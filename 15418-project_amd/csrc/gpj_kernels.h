@@ -28,12 +28,25 @@ __global__ void newline_write_kernel(const uint8_t *in, uint64_t n, int aligned,
 __global__ void escape_quote_kernel(const uint8_t *in, uint64_t n, uint64_t words,
                                     uint64_t *escape_out, uint64_t *quote_out);
 
-__global__ void query_kernel(const uint8_t *file, int64_t n,
-                             const int64_t *newline, const int64_t *num_lines,
-                             int64_t max_lines, const uint64_t *string,
-                             const uint64_t *leveled, int64_t level_size,
-                             int32_t num_levels, const uint8_t *ir, uint32_t ir_len,
-                             int32_t num_results, int32_t variable, int64_t *result,
-                             int64_t *counts);
+constexpr int kMaxBatchQueries = 8;
+constexpr uint32_t kBatchIrBytes = 8192;
+
+struct QuerySlot {
+  const uint8_t *ir;
+  uint32_t ir_len;
+  int32_t cap;        // result slots (fixed mode) or inline capacity (variable)
+  int32_t variable;   // 0: reference fixed slots, 1: extension variable-length
+  int64_t *spans;     // [lines][cap][2]
+  int64_t *counts;    // [lines] or null (fixed mode)
+};
+struct QueryBatch {
+  QuerySlot q[kMaxBatchQueries];
+  int32_t count;
+};
+
+__global__ void query_batch_kernel(const uint8_t *file, int64_t n, const int64_t *newline,
+                                   const int64_t *num_lines, int64_t max_lines,
+                                   const uint64_t *string, const uint64_t *leveled,
+                                   int64_t level_size, int32_t num_levels, QueryBatch batch);
 
 } // namespace gpj

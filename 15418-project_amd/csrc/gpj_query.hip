@@ -26,8 +26,12 @@ enum : uint8_t {
   kMoveToEach = 10, kJumpIfNotArray = 11 // extension opcodes
 };
 constexpr int kMaxLevels = 16;
-constexpr uint32_t kMaxIr = 4096;
 
+// Per-lane interpreter context.  Two small register caches cut the number of
+// scattered global loads per record: the last aligned 8 text bytes read, and
+// the last two bitmap words read (the key search alternates between a leveled
+// row and the string mask).  Both stay valid across the queries of a batch,
+// which run over the same record.
 struct Ctx {
   const uint8_t *file;
   int64_t n;
@@ -35,12 +39,46 @@ struct Ctx {
   const uint64_t *leveled;
   int64_t level_size;
   int num_levels;
-  const uint8_t *ir; // LDS copy
+  const uint8_t *ir; // LDS copy of the current query
   uint32_t ir_len;
+  int64_t text_at;   // aligned 8-byte block held in text_val (-1: none)
+  uint64_t text_val;
+  const uint64_t *w_ptr[2];
+  uint64_t w_val[2];
+  uint32_t w_next;   // slot to replace next
 };
 
-__device__ __forceinline__ uint8_t byte_at(const Ctx &c, int64_t p) {
-  return (p >= 0 && p < c.n) ? c.file[p] : (uint8_t)0;
+// Blocks are aligned 8-byte groups of device addresses; a block that is not
+// wholly inside [file, file + n) is assembled byte by byte.
+__device__ __forceinline__ uint8_t byte_at(Ctx &c, int64_t p) {
+  if (p < 0 || p >= c.n) return 0;
+  const uintptr_t base = reinterpret_cast<uintptr_t>(c.file);
+  const int64_t blk = (int64_t)((base + (uintptr_t)p) >> 3);
+  if (blk != c.text_at) {
+    const uintptr_t lo = (uintptr_t)blk << 3;
+    if (lo >= base && lo + 8 <= base + (uintptr_t)c.n) {
+      c.text_val = *reinterpret_cast<const uint64_t *>(lo);
+    } else {
+      uint64_t v = 0;
+      for (uintptr_t a = lo; a < lo + 8; ++a)
+        if (a >= base && a < base + (uintptr_t)c.n)
+          v |= (uint64_t)(*reinterpret_cast<const uint8_t *>(a)) << (8 * (a & 7));
+      c.text_val = v;
+    }
+    c.text_at = blk;
+  }
+  return (uint8_t)(c.text_val >> (8 * ((base + (uintptr_t)p) & 7)));
+}
+
+__device__ __forceinline__ uint64_t word_at(Ctx &c, const uint64_t *p) {
+  if (p == c.w_ptr[0]) return c.w_val[0];
+  if (p == c.w_ptr[1]) return c.w_val[1];
+  const uint64_t v = *p;
+  const uint32_t s = c.w_next;
+  c.w_ptr[s] = p;
+  c.w_val[s] = v;
+  c.w_next = s ^ 1u;
+  return v;
 }
 __device__ __forceinline__ uint8_t ir_at(const Ctx &c, int64_t p) {
   return (p >= 0 && p < (int64_t)c.ir_len) ? c.ir[p] : (uint8_t)0;
@@ -50,10 +88,10 @@ __device__ __forceinline__ uint64_t through(int b) {
 }
 
 // Last set bit of `bm` in [lo, idx], or -1.
-__device__ int64_t last_bit_in(const uint64_t *bm, int64_t lo, int64_t idx) {
+__device__ int64_t last_bit_in(Ctx &c, const uint64_t *bm, int64_t lo, int64_t idx) {
   if (idx < lo || idx < 0) return -1;
   int64_t wi = idx >> 6, wb = wi << 6;
-  uint64_t word = bm[wi] & through((int)(idx - wb));
+  uint64_t word = word_at(c, bm + wi) & through((int)(idx - wb));
   while (wb + 63 >= lo) {
     uint64_t cand = word;
     const int64_t first = lo - wb;
@@ -63,20 +101,20 @@ __device__ int64_t last_bit_in(const uint64_t *bm, int64_t lo, int64_t idx) {
     --wi;
     wb -= 64;
     if (wb + 63 < lo) break;
-    word = bm[wi];
+    word = word_at(c, bm + wi);
   }
   return -1;
 }
 
 // First structural bit of `level` in [idx, level_end], else level_end.
-__device__ int64_t next_structural(const Ctx &c, int64_t level_end, int64_t idx,
+__device__ int64_t next_structural(Ctx &c, int64_t level_end, int64_t idx,
                                    int level) {
   if (idx >= level_end) return level_end;
   if (level < 0 || level >= c.num_levels) return level_end;
   if (idx < 0) idx = 0;
   const uint64_t *row = c.leveled + (uint64_t)c.level_size * (uint64_t)level;
   int64_t wi = idx >> 6, wb = wi << 6;
-  uint64_t word = row[wi] & (~0ull << (idx - wb));
+  uint64_t word = word_at(c, row + wi) & (~0ull << (idx - wb));
   while (wb <= level_end) {
     uint64_t cand = word;
     const int64_t last = level_end - wb;
@@ -85,17 +123,17 @@ __device__ int64_t next_structural(const Ctx &c, int64_t level_end, int64_t idx,
     ++wi;
     wb += 64;
     if (wb > level_end || wi >= c.level_size) break;
-    word = row[wi];
+    word = word_at(c, row + wi);
   }
   return level_end;
 }
 
-__device__ int64_t prev_structural(const Ctx &c, int64_t level_start,
+__device__ int64_t prev_structural(Ctx &c, int64_t level_start,
                                    int64_t idx, int level) {
   if (idx <= level_start) return level_start;
   if (level < 0 || level >= c.num_levels) return level_start;
   const uint64_t *row = c.leveled + (uint64_t)c.level_size * (uint64_t)level;
-  const int64_t s = last_bit_in(row, level_start, idx);
+  const int64_t s = last_bit_in(c, row, level_start, idx);
   return s == -1 ? level_start : s;
 }
 
@@ -126,12 +164,10 @@ constexpr int kMaxFrames = 8;
 // writes slot i; any failure ends the record.  Variable mode (extension):
 // STORE appends (first `cap` stored, all counted) and a finished path
 // resumes the innermost MOVE_TO_EACH at its next element.
-__device__ int64_t run_record(const Ctx &c, int64_t line_start, int64_t line_end,
+// line_start/line_end are already trimmed to the record's first '{' and
+// last '}' (line_start < line_end).
+__device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
                               int32_t cap, bool variable, int64_t *res) {
-  while (line_end > line_start && byte_at(c, line_end) != '}') --line_end;
-  while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
-  if (line_start == line_end) return 0;
-
   int64_t idx = line_start;
   int lvl = 0;
   int64_t qp = 0;
@@ -194,7 +230,7 @@ __device__ int64_t run_record(const Ctx &c, int64_t line_start, int64_t line_end
           ++idx;
           idx = next_structural(c, level_end[lvl], idx, lvl);
           if (byte_at(c, idx) != ':') break;
-          const int64_t string_end = last_bit_in(c.string, line_start + 1, idx - 1);
+          const int64_t string_end = last_bit_in(c, c.string, line_start + 1, idx - 1);
           const int64_t string_start = string_end - key_len;
           bool match = string_start >= line_start && byte_at(c, string_start) == '"';
           for (int32_t i = 0; match && i < key_len; ++i)
@@ -336,33 +372,54 @@ __device__ int64_t run_record(const Ctx &c, int64_t line_start, int64_t line_end
 
 } // namespace
 
-__global__ __launch_bounds__(256) void query_kernel(
+// All queries of a batch over every line: each lane takes one record and runs
+// the batch's bytecodes over it in turn, so the record's bytes and bitmap
+// words are fetched once for the whole batch.
+__global__ __launch_bounds__(256) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
     const int64_t *__restrict__ newline, const int64_t *__restrict__ num_lines,
     int64_t max_lines, const uint64_t *__restrict__ string,
     const uint64_t *__restrict__ leveled, int64_t level_size, int32_t num_levels,
-    const uint8_t *__restrict__ ir, uint32_t ir_len, int32_t num_results,
-    int32_t variable, int64_t *__restrict__ result, int64_t *__restrict__ counts) {
-  __shared__ uint8_t s_ir[kMaxIr];
-  const uint32_t len = ir_len < kMaxIr ? ir_len : kMaxIr;
-  for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) s_ir[i] = ir[i];
+    QueryBatch batch) {
+  __shared__ uint8_t s_ir[kBatchIrBytes];
+  uint32_t off[kMaxBatchQueries];
+  uint32_t total = 0;
+  for (int q = 0; q < batch.count; ++q) {
+    off[q] = total;
+    for (uint32_t i = threadIdx.x; i < batch.q[q].ir_len; i += blockDim.x)
+      s_ir[total + i] = batch.q[q].ir[i];
+    total += batch.q[q].ir_len;
+  }
   __syncthreads();
-  const Ctx c{file, n, string, leveled, level_size, num_levels, s_ir, len};
-  const int64_t total = *num_lines;
-  const int64_t lines = total < max_lines ? total : max_lines;
+  const int64_t all = *num_lines;
+  const int64_t lines = all < max_lines ? all : max_lines;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; line < lines;
        line += stride) {
-    int64_t *res = result + line * 2 * (int64_t)num_results;
-    for (int i = 0; i < 2 * num_results; ++i) res[i] = -1;
-    const int64_t line_start = newline[line];
-    int64_t line_end = (line + 1 < total) ? newline[line + 1] : n - 1;
-    int64_t found = 0;
-    if (line_start >= 0 && line_start < n && line_end >= 0) {
+    Ctx c{file, n, string, leveled, level_size, num_levels, nullptr, 0, -1, 0,
+          {nullptr, nullptr}, {0, 0}, 0};
+    int64_t line_start = newline[line];
+    int64_t line_end = (line + 1 < all) ? newline[line + 1] : n - 1;
+    bool record = line_start >= 0 && line_start < n && line_end >= 0;
+    if (record) {
       if (line_end >= n) line_end = n - 1;
-      found = run_record(c, line_start, line_end, num_results, variable != 0, res);
+      while (line_end > line_start && byte_at(c, line_end) != '}') --line_end;
+      while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
+      record = line_start != line_end;
     }
-    if (counts) counts[line] = found;
+    for (int q = 0; q < batch.count; ++q) {
+      const QuerySlot &qs = batch.q[q];
+      int64_t *res = qs.spans + line * 2 * (int64_t)qs.cap;
+      if (!qs.variable)
+        for (int i = 0; i < 2 * qs.cap; ++i) res[i] = -1;
+      int64_t found = 0;
+      if (record) {
+        c.ir = s_ir + off[q];
+        c.ir_len = qs.ir_len;
+        found = run_record(c, line_start, line_end, qs.cap, qs.variable != 0, res);
+      }
+      if (qs.counts) qs.counts[line] = found;
+    }
   }
 }
 

@@ -6,6 +6,7 @@
 #include "gpjson/engine.hpp"
 #include "gpjson/query/error.hpp"
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -61,39 +62,63 @@ BatchQueryResult QueryExecutor::execute_batch(const BatchCompiledQuery &compiled
 
   const char *host = static_cast<const char *>(partition.host_bytes());
   const size_t base = partition.global_start_offset();
-  for (size_t q = 0; q < compiled.size(); ++q) {
-    const CompiledQuery &cq = compiled.queries()[q];
-    validate(cq, idx);
-    const int slots = static_cast<int>(cq.num_result_slots());
-    device::DeviceArray d_ir(cq.ir_bytes().size());
-    d_ir.copy_from_host(cq.ir_bytes().data(), cq.ir_bytes().size());
-    const size_t count = static_cast<size_t>(num_lines) * slots * 2;
-    device::DeviceArray d_spans(count * sizeof(int64_t));
-    device::check(gpj_query_run(static_cast<const uint8_t *>(partition.device_bytes()),
-                                partition.size_bytes(), &view, d_ir.as<uint8_t>(),
-                                static_cast<uint32_t>(cq.ir_bytes().size()), slots, num_lines,
-                                d_spans.as<int64_t>(), nullptr),
-                  "execute_query_kernel launch");
-    std::vector<int64_t> spans(count, -1);
-    d_spans.copy_to_host(spans.data(), count * sizeof(int64_t));
-    for (int64_t line = 0; line < num_lines; ++line) {
-      LineQueryResult lr;
-      for (int s = 0; s < slots; ++s) {
-        const int64_t a = spans[(static_cast<size_t>(line) * slots + s) * 2];
-        const int64_t b = spans[(static_cast<size_t>(line) * slots + s) * 2 + 1];
-        if (a < 0 || b < 0) {
-          lr.add_offset(QueryOffset{});
-          continue;
+  // Queries run GPJ_MAX_BATCH_QUERIES at a time in one launch each (the
+  // record is read once per group); results are the per-query results.
+  for (size_t first = 0; first < compiled.size(); first += GPJ_MAX_BATCH_QUERIES) {
+    const size_t last = std::min(compiled.size(), first + GPJ_MAX_BATCH_QUERIES);
+    std::vector<device::DeviceArray> irs, spans;
+    std::vector<gpj_query_desc> descs;
+    uint32_t ir_total = 0;
+    for (size_t q = first; q < last; ++q) {
+      const CompiledQuery &cq = compiled.queries()[q];
+      validate(cq, idx);
+      ir_total += static_cast<uint32_t>(cq.ir_bytes().size());
+      irs.emplace_back(cq.ir_bytes().size());
+      irs.back().copy_from_host(cq.ir_bytes().data(), cq.ir_bytes().size());
+      const int slots = static_cast<int>(cq.num_result_slots());
+      spans.emplace_back(static_cast<size_t>(num_lines) * slots * 2 * sizeof(int64_t));
+      descs.push_back(gpj_query_desc{irs.back().as<uint8_t>(),
+                                     static_cast<uint32_t>(cq.ir_bytes().size()), slots, 0,
+                                     spans.back().as<int64_t>(), nullptr});
+    }
+    if (ir_total > 8192) {
+      // oversized bytecode: one launch per query
+      for (size_t i = 0; i < descs.size(); ++i)
+        device::check(gpj_query_run_batch(static_cast<const uint8_t *>(partition.device_bytes()),
+                                          partition.size_bytes(), &view, &descs[i], 1, num_lines,
+                                          nullptr),
+                      "execute_query_kernel launch");
+    } else {
+      device::check(gpj_query_run_batch(static_cast<const uint8_t *>(partition.device_bytes()),
+                                        partition.size_bytes(), &view, descs.data(),
+                                        static_cast<int32_t>(descs.size()), num_lines, nullptr),
+                    "execute_query_kernel launch");
+    }
+    for (size_t q = first; q < last; ++q) {
+      const int slots = descs[q - first].capacity;
+      const size_t count = static_cast<size_t>(num_lines) * slots * 2;
+      std::vector<int64_t> host_spans(count, -1);
+      spans[q - first].copy_to_host(host_spans.data(), count * sizeof(int64_t));
+      for (int64_t line = 0; line < num_lines; ++line) {
+        LineQueryResult lr;
+        for (int s = 0; s < slots; ++s) {
+          const int64_t a = host_spans[(static_cast<size_t>(line) * slots + s) * 2];
+          const int64_t b = host_spans[(static_cast<size_t>(line) * slots + s) * 2 + 1];
+          if (a < 0 || b < 0) {
+            lr.add_offset(QueryOffset{});
+            continue;
+          }
+          if (a > b || static_cast<size_t>(b) > partition.size_bytes())
+            throw error::query::QueryExecutionError(
+                "Query kernel produced an invalid result range");
+          QueryOffset off;
+          off.start = base + static_cast<size_t>(a);
+          off.end = base + static_cast<size_t>(b);
+          off.json_text = std::string(host + a, static_cast<size_t>(b - a));
+          lr.add_offset(std::move(off));
         }
-        if (a > b || static_cast<size_t>(b) > partition.size_bytes())
-          throw error::query::QueryExecutionError("Query kernel produced an invalid result range");
-        QueryOffset off;
-        off.start = base + static_cast<size_t>(a);
-        off.end = base + static_cast<size_t>(b);
-        off.json_text = std::string(host + a, static_cast<size_t>(b - a));
-        lr.add_offset(std::move(off));
+        batch.add_line_result(q, std::move(lr));
       }
-      batch.add_line_result(q, std::move(lr));
     }
   }
   return batch;

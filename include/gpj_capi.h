@@ -134,13 +134,35 @@ gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index
  * counterpart in the reference, whose compiler rejects `[*]` at
  * src/gpjson/query/query_compiler.cpp:287-290): a record may yield any number
  * of results, in execution order.  d_spans receives the first `capacity`
- * {start, end} pairs of each line ([max_lines][capacity][2], -1 padded) and
- * d_counts the full count per line; a caller re-runs with a larger capacity
- * when any count exceeds it.  Stream-ordered; allocates nothing. */
+ * {start, end} pairs of each line ([max_lines][capacity][2]; entries past
+ * the line's count are left untouched) and d_counts the full count per line;
+ * a caller re-runs with a larger capacity when any count exceeds it.
+ * Stream-ordered; allocates nothing. */
 gpj_status gpj_query_run_ext(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                              const uint8_t *d_ir, uint32_t ir_len, int32_t capacity,
                              int64_t max_lines, int64_t *d_spans, int64_t *d_counts,
                              void *stream);
+
+/* A batch of compiled queries run over the same index in one launch: each
+ * record is read once for all of them (the reference launches one kernel per
+ * query, query_executor_host.cu:179-237; results are the same).  Up to
+ * GPJ_MAX_BATCH_QUERIES queries with at most 8192 bytecode bytes in total.
+ * variable = 0: reference fixed slots (capacity = slot count, -1 = no match);
+ * variable = 1: extension mode as in gpj_query_run_ext (entries past a line's
+ * count are left untouched).  `queries` is a host array.  Stream-ordered;
+ * allocates nothing. */
+#define GPJ_MAX_BATCH_QUERIES 8
+typedef struct {
+  const uint8_t *d_ir;
+  uint32_t ir_len;
+  int32_t capacity;
+  int32_t variable;
+  int64_t *d_spans;  /* [max_lines][capacity][2] */
+  int64_t *d_counts; /* [max_lines], variable mode only */
+} gpj_query_desc;
+gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                               const gpj_query_desc *queries, int32_t num_queries,
+                               int64_t max_lines, void *stream);
 
 /* ---- query compiler: replaces QueryCompiler::compile
  *      (src/gpjson/query/query_compiler.cpp:467-477).  Host only.

@@ -40,3 +40,24 @@ def test_extension_on_generated_throughput_records(gpu, orc):
         np.testing.assert_array_equal(counts, wc)
         np.testing.assert_array_equal(spans, ws)
         assert counts.sum() > 0
+
+
+def test_batch_launch_equals_single_launches(gpu, orc):
+    data = bytes(gpu.generate(gpu.SCHEMA_NDJSON_1K, 2 << 20, 0x15418002))
+    queries = ["$.user.lang", "$.e[0].f", "$.h[*].t", '$.h[?(@.k == "v")].id', "$.arr[0:2]",
+               "$.h[1].k"]
+    compiled = [gpu.compile_query(q, extensions=True) for q in queries]
+    levels = max(c[1] for c in compiled)
+    dev = gpu.DeviceBuffer.from_host(np.frombuffer(data, dtype=np.uint8))
+    idx = gpu.build_index(dev, len(data), levels)
+    batch = gpu.run_query_batch(dev, len(data), idx, compiled, capacity=8)
+    oidx = orc.build_index(data, levels)
+    for (ir, _d, slots), (spans, counts), q in zip(compiled, batch, queries):
+        print(f"[batch] {q}", flush=True)
+        if counts is None:
+            np.testing.assert_array_equal(spans, gpu.run_query(dev, len(data), idx, ir, slots))
+            np.testing.assert_array_equal(spans, orc.query(data, oidx, ir, slots))
+        else:
+            ws, wc = orc.query_ext(data, oidx, ir, spans.shape[1])
+            np.testing.assert_array_equal(counts, wc)
+            np.testing.assert_array_equal(spans, ws)
