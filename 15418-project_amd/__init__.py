@@ -84,6 +84,7 @@ class QueryDescT(ctypes.Structure):
         ("ir_len", ctypes.c_uint32),
         ("capacity", ctypes.c_int32),
         ("variable", ctypes.c_int32),
+        ("max_level", ctypes.c_int32),
         ("d_spans", ctypes.c_void_p),
         ("d_counts", ctypes.c_void_p),
     ]
@@ -369,14 +370,15 @@ def run_query_batch(dev: DeviceBuffer | int, n: int, index: BuiltIndex, compiled
     ptr = dev.ptr if isinstance(dev, DeviceBuffer) else int(dev)
     lines = index.num_lines
     keep, descs = [], (QueryDescT * max(len(compiled), 1))()
-    for i, (ir, _d, slots) in enumerate(compiled):
+    for i, (ir, depth, slots) in enumerate(compiled):
         var = uses_extension_opcodes(ir)
         cap = capacity if var else slots
         d_ir = DeviceBuffer.from_host(np.frombuffer(ir, dtype=np.uint8), stream)
         sp = DeviceBuffer(max(lines * cap * 16, 16))
         ct = DeviceBuffer(max(lines * 8, 8)) if var else None
         keep.append((d_ir, sp, ct, cap, var))
-        descs[i] = QueryDescT(d_ir.ptr, len(ir), cap, int(var), sp.ptr, ct.ptr if ct else None)
+        descs[i] = QueryDescT(d_ir.ptr, len(ir), cap, int(var), int(depth), sp.ptr,
+                              ct.ptr if ct else None)
     check(lib().gpj_query_run_batch(ptr, int(n), ctypes.byref(index.raw), descs, len(compiled),
                                     lines, stream), "query batch")
     check(lib().gpj_stream_sync(stream))

@@ -316,15 +316,26 @@ gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index 
   if (total_ir > gpj::kBatchIrBytes)
     return fail(GPJ_ERR_QUERY_EXECUTION, "query batch bytecode exceeds 8192 bytes; split it");
   batch.count = num_queries;
+  // smallest interpreter variant that holds every query's deepest level
+  int deepest = 0;
+  for (int32_t q = 0; q < num_queries; ++q) {
+    const int lv = queries[q].max_level > 0 ? queries[q].max_level : 15;
+    deepest = std::max(deepest, std::min(lv, 15));
+  }
   if (max_lines <= 0 || n == 0 || num_queries == 0) return GPJ_OK;
   if (!d_in || !index->d_newline || !index->d_num_lines || !index->d_string || !index->d_leveled)
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "Query execution requires GPU-resident newline, string, and bitmap indices");
   const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_lines, 256), 1u << 20);
-  hipLaunchKernelGGL(gpj::query_batch_kernel, dim3((uint32_t)blocks), dim3(256), 0,
-                     as_stream(stream), d_in, (int64_t)n, index->d_newline, index->d_num_lines,
-                     max_lines, index->d_string, index->d_leveled, index->level_size,
-                     index->num_levels, batch);
+#define GPJ_QLAUNCH(LV)                                                                   \
+  hipLaunchKernelGGL(gpj::query_batch_kernel<LV>, dim3((uint32_t)blocks), dim3(256), 0,      \
+                     as_stream(stream), d_in, (int64_t)n, index->d_newline,                  \
+                     index->d_num_lines, max_lines, index->d_string, index->d_leveled,       \
+                     index->level_size, index->num_levels, batch)
+  if (deepest + 1 <= 4) GPJ_QLAUNCH(4);
+  else if (deepest + 1 <= 8) GPJ_QLAUNCH(8);
+  else GPJ_QLAUNCH(16);
+#undef GPJ_QLAUNCH
   GPJ_HIP(hipGetLastError(), "query_batch_kernel launch");
   return GPJ_OK;
 }
@@ -332,7 +343,7 @@ gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index 
 gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                          const uint8_t *d_ir, uint32_t ir_len, int32_t num_results,
                          int64_t max_lines, int64_t *d_spans, void *stream) {
-  const gpj_query_desc d{d_ir, ir_len, num_results, 0, d_spans, nullptr};
+  const gpj_query_desc d{d_ir, ir_len, num_results, 0, 0, d_spans, nullptr};
   return gpj_query_run_batch(d_in, n, index, &d, 1, max_lines, stream);
 }
 
@@ -340,7 +351,7 @@ gpj_status gpj_query_run_ext(const uint8_t *d_in, uint64_t n, const gpj_index *i
                              const uint8_t *d_ir, uint32_t ir_len, int32_t capacity,
                              int64_t max_lines, int64_t *d_spans, int64_t *d_counts,
                              void *stream) {
-  const gpj_query_desc d{d_ir, ir_len, capacity, 1, d_spans, d_counts};
+  const gpj_query_desc d{d_ir, ir_len, capacity, 1, 0, d_spans, d_counts};
   return gpj_query_run_batch(d_in, n, index, &d, 1, max_lines, stream);
 }
 

@@ -228,14 +228,6 @@ __device__ __forceinline__ void desc_store(uint64_t *p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Add a segment's structural bits to the row of `depth`.
-template <int LMAX>
-__device__ __forceinline__ void add_row(uint64_t (&rows)[LMAX], int depth,
-                                        uint64_t bits) {
-#pragma unroll
-  for (int k = 0; k < LMAX; ++k) rows[k] |= (k == depth) ? bits : 0ull;
-}
-
 } // namespace
 
 // Clears the tile descriptors and the tile counter (kept out of
@@ -271,6 +263,7 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   __shared__ uint64_t s_wsum[kTileWords / 64];
   __shared__ uint32_t s_wscan[kTileWords / 64];
   __shared__ uint32_t s_first;              // k | lead_k << 16 | g << 24
+  __shared__ uint64_t s_rows[LMAX][kTileWords]; // per-lane leveled words
   __shared__ uint32_t s_tail_esc;           // tile escape out (X=0) | all-bs << 1
   __shared__ Carry s_carry;
 
@@ -412,21 +405,23 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
           }
           __builtin_amdgcn_s_sleep(1);
         }
-        // tables of lanes [0, first_incl), identity beyond
-        uint64_t t = (lane < first_incl) ? (d & ((1ull << 40) - 1)) : kIdentityTable;
-        uint64_t nlv = (lane < first_incl) ? ((d >> 40) & ((1ull << 22) - 1)) : 0ull;
-#pragma unroll
-        for (uint32_t off = 1; off < 64; off <<= 1) {
-          const uint64_t ot = __shfl_down(t, off, 64);
-          const uint64_t on = __shfl_down(nlv, off, 64);
-          if ((lane & (2 * off - 1)) == 0 && lane + off < 64) {
-            t = tbl_compose(t, ot);
-            nlv += on;
+        // tables of lanes [0, first_incl), identity beyond; only as many
+        // reduction steps as the window needs (none when the nearest
+        // predecessor is already inclusive)
+        if (first_incl > 0) {
+          uint64_t t = (lane < first_incl) ? (d & ((1ull << 40) - 1)) : kIdentityTable;
+          uint64_t nlv = (lane < first_incl) ? ((d >> 40) & ((1ull << 22) - 1)) : 0ull;
+          for (uint32_t off = 1; off < first_incl; off <<= 1) {
+            const uint64_t ot = __shfl_down(t, off, 64);
+            const uint64_t on = __shfl_down(nlv, off, 64);
+            if ((lane & (2 * off - 1)) == 0 && lane + off < 64) {
+              t = tbl_compose(t, ot);
+              nlv += on;
+            }
           }
+          acc = tbl_compose(acc, __shfl(t, 0, 64));
+          acc_nl += __shfl(nlv, 0, 64);
         }
-        const uint64_t window = __shfl(t, 0, 64);
-        acc = tbl_compose(acc, window);
-        acc_nl += __shfl(nlv, 0, 64);
         if (first_incl < 64) {
           const uint64_t inc = __shfl(d, (int)first_incl, 64);
           excl = tbl_apply(acc, acc_nl, decode_inclusive(inc));
@@ -474,43 +469,45 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
       (int)(int8_t)(uint8_t)((uint32_t)c.depth + (excl_v & 0xFFFFu) - 64u * t);
   const uint64_t nl_base = 1 + c.nl + (excl_v >> 16);
 
-  if (w >= words) return;
-
-  // 8. leveled rows: walk the depth changes of this word
-  uint64_t rows[LMAX];
+  // 8. leveled rows: walk this word's depth changes in order; every
+  //    structural bit between two changes belongs to the current depth
+  //    (an opener's own bit to the depth it opens, a closer's to the depth
+  //    it closes).  Rows accumulate in this lane's LDS slots.
+  const bool live = w < words;
+  if (live) {
 #pragma unroll
-  for (int k = 0; k < LMAX; ++k) rows[k] = 0;
-  {
+    for (int k = 0; k < LMAX; ++k) s_rows[k][t] = 0;
     const uint64_t E = O | C | P;
     uint64_t chg = O | C;
+    uint64_t done = 0;
     int depth = depth_in;
-    int seg_start = 0;
     while (chg) {
-      const int p = __builtin_ctzll(chg);
-      chg &= chg - 1;
-      const bool is_open = (O >> p) & 1ull;
-      const int b = is_open ? p : p + 1;
-      const uint64_t seg = low_mask(b) & ~low_mask(seg_start);
-      add_row<LMAX>(rows, depth, E & seg);
-      depth = (int)(int8_t)(uint8_t)(depth + (is_open ? 1 : -1));
-      seg_start = b;
+      const uint64_t low = chg & (0ull - chg);
+      chg ^= low;
+      const bool open = (O & low) != 0;
+      const uint64_t upto = open ? (low - 1) : (low | (low - 1));
+      const uint64_t seg = upto & ~done & E;
+      done = upto;
+      if ((unsigned)depth < (unsigned)LMAX && seg) s_rows[depth][t] |= seg;
+      depth = (int)(int8_t)(uint8_t)(depth + (open ? 1 : -1));
     }
-    add_row<LMAX>(rows, depth, E & ~low_mask(seg_start));
-  }
-  string_out[w] = S;
+    const uint64_t seg = E & ~done;
+    if ((unsigned)depth < (unsigned)LMAX && seg) s_rows[depth][t] |= seg;
+    string_out[w] = S;
 #pragma unroll
-  for (int k = 0; k < LMAX; ++k)
-    if (k < levels) leveled_out[(uint64_t)k * words + w] = rows[k];
+    for (int k = 0; k < LMAX; ++k)
+      if (k < levels) leveled_out[(uint64_t)k * words + w] = s_rows[k][t];
 
-  // 9. newline positions
-  if (tile == 0 && t == 0 && newline_cap > 0) newline_out[0] = 0;
-  uint64_t nlm = m.newline;
-  uint64_t o = nl_base;
-  while (nlm) {
-    const int b = __builtin_ctzll(nlm);
-    nlm &= nlm - 1;
-    if ((int64_t)o < newline_cap) newline_out[o] = (int64_t)(pos + (uint64_t)b);
-    ++o;
+    // 9. newline positions
+    if (tile == 0 && t == 0 && newline_cap > 0) newline_out[0] = 0;
+    uint64_t nlm = m.newline;
+    int64_t o = (int64_t)nl_base;
+    while (nlm) {
+      const int b = __builtin_ctzll(nlm);
+      nlm &= nlm - 1;
+      if (o < newline_cap) newline_out[o] = (int64_t)(pos + (uint64_t)b);
+      ++o;
+    }
   }
 }
 

@@ -44,6 +44,7 @@ struct QueryBatch {
   int32_t count;
 };
 
+template <int MAXLV>
 __global__ void query_batch_kernel(const uint8_t *file, int64_t n, const int64_t *newline,
                                    const int64_t *num_lines, int64_t max_lines,
                                    const uint64_t *string, const uint64_t *leveled,

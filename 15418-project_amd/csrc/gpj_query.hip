@@ -25,7 +25,7 @@ enum : uint8_t {
   kResetPosition = 8, kExpressionStringEquals = 9,
   kMoveToEach = 10, kJumpIfNotArray = 11 // extension opcodes
 };
-constexpr int kMaxLevels = 16;
+constexpr int kMaxFrames = 8; // element iterations per record (as the oracle)
 
 // Per-lane interpreter context.  Two small register caches cut the number of
 // scattered global loads per record: the last aligned 8 text bytes read, and
@@ -43,9 +43,9 @@ struct Ctx {
   uint32_t ir_len;
   int64_t text_at;   // aligned 8-byte block held in text_val (-1: none)
   uint64_t text_val;
-  const uint64_t *w_ptr[2];
-  uint64_t w_val[2];
-  uint32_t w_next;   // slot to replace next
+  const uint64_t *w_ptr0, *w_ptr1;
+  uint64_t w_val0, w_val1;
+  bool w_next1;      // slot to replace next
 };
 
 // Blocks are aligned 8-byte groups of device addresses; a block that is not
@@ -71,13 +71,17 @@ __device__ __forceinline__ uint8_t byte_at(Ctx &c, int64_t p) {
 }
 
 __device__ __forceinline__ uint64_t word_at(Ctx &c, const uint64_t *p) {
-  if (p == c.w_ptr[0]) return c.w_val[0];
-  if (p == c.w_ptr[1]) return c.w_val[1];
+  if (p == c.w_ptr0) return c.w_val0;
+  if (p == c.w_ptr1) return c.w_val1;
   const uint64_t v = *p;
-  const uint32_t s = c.w_next;
-  c.w_ptr[s] = p;
-  c.w_val[s] = v;
-  c.w_next = s ^ 1u;
+  if (c.w_next1) {
+    c.w_ptr1 = p;
+    c.w_val1 = v;
+  } else {
+    c.w_ptr0 = p;
+    c.w_val0 = v;
+  }
+  c.w_next1 = !c.w_next1;
   return v;
 }
 __device__ __forceinline__ uint8_t ir_at(const Ctx &c, int64_t p) {
@@ -149,16 +153,32 @@ __device__ __forceinline__ int32_t read_varint(const Ctx &c, int64_t &qp) {
   return (int32_t)(value | (byte << shift));
 }
 
+// Fixed-size per-lane array indexed by a lane-varying value.  Every access is
+// an unrolled compare/select over the N elements, so the array lives in
+// registers (a plain indexed array would be placed in scratch memory).
+template <typename T, int N> struct RegArray {
+  T v[N];
+  __device__ __forceinline__ T get(int i) const {
+    T r = v[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) r = (k == i) ? v[k] : r;
+    return r;
+  }
+  __device__ __forceinline__ void set(int i, T x) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = (k == i) ? x : v[k];
+  }
+  __device__ __forceinline__ void fill(T x) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = x;
+  }
+};
+
 // One element iteration of MOVE_TO_EACH (extension): the array's level, the
 // separator before the current element, where the path restarts, and the
 // mark-stack depth to restore.
-struct Frame {
-  int lvl;
-  int64_t sep;
-  int64_t qp;
-  int mi;
-};
-constexpr int kMaxFrames = 8;
+// (packed into registers: level, separator position, restart position,
+// mark-stack depth)
 
 // Interprets the bytecode over one record.  Fixed mode (reference): STORE i
 // writes slot i; any failure ends the record.  Variable mode (extension):
@@ -166,21 +186,31 @@ constexpr int kMaxFrames = 8;
 // resumes the innermost MOVE_TO_EACH at its next element.
 // line_start/line_end are already trimmed to the record's first '{' and
 // last '}' (line_start < line_end).
+// MAXLV bounds the interpreter's level (the host derives it from the
+// bytecode); levels, marks and element iterations each use at most MAXLV
+// entries.
+template <int MAXLV>
 __device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
                               int32_t cap, bool variable, int64_t *res) {
   int64_t idx = line_start;
   int lvl = 0;
   int64_t qp = 0;
-  int64_t level_end[kMaxLevels];
-  int64_t curr_index[kMaxLevels];
-  int64_t marked_pos[kMaxLevels];
-  int marked_lvl[kMaxLevels];
-  Frame fr[kMaxFrames];
-  level_end[0] = line_end;
-#pragma unroll
-  for (int i = 1; i < kMaxLevels; ++i) level_end[i] = -1;
-#pragma unroll
-  for (int i = 0; i < kMaxLevels; ++i) curr_index[i] = -1;
+  RegArray<int64_t, MAXLV> level_end;
+  RegArray<int32_t, MAXLV> curr_index;
+  RegArray<int64_t, MAXLV> marked_pos;
+  RegArray<int32_t, MAXLV> marked_lvl;
+  constexpr int kF = MAXLV < kMaxFrames ? MAXLV : kMaxFrames;
+  RegArray<int32_t, kF> fr_lvl, fr_qp, fr_mi;
+  RegArray<int64_t, kF> fr_sep;
+  level_end.fill(-1);
+  level_end.set(0, line_end);
+  curr_index.fill(-1);
+  marked_pos.fill(0);
+  marked_lvl.fill(0);
+  fr_lvl.fill(0);
+  fr_qp.fill(0);
+  fr_mi.fill(0);
+  fr_sep.fill(0);
   int mi = -1;
   int nres = 0;
   int nfr = 0;
@@ -195,8 +225,8 @@ __device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
         goto stop;
       case kStoreResult: {
         if (!variable && nres >= cap) goto stop;
-        int64_t end_str = level_end[lvl];
-        while (byte_at(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
+        int64_t end_str = level_end.get(lvl);
+        while (byte_at(c, idx) == ' ' && idx < level_end.get(lvl)) ++idx;
         while (end_str > idx && byte_at(c, end_str - 1) == ' ') --end_str;
         const int64_t slot = variable ? count : nres;
         if (slot < cap) {
@@ -209,15 +239,15 @@ __device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
       }
       case kMoveUp:
         if (lvl <= 0) goto stop;
-        idx = level_end[lvl];
-        level_end[lvl] = -1;
+        idx = level_end.get(lvl);
+        level_end.set(lvl, -1);
         --lvl;
         break;
       case kMoveDown:
-        if (lvl + 1 >= kMaxLevels) goto stop;
+        if (lvl + 1 >= MAXLV) goto stop;
         ++lvl;
-        if (level_end[lvl] == -1) {
-          level_end[lvl] = next_structural(c, level_end[lvl - 1], idx + 1, lvl - 1);
+        if (level_end.get(lvl) == -1) {
+          level_end.set(lvl, next_structural(c, level_end.get(lvl - 1), idx + 1, lvl - 1));
           while (byte_at(c, idx) == ' ') ++idx;
         }
         break;
@@ -228,7 +258,7 @@ __device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
         if (byte_at(c, idx) != '{') goto stop;
         for (;;) {
           ++idx;
-          idx = next_structural(c, level_end[lvl], idx, lvl);
+          idx = next_structural(c, level_end.get(lvl), idx, lvl);
           if (byte_at(c, idx) != ':') break;
           const int64_t string_end = last_bit_in(c, c.string, line_start + 1, idx - 1);
           const int64_t string_start = string_end - key_len;
@@ -237,7 +267,7 @@ __device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
             match = ir_at(c, key_pos + i) == byte_at(c, string_start + i + 1);
           if (match) break;
           ++idx;
-          idx = next_structural(c, level_end[lvl], idx, lvl);
+          idx = next_structural(c, level_end.get(lvl), idx, lvl);
           if (byte_at(c, idx) == '}') goto stop;
         }
         ++idx;
@@ -247,78 +277,88 @@ __device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
         const int32_t target = read_varint(c, qp);
         const uint8_t ch = byte_at(c, idx);
         if (ch != '[' && ch != ',' && ch != ']') goto stop;
-        if (ch == '[') curr_index[lvl] = 0;
-        else ++curr_index[lvl];
+        const int64_t lend = level_end.get(lvl);
+        int32_t ci = (ch == '[') ? 0 : curr_index.get(lvl) + 1;
         for (;;) {
-          if (curr_index[lvl] < target) {
+          if (ci < target) {
             ++idx;
-            idx = next_structural(c, level_end[lvl], idx, lvl);
-            if (byte_at(c, idx) == ',') { ++curr_index[lvl]; continue; }
+            idx = next_structural(c, lend, idx, lvl);
+            if (byte_at(c, idx) == ',') { ++ci; continue; }
+            curr_index.set(lvl, ci);
             goto stop;
           }
-          if (curr_index[lvl] > target) {
+          if (ci > target) {
             --idx;
             idx = prev_structural(c, 0, idx, lvl);
-            --curr_index[lvl];
+            --ci;
             continue;
           }
           break;
         }
+        curr_index.set(lvl, ci);
         if (byte_at(c, idx) == ']') goto stop;
         ++idx;
         break;
       }
       case kMoveToIndexReverse: {
         const int32_t target = read_varint(c, qp);
-        idx = level_end[lvl] - 1;
+        const int64_t lend = level_end.get(lvl);
+        idx = lend - 1;
         while (idx >= 0 && byte_at(c, idx) == ' ') --idx;
         const uint8_t ch = byte_at(c, idx);
         if (ch != ']' && ch != ',' && ch != '[') goto stop;
-        if (ch == ']') curr_index[lvl] = 0;
-        else ++curr_index[lvl];
+        int32_t ci = (ch == ']') ? 0 : curr_index.get(lvl) + 1;
         for (;;) {
-          if (curr_index[lvl] < target) {
+          if (ci < target) {
             --idx;
             idx = prev_structural(c, 0, idx, lvl);
-            if (byte_at(c, idx) == ',' || curr_index[lvl] + 1 == target) {
-              ++curr_index[lvl];
+            if (byte_at(c, idx) == ',' || ci + 1 == target) {
+              ++ci;
               continue;
             }
+            curr_index.set(lvl, ci);
             goto stop;
           }
-          if (curr_index[lvl] > target) {
+          if (ci > target) {
             ++idx;
-            idx = next_structural(c, level_end[lvl], idx, lvl);
-            --curr_index[lvl];
+            idx = next_structural(c, lend, idx, lvl);
+            --ci;
             continue;
           }
           break;
         }
+        curr_index.set(lvl, ci);
         ++idx;
         break;
       }
       case kMarkPosition:
-        if (mi + 1 >= kMaxLevels) goto stop;
+        if (mi + 1 >= MAXLV) goto stop;
         ++mi;
-        marked_pos[mi] = idx;
-        marked_lvl[mi] = lvl;
+        marked_pos.set(mi, idx);
+        marked_lvl.set(mi, lvl);
         break;
-      case kResetPosition:
+      case kResetPosition: {
         if (mi < 0) goto stop;
-        idx = marked_pos[mi];
-        lvl = marked_lvl[mi];
+        idx = marked_pos.get(mi);
+        lvl = marked_lvl.get(mi);
         --mi;
-        for (int i = lvl + 1; i < kMaxLevels; ++i) {
-          if (level_end[i] == -1) break;
-          level_end[i] = -1;
+        // clear the levels above, up to the first one already clear
+        bool clearing = true;
+#pragma unroll
+        for (int i = 1; i < MAXLV; ++i) {
+          if (i > lvl) {
+            if (level_end.v[i] == -1) clearing = false;
+            if (clearing) level_end.v[i] = -1;
+          }
         }
         break;
+      }
       case kExpressionStringEquals: {
         const int32_t key_len = read_varint(c, qp);
         const int64_t key_pos = qp;
         qp += key_len;
-        while (byte_at(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
-        int64_t string_end = level_end[lvl] - 1;
+        while (byte_at(c, idx) == ' ' && idx < level_end.get(lvl)) ++idx;
+        int64_t string_end = level_end.get(lvl) - 1;
         while (idx < string_end && byte_at(c, string_end) == ' ') --string_end;
         if (string_end - idx + 1 != key_len) goto stop;
         bool eq = true;
@@ -328,15 +368,15 @@ __device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
         break;
       }
       case kMoveToEach: {
-        if (!variable || byte_at(c, idx) != '[' || nfr >= kMaxFrames) goto stop;
-        const int64_t first = next_structural(c, level_end[lvl], idx + 1, lvl);
+        if (!variable || byte_at(c, idx) != '[' || nfr >= kF) goto stop;
+        const int64_t first = next_structural(c, level_end.get(lvl), idx + 1, lvl);
         int64_t p = idx + 1;
         while (p < first && byte_at(c, p) == ' ') ++p;
         if (p == first && byte_at(c, first) == ']') goto stop; // empty array
-        fr[nfr].lvl = lvl;
-        fr[nfr].sep = idx;
-        fr[nfr].qp = qp;
-        fr[nfr].mi = mi;
+        fr_lvl.set(nfr, lvl);
+        fr_sep.set(nfr, idx);
+        fr_qp.set(nfr, (int32_t)qp);
+        fr_mi.set(nfr, mi);
         ++nfr;
         ++idx;
         break;
@@ -354,15 +394,19 @@ __device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
   stop:
     for (;;) {
       if (nfr == 0) return count;
-      Frame &f = fr[nfr - 1];
-      const int64_t sep = next_structural(c, level_end[f.lvl], f.sep + 1, f.lvl);
-      if (byte_at(c, sep) == ',' && sep < level_end[f.lvl]) {
-        f.sep = sep;
-        lvl = f.lvl;
-        for (int i = lvl + 1; i < kMaxLevels; ++i) level_end[i] = -1;
-        mi = f.mi;
+      const int f = nfr - 1;
+      const int flvl = fr_lvl.get(f);
+      const int64_t lend = level_end.get(flvl);
+      const int64_t sep = next_structural(c, lend, fr_sep.get(f) + 1, flvl);
+      if (byte_at(c, sep) == ',' && sep < lend) {
+        fr_sep.set(f, sep);
+        lvl = flvl;
+#pragma unroll
+        for (int i = 1; i < MAXLV; ++i)
+          if (i > lvl) level_end.set(i, -1);
+        mi = fr_mi.get(f);
         idx = sep + 1;
-        qp = f.qp;
+        qp = fr_qp.get(f);
         break;
       }
       --nfr;
@@ -375,6 +419,7 @@ __device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
 // All queries of a batch over every line: each lane takes one record and runs
 // the batch's bytecodes over it in turn, so the record's bytes and bitmap
 // words are fetched once for the whole batch.
+template <int MAXLV>
 __global__ __launch_bounds__(256) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
     const int64_t *__restrict__ newline, const int64_t *__restrict__ num_lines,
@@ -382,13 +427,13 @@ __global__ __launch_bounds__(256) void query_batch_kernel(
     const uint64_t *__restrict__ leveled, int64_t level_size, int32_t num_levels,
     QueryBatch batch) {
   __shared__ uint8_t s_ir[kBatchIrBytes];
-  uint32_t off[kMaxBatchQueries];
-  uint32_t total = 0;
-  for (int q = 0; q < batch.count; ++q) {
-    off[q] = total;
-    for (uint32_t i = threadIdx.x; i < batch.q[q].ir_len; i += blockDim.x)
-      s_ir[total + i] = batch.q[q].ir[i];
-    total += batch.q[q].ir_len;
+  {
+    uint32_t total = 0;
+    for (int q = 0; q < batch.count; ++q) {
+      for (uint32_t i = threadIdx.x; i < batch.q[q].ir_len; i += blockDim.x)
+        s_ir[total + i] = batch.q[q].ir[i];
+      total += batch.q[q].ir_len;
+    }
   }
   __syncthreads();
   const int64_t all = *num_lines;
@@ -397,7 +442,7 @@ __global__ __launch_bounds__(256) void query_batch_kernel(
   for (int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; line < lines;
        line += stride) {
     Ctx c{file, n, string, leveled, level_size, num_levels, nullptr, 0, -1, 0,
-          {nullptr, nullptr}, {0, 0}, 0};
+          nullptr, nullptr, 0, 0, false};
     int64_t line_start = newline[line];
     int64_t line_end = (line + 1 < all) ? newline[line + 1] : n - 1;
     bool record = line_start >= 0 && line_start < n && line_end >= 0;
@@ -407,6 +452,7 @@ __global__ __launch_bounds__(256) void query_batch_kernel(
       while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
       record = line_start != line_end;
     }
+    uint32_t ir_off = 0;
     for (int q = 0; q < batch.count; ++q) {
       const QuerySlot &qs = batch.q[q];
       int64_t *res = qs.spans + line * 2 * (int64_t)qs.cap;
@@ -414,13 +460,18 @@ __global__ __launch_bounds__(256) void query_batch_kernel(
         for (int i = 0; i < 2 * qs.cap; ++i) res[i] = -1;
       int64_t found = 0;
       if (record) {
-        c.ir = s_ir + off[q];
+        c.ir = s_ir + ir_off;
         c.ir_len = qs.ir_len;
-        found = run_record(c, line_start, line_end, qs.cap, qs.variable != 0, res);
+        found = run_record<MAXLV>(c, line_start, line_end, qs.cap, qs.variable != 0, res);
       }
       if (qs.counts) qs.counts[line] = found;
+      ir_off += qs.ir_len;
     }
   }
 }
+
+template __global__ void query_batch_kernel<4>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, QueryBatch);
+template __global__ void query_batch_kernel<8>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, QueryBatch);
+template __global__ void query_batch_kernel<16>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, QueryBatch);
 
 } // namespace gpj

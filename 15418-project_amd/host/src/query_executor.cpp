@@ -79,6 +79,7 @@ BatchQueryResult QueryExecutor::execute_batch(const BatchCompiledQuery &compiled
       spans.emplace_back(static_cast<size_t>(num_lines) * slots * 2 * sizeof(int64_t));
       descs.push_back(gpj_query_desc{irs.back().as<uint8_t>(),
                                      static_cast<uint32_t>(cq.ir_bytes().size()), slots, 0,
+                                     static_cast<int32_t>(cq.max_depth()),
                                      spans.back().as<int64_t>(), nullptr});
     }
     if (ir_total > 8192) {

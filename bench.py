@@ -158,7 +158,7 @@ def main():
 
     descs = (gpj.QueryDescT * len(compiled))()
     for qi, c in enumerate(compiled):
-        descs[qi] = gpj.QueryDescT(d_irs[qi].ptr, len(c[0]), caps[qi], int(variable[qi]),
+        descs[qi] = gpj.QueryDescT(d_irs[qi].ptr, len(c[0]), caps[qi], int(variable[qi]), c[1],
                                    d_spans[qi].ptr, d_counts[qi].ptr if variable[qi] else None)
 
     def step(ev=None):

@@ -157,6 +157,8 @@ typedef struct {
   uint32_t ir_len;
   int32_t capacity;
   int32_t variable;
+  int32_t max_level; /* the compiler's max_depth (deepest level the bytecode
+                        reaches); 0 = unknown.  Picks a smaller interpreter. */
   int64_t *d_spans;  /* [max_lines][capacity][2] */
   int64_t *d_counts; /* [max_lines], variable mode only */
 } gpj_query_desc;
