@@ -326,9 +326,11 @@ gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index 
   if (!d_in || !index->d_newline || !index->d_num_lines || !index->d_string || !index->d_leveled)
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "Query execution requires GPU-resident newline, string, and bitmap indices");
-  const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_lines, 256), 1u << 20);
+  // one wave per block (the kernel keeps each wave's interpreter arrays in
+  // its block's LDS)
+  const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_lines, 64), 1u << 22);
 #define GPJ_QLAUNCH(LV)                                                                   \
-  hipLaunchKernelGGL(gpj::query_batch_kernel<LV>, dim3((uint32_t)blocks), dim3(256), 0,      \
+  hipLaunchKernelGGL(gpj::query_batch_kernel<LV>, dim3((uint32_t)blocks), dim3(64), 0,       \
                      as_stream(stream), d_in, (int64_t)n, index->d_newline,                  \
                      index->d_num_lines, max_lines, index->d_string, index->d_leveled,       \
                      index->level_size, index->num_levels, batch)

@@ -5,11 +5,23 @@
 // newline-index entries is one record, trimmed to its first '{' and last '}';
 // the bytecode walks the leveled rows and the string mask; STORE_RESULT
 // writes {start, end} of the current value (spaces trimmed), -1 = no match.
-// Two deliberate differences, both outside what the reference defines:
-// bytes outside [0, n) read as 0 (the reference reads out of bounds there),
-// and an unknown opcode or running off the bytecode ends the record.
-// The bytecode comes in through a device pointer and is staged in LDS, so
-// concurrent launches with different queries are independent.
+// Deliberate differences, all outside what the reference defines: bytes
+// outside [0, n) read as 0 (the reference reads out of bounds there); an
+// unknown opcode or running off the bytecode ends the path; so does a
+// negative key length or jump offset (which would move the bytecode position
+// backwards and never end).
+//
+// Execution model.  Each lane interprets one record, but the bytecode is
+// decoded once per wave: every step picks the lowest bytecode position held
+// by a live lane, reads that instruction through the scalar unit, and runs it
+// on exactly the lanes that sit there.  Lanes of one query normally move in
+// lockstep, so an opcode costs one scalar decode instead of a divergent
+// switch per lane; lanes that drift apart (failed filters, element
+// iterations of different lengths) reconverge at the lowest position.  Each
+// lane still executes its own instruction sequence in order, so results are
+// those of a per-record sequential interpreter.  The bytecode is read from
+// its device buffer directly (no constant-memory copy), so concurrent
+// launches with different queries are independent.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -19,453 +31,579 @@ namespace gpj {
 
 namespace {
 
-enum : uint8_t {
+enum : uint32_t {
   kEnd = 0, kStoreResult = 1, kMoveUp = 2, kMoveDown = 3, kMoveToKey = 4,
   kMoveToIndex = 5, kMoveToIndexReverse = 6, kMarkPosition = 7,
   kResetPosition = 8, kExpressionStringEquals = 9,
-  kMoveToEach = 10, kJumpIfNotArray = 11 // extension opcodes
+  kMoveToEach = 10, kJumpIfNotArray = 11, // extension opcodes
+  kOffEnd = 256                           // past the bytecode: ends the path
 };
 constexpr int kMaxFrames = 8; // element iterations per record (as the oracle)
 
-// Per-lane interpreter context.  Two small register caches cut the number of
-// scattered global loads per record: the last aligned 8 text bytes read, and
-// the last two bitmap words read (the key search alternates between a leveled
-// row and the string mask).  Both stay valid across the queries of a batch,
-// which run over the same record.
-struct Ctx {
-  const uint8_t *file;
-  int64_t n;
-  const uint64_t *string;
-  const uint64_t *leveled;
-  int64_t level_size;
-  int num_levels;
-  const uint8_t *ir; // LDS copy of the current query
-  uint32_t ir_len;
-  int64_t text_at;   // aligned 8-byte block held in text_val (-1: none)
-  uint64_t text_val;
-  const uint64_t *w_ptr0, *w_ptr1;
-  uint64_t w_val0, w_val1;
-  bool w_next1;      // slot to replace next
+typedef const __attribute__((address_space(4))) uint32_t *ConstU32;
+
+// Wave-uniform reads of one query's bytecode through the scalar data cache.
+// Only aligned dwords that hold at least one byte of the buffer are loaded;
+// bytes at or past `len` read as 0.
+struct IrReader {
+  uintptr_t base;
+  uint32_t len;
+  __device__ __forceinline__ uint32_t dword_at(uintptr_t a) const {
+    // a is 4-byte aligned; loaded only when it overlaps [base, base + len)
+    if (a + 4 <= base || a >= base + len) return 0;
+    return *reinterpret_cast<ConstU32>(a);
+  }
+  __device__ __forceinline__ uint32_t byte(int64_t p) const {
+    if (p < 0 || p >= (int64_t)len) return 0;
+    const uintptr_t a = base + (uintptr_t)p;
+    return (dword_at(a & ~(uintptr_t)3) >> (8 * (a & 3))) & 0xFFu;
+  }
+  // Little-endian 8 bytes from p (bytes outside [0, len) read as 0).
+  __device__ __forceinline__ uint64_t u64(int64_t p) const {
+    const uintptr_t a = base + (uintptr_t)p;
+    const uintptr_t a0 = a & ~(uintptr_t)3;
+    const uint32_t sh = 8 * (uint32_t)(a & 3);
+    const uint64_t lo = (uint64_t)dword_at(a0) | ((uint64_t)dword_at(a0 + 4) << 32);
+    const uint64_t hi = dword_at(a0 + 8);
+    uint64_t v = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+    // clear bytes at or past len
+    const int64_t valid = (int64_t)len - p;
+    if (valid <= 0) return 0;
+    if (valid < 8) v &= (1ull << (8 * valid)) - 1;
+    return v;
+  }
+  __device__ __forceinline__ int32_t varint(int64_t &qp) const {
+    uint32_t value = 0, b = 0;
+    int shift = 0;
+    for (;;) {
+      b = byte(qp++);
+      if (!(b & 0x80u)) break;
+      value |= (b & 0x7Fu) << shift;
+      shift += 7;
+    }
+    return (int32_t)(value | (b << shift));
+  }
 };
 
-// Blocks are aligned 8-byte groups of device addresses; a block that is not
-// wholly inside [file, file + n) is assembled byte by byte.
-__device__ __forceinline__ uint8_t byte_at(Ctx &c, int64_t p) {
-  if (p < 0 || p >= c.n) return 0;
-  const uintptr_t base = reinterpret_cast<uintptr_t>(c.file);
-  const int64_t blk = (int64_t)((base + (uintptr_t)p) >> 3);
-  if (blk != c.text_at) {
-    const uintptr_t lo = (uintptr_t)blk << 3;
-    if (lo >= base && lo + 8 <= base + (uintptr_t)c.n) {
-      c.text_val = *reinterpret_cast<const uint64_t *>(lo);
-    } else {
-      uint64_t v = 0;
-      for (uintptr_t a = lo; a < lo + 8; ++a)
-        if (a >= base && a < base + (uintptr_t)c.n)
-          v |= (uint64_t)(*reinterpret_cast<const uint8_t *>(a)) << (8 * (a & 7));
-      c.text_val = v;
-    }
-    c.text_at = blk;
+// The input as aligned 8-byte blocks.  Block b holds file offsets
+// [8b - head, 8b - head + 8), where head is the input address modulo 8.  The
+// first and last blocks may hold bytes outside [0, n); they are assembled
+// once (those bytes read as 0), every block between is loaded whole.
+struct Text {
+  const uint64_t *__restrict__ blocks;
+  int64_t head;
+  int64_t last; // index of the last block that holds a byte of the input
+  uint64_t first_val, last_val;
+};
+
+__device__ __forceinline__ Text make_text(const uint8_t *__restrict__ file, int64_t n) {
+  Text t;
+  const int64_t head = (int64_t)(reinterpret_cast<uintptr_t>(file) & 7);
+  t.head = head;
+  t.blocks = reinterpret_cast<const uint64_t *>(file - head);
+  t.last = n > 0 ? (n - 1 + head) >> 3 : -1;
+  t.first_val = 0;
+  t.last_val = 0;
+  for (int64_t k = 0; k < 8; ++k) {
+    const int64_t p0 = k - head, p1 = 8 * t.last + k - head;
+    if (p0 >= 0 && p0 < n) t.first_val |= (uint64_t)file[p0] << (8 * k);
+    if (p1 >= 0 && p1 < n) t.last_val |= (uint64_t)file[p1] << (8 * k);
   }
-  return (uint8_t)(c.text_val >> (8 * ((base + (uintptr_t)p) & 7)));
+  return t;
 }
 
-__device__ __forceinline__ uint64_t word_at(Ctx &c, const uint64_t *p) {
-  if (p == c.w_ptr0) return c.w_val0;
-  if (p == c.w_ptr1) return c.w_val1;
-  const uint64_t v = *p;
-  if (c.w_next1) {
-    c.w_ptr1 = p;
-    c.w_val1 = v;
-  } else {
-    c.w_ptr0 = p;
-    c.w_val0 = v;
+// Per-lane view of the record's text and bitmaps, with one-entry caches for
+// the last text block, the last leveled-row word and the last string-mask
+// word read.  The caches stay valid across the queries of a batch, which run
+// over the same record.
+struct Ctx {
+  Text t;
+  int64_t n;
+  const uint64_t *__restrict__ string;
+  const uint64_t *__restrict__ leveled;
+  int64_t level_size;
+  int num_levels;
+  int64_t text_at; // block held in text_val (-1: none)
+  uint64_t text_val;
+  int64_t row_at;  // word index (into the leveled rows) held in row_val
+  uint64_t row_val;
+  int64_t str_at;  // word index held in str_val
+  uint64_t str_val;
+};
+
+// Block b of the input (blocks outside the input read as 0).
+__device__ __forceinline__ uint64_t text_block(const Ctx &c, int64_t b) {
+  const int64_t bc = b < 0 ? 0 : (b > c.t.last ? c.t.last : b);
+  uint64_t v = c.t.blocks[bc < 0 ? 0 : bc];
+  v = (b == 0) ? c.t.first_val : v;
+  v = (b == c.t.last) ? c.t.last_val : v;
+  return (b < 0 || b > c.t.last) ? 0 : v;
+}
+
+__device__ __forceinline__ uint8_t byte_at(Ctx &c, int64_t p) {
+  if ((uint64_t)p >= (uint64_t)c.n) return 0;
+  const int64_t a = p + c.t.head;
+  const int64_t b = a >> 3;
+  if (b != c.text_at) {
+    c.text_val = text_block(c, b);
+    c.text_at = b;
   }
-  c.w_next1 = !c.w_next1;
-  return v;
+  return (uint8_t)(c.text_val >> (8 * (a & 7)));
 }
-__device__ __forceinline__ uint8_t ir_at(const Ctx &c, int64_t p) {
-  return (p >= 0 && p < (int64_t)c.ir_len) ? c.ir[p] : (uint8_t)0;
+
+// Little-endian 8 text bytes from p (bytes outside [0, n) read as 0).
+__device__ __forceinline__ uint64_t text8(const Ctx &c, int64_t p) {
+  const int64_t a = p + c.t.head;
+  const int64_t b = a >> 3;
+  const uint32_t sh = 8 * (uint32_t)(a & 7);
+  const uint64_t lo = text_block(c, b);
+  const uint64_t hi = text_block(c, b + 1);
+  return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
 }
+
+__device__ __forceinline__ uint64_t row_word(Ctx &c, int64_t wi) {
+  if (wi != c.row_at) {
+    c.row_val = c.leveled[wi];
+    c.row_at = wi;
+  }
+  return c.row_val;
+}
+__device__ __forceinline__ uint64_t str_word(Ctx &c, int64_t wi) {
+  if (wi != c.str_at) {
+    c.str_val = c.string[wi];
+    c.str_at = wi;
+  }
+  return c.str_val;
+}
+
 __device__ __forceinline__ uint64_t through(int b) {
   return b >= 63 ? ~0ull : ((1ull << (b + 1)) - 1);
 }
 
-// Last set bit of `bm` in [lo, idx], or -1.
-__device__ int64_t last_bit_in(Ctx &c, const uint64_t *bm, int64_t lo, int64_t idx) {
+// Last set bit of the string mask in [lo, idx], or -1.
+__device__ __forceinline__ int64_t last_string_bit(Ctx &c, int64_t lo, int64_t idx) {
   if (idx < lo || idx < 0) return -1;
   int64_t wi = idx >> 6, wb = wi << 6;
-  uint64_t word = word_at(c, bm + wi) & through((int)(idx - wb));
-  while (wb + 63 >= lo) {
+  uint64_t word = str_word(c, wi) & through((int)(idx - wb));
+  for (;;) {
     uint64_t cand = word;
     const int64_t first = lo - wb;
     if (first > 0) cand &= ~through((int)(first - 1));
     if (cand) return wb + 63 - __builtin_clzll(cand);
-    if (wi == 0) break;
+    if (wi == 0 || wb - 1 < lo) return -1;
     --wi;
     wb -= 64;
-    if (wb + 63 < lo) break;
-    word = word_at(c, bm + wi);
+    word = str_word(c, wi);
   }
-  return -1;
 }
 
-// First structural bit of `level` in [idx, level_end], else level_end.
-__device__ int64_t next_structural(Ctx &c, int64_t level_end, int64_t idx,
-                                   int level) {
+// Last set bit of the leveled row starting at word `row` in [lo, idx], or -1.
+__device__ __forceinline__ int64_t last_row_bit(Ctx &c, int64_t row, int64_t lo, int64_t idx) {
+  if (idx < lo || idx < 0) return -1;
+  int64_t wi = idx >> 6, wb = wi << 6;
+  uint64_t word = row_word(c, row + wi) & through((int)(idx - wb));
+  for (;;) {
+    uint64_t cand = word;
+    const int64_t first = lo - wb;
+    if (first > 0) cand &= ~through((int)(first - 1));
+    if (cand) return wb + 63 - __builtin_clzll(cand);
+    if (wi == 0 || wb - 1 < lo) return -1;
+    --wi;
+    wb -= 64;
+    word = row_word(c, row + wi);
+  }
+}
+
+// First structural bit of `level` in [idx, level_end], else level_end
+// (find_next_structural_char, query.cu:79-115).
+__device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, int64_t idx, int level) {
   if (idx >= level_end) return level_end;
   if (level < 0 || level >= c.num_levels) return level_end;
   if (idx < 0) idx = 0;
-  const uint64_t *row = c.leveled + (uint64_t)c.level_size * (uint64_t)level;
+  const int64_t row = c.level_size * (int64_t)level;
   int64_t wi = idx >> 6, wb = wi << 6;
-  uint64_t word = word_at(c, row + wi) & (~0ull << (idx - wb));
-  while (wb <= level_end) {
+  uint64_t word = row_word(c, row + wi) & (~0ull << (idx - wb));
+  for (;;) {
     uint64_t cand = word;
     const int64_t last = level_end - wb;
     if (last < 63) cand &= through((int)last);
     if (cand) return wb + __builtin_ctzll(cand);
     ++wi;
     wb += 64;
-    if (wb > level_end || wi >= c.level_size) break;
-    word = word_at(c, row + wi);
+    if (wb > level_end || wi >= c.level_size) return level_end;
+    word = row_word(c, row + wi);
   }
-  return level_end;
 }
 
-__device__ int64_t prev_structural(Ctx &c, int64_t level_start,
-                                   int64_t idx, int level) {
+// find_previous_structural_char, query.cu:117-128
+__device__ __forceinline__ int64_t prev_structural(Ctx &c, int64_t level_start, int64_t idx, int level) {
   if (idx <= level_start) return level_start;
   if (level < 0 || level >= c.num_levels) return level_start;
-  const uint64_t *row = c.leveled + (uint64_t)c.level_size * (uint64_t)level;
-  const int64_t s = last_bit_in(c, row, level_start, idx);
+  const int64_t row = c.level_size * (int64_t)level;
+  const int64_t s = last_row_bit(c, row, level_start, idx);
   return s == -1 ? level_start : s;
 }
 
-__device__ __forceinline__ int32_t read_varint(const Ctx &c, int64_t &qp) {
-  uint32_t value = 0, byte = 0;
-  int shift = 0;
-  for (;;) {
-    byte = ir_at(c, qp++);
-    if (!(byte & 0x80u)) break;
-    value |= (byte & 0x7Fu) << shift;
-    shift += 7;
-  }
-  return (int32_t)(value | (byte << shift));
-}
-
-// Fixed-size per-lane array indexed by a lane-varying value.  Every access is
-// an unrolled compare/select over the N elements, so the array lives in
-// registers (a plain indexed array would be placed in scratch memory).
-template <typename T, int N> struct RegArray {
-  T v[N];
-  __device__ __forceinline__ T get(int i) const {
-    T r = v[0];
-#pragma unroll
-    for (int k = 1; k < N; ++k) r = (k == i) ? v[k] : r;
-    return r;
-  }
-  __device__ __forceinline__ void set(int i, T x) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = (k == i) ? x : v[k];
-  }
+// Per-lane array of N entries kept in LDS as [N][64] (one column per lane of
+// the wave), so a lane-varying index is an ordinary LDS address and the
+// interpreter's registers stay free.  Indices are in [0, N) by construction:
+// the interpreter checks every level, mark and iteration depth against N
+// before it moves there.
+template <typename T, int N> struct LdsArray {
+  T (*a)[64];
+  int lane;
+  __device__ __forceinline__ T get(int i) const { return a[i][lane]; }
+  __device__ __forceinline__ void set(int i, T x) { a[i][lane] = x; }
   __device__ __forceinline__ void fill(T x) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = x;
+    for (int k = 0; k < N; ++k) a[k][lane] = x;
+  }
+  // entry k := x for every k > i (i >= 0)
+  __device__ __forceinline__ void fill_above(int i, T x) {
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      if (k > i) a[k][lane] = x;
+  }
+  // walking up from entry i + 1 (i >= 0): := x until the first entry that
+  // already holds x (which stays, as do all after it)
+  __device__ __forceinline__ void fill_above_until(int i, T x) {
+    for (int k = i + 1; k < N; ++k) {
+      if (a[k][lane] == x) break;
+      a[k][lane] = x;
+    }
   }
 };
 
-// One element iteration of MOVE_TO_EACH (extension): the array's level, the
-// separator before the current element, where the path restarts, and the
-// mark-stack depth to restore.
-// (packed into registers: level, separator position, restart position,
-// mark-stack depth)
+// LDS home of the per-lane arrays of one wave.
+template <int MAXLV> struct WaveArrays {
+  static constexpr int kF = MAXLV < kMaxFrames ? MAXLV : kMaxFrames;
+  int64_t level_end[MAXLV][64];
+  int64_t marked_pos[MAXLV][64];
+  int64_t fr_sep[kF][64];
+  int32_t curr_index[MAXLV][64];
+  uint32_t fr_pack[kF][64]; // qp | lvl << 16 | (mi + 1) << 21
+  uint8_t marked_lvl[MAXLV][64];
+};
 
-// Interprets the bytecode over one record.  Fixed mode (reference): STORE i
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    const int32_t o = __shfl_xor(v, m, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Per-lane interpreter state for one record and one query: the scalars in
+// registers, the arrays in the wave's LDS block.
+// MAXLV bounds the level (the host derives it from the bytecode); levels,
+// marks and element iterations each use at most MAXLV entries.
+template <int MAXLV> struct LaneState {
+  static constexpr int kF = WaveArrays<MAXLV>::kF;
+  int64_t idx;
+  int32_t lvl, qp, mi, nfr;
+  int64_t count; // results stored (fixed) or found (variable)
+  LdsArray<int64_t, MAXLV> level_end;
+  LdsArray<int32_t, MAXLV> curr_index;
+  LdsArray<int64_t, MAXLV> marked_pos;
+  LdsArray<uint8_t, MAXLV> marked_lvl;
+  LdsArray<uint32_t, kF> fr_pack; // bytecode position (after MOVE_TO_EACH, so
+                                  // <= 4097), level and mark depth + 1
+  LdsArray<int64_t, kF> fr_sep;
+  __device__ __forceinline__ LaneState(WaveArrays<MAXLV> &w, int lane)
+      : level_end{w.level_end, lane}, curr_index{w.curr_index, lane},
+        marked_pos{w.marked_pos, lane}, marked_lvl{w.marked_lvl, lane},
+        fr_pack{w.fr_pack, lane}, fr_sep{w.fr_sep, lane} {}
+};
+
+// A finished path (END or any failure): resume the innermost element
+// iteration at its next element, or end the record.  Returns false when the
+// record is done.
+template <int MAXLV>
+__device__ __forceinline__ bool resume(Ctx &c, LaneState<MAXLV> &s) {
+  for (;;) {
+    if (s.nfr == 0) return false;
+    const int f = s.nfr - 1;
+    const uint32_t pack = s.fr_pack.get(f);
+    const int flvl = (int)((pack >> 16) & 31u);
+    const int64_t lend = s.level_end.get(flvl);
+    const int64_t sep = next_structural(c, lend, s.fr_sep.get(f) + 1, flvl);
+    if (byte_at(c, sep) == ',' && sep < lend) {
+      s.fr_sep.set(f, sep);
+      s.lvl = flvl;
+      s.level_end.fill_above(s.lvl, -1);
+      s.mi = (int32_t)(pack >> 21) - 1;
+      s.idx = sep + 1;
+      s.qp = (int32_t)(pack & 0xFFFFu);
+      return true;
+    }
+    --s.nfr;
+  }
+}
+
+// Runs one query over each lane's record.  `live` marks lanes holding a
+// record (trimmed to line_start < line_end).  Fixed mode (reference): STORE i
 // writes slot i; any failure ends the record.  Variable mode (extension):
 // STORE appends (first `cap` stored, all counted) and a finished path
-// resumes the innermost MOVE_TO_EACH at its next element.
-// line_start/line_end are already trimmed to the record's first '{' and
-// last '}' (line_start < line_end).
-// MAXLV bounds the interpreter's level (the host derives it from the
-// bytecode); levels, marks and element iterations each use at most MAXLV
-// entries.
+// resumes the innermost MOVE_TO_EACH at its next element.  Returns the
+// lane's result count.
 template <int MAXLV>
-__device__ int64_t run_record(Ctx &c, int64_t line_start, int64_t line_end,
-                              int32_t cap, bool variable, int64_t *res) {
-  int64_t idx = line_start;
-  int lvl = 0;
-  int64_t qp = 0;
-  RegArray<int64_t, MAXLV> level_end;
-  RegArray<int32_t, MAXLV> curr_index;
-  RegArray<int64_t, MAXLV> marked_pos;
-  RegArray<int32_t, MAXLV> marked_lvl;
-  constexpr int kF = MAXLV < kMaxFrames ? MAXLV : kMaxFrames;
-  RegArray<int32_t, kF> fr_lvl, fr_qp, fr_mi;
-  RegArray<int64_t, kF> fr_sep;
-  level_end.fill(-1);
-  level_end.set(0, line_end);
-  curr_index.fill(-1);
-  marked_pos.fill(0);
-  marked_lvl.fill(0);
-  fr_lvl.fill(0);
-  fr_qp.fill(0);
-  fr_mi.fill(0);
-  fr_sep.fill(0);
-  int mi = -1;
-  int nres = 0;
-  int nfr = 0;
-  int64_t count = 0;
+__device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool live,
+                                             int64_t line_start, int64_t line_end,
+                                             const IrReader ir, int32_t cap, bool variable,
+                                             int64_t *res) {
+  LaneState<MAXLV> s(w, (int)(threadIdx.x & 63));
+  s.idx = line_start;
+  s.lvl = 0;
+  s.qp = 0;
+  s.mi = -1;
+  s.nfr = 0;
+  s.count = 0;
+  s.level_end.fill(-1);
+  s.level_end.set(0, line_end);
+  s.curr_index.fill(-1);
 
   for (;;) {
-    if (qp >= (int64_t)c.ir_len) goto stop;
-    {
-      const uint8_t op = c.ir[qp++];
-      switch (op) {
-      case kEnd:
-        goto stop;
-      case kStoreResult: {
-        if (!variable && nres >= cap) goto stop;
-        int64_t end_str = level_end.get(lvl);
-        while (byte_at(c, idx) == ' ' && idx < level_end.get(lvl)) ++idx;
-        while (end_str > idx && byte_at(c, end_str - 1) == ' ') --end_str;
-        const int64_t slot = variable ? count : nres;
-        if (slot < cap) {
-          res[slot * 2] = idx;
-          res[slot * 2 + 1] = end_str;
-        }
-        if (variable) ++count;
-        else ++nres;
-        break;
+    const uint64_t act = __ballot(live);
+    if (!act) break;
+    // bytecode position to run: the one all live lanes share, else the
+    // furthest ahead (lanes that run ahead, e.g. past a filter, then loop
+    // back to where the others wait)
+    const int32_t q0 = __builtin_amdgcn_readlane(s.qp, (int)__builtin_ctzll(act));
+    int32_t qpu = q0;
+    if (__ballot(live && s.qp == q0) != act) {
+      const int32_t m =
+          __builtin_amdgcn_readfirstlane(wave_max_i32(live ? s.qp : INT32_MIN));
+      if (__ballot(live && s.qp == m)) qpu = m;
+    }
+    // decode the instruction at qpu (wave-uniform)
+    int64_t qn = qpu;
+    const uint32_t op = qpu >= (int32_t)ir.len ? (uint32_t)kOffEnd : ir.byte(qn++);
+    int32_t arg = 0;
+    int64_t key_pos = 0;
+    if (op == kMoveToKey || op == kMoveToIndex || op == kMoveToIndexReverse ||
+        op == kExpressionStringEquals || op == kJumpIfNotArray) {
+      arg = ir.varint(qn);
+      key_pos = qn;
+      if (op == kMoveToKey || op == kExpressionStringEquals) qn += arg;
+    }
+    if (!(live && s.qp == qpu)) continue;
+
+    s.qp = (int32_t)qn;
+    bool fail = false;
+    switch (op) {
+    case kStoreResult: {
+      if (!variable && s.count >= cap) { fail = true; break; }
+      const int64_t lend = s.level_end.get(s.lvl);
+      int64_t end_str = lend;
+      while (byte_at(c, s.idx) == ' ' && s.idx < lend) ++s.idx;
+      while (end_str > s.idx && byte_at(c, end_str - 1) == ' ') --end_str;
+      if (s.count < cap) {
+        res[s.count * 2] = s.idx;
+        res[s.count * 2 + 1] = end_str;
       }
-      case kMoveUp:
-        if (lvl <= 0) goto stop;
-        idx = level_end.get(lvl);
-        level_end.set(lvl, -1);
-        --lvl;
-        break;
-      case kMoveDown:
-        if (lvl + 1 >= MAXLV) goto stop;
-        ++lvl;
-        if (level_end.get(lvl) == -1) {
-          level_end.set(lvl, next_structural(c, level_end.get(lvl - 1), idx + 1, lvl - 1));
-          while (byte_at(c, idx) == ' ') ++idx;
-        }
-        break;
-      case kMoveToKey: {
-        const int32_t key_len = read_varint(c, qp);
-        const int64_t key_pos = qp;
-        qp += key_len;
-        if (byte_at(c, idx) != '{') goto stop;
-        for (;;) {
-          ++idx;
-          idx = next_structural(c, level_end.get(lvl), idx, lvl);
-          if (byte_at(c, idx) != ':') break;
-          const int64_t string_end = last_bit_in(c, c.string, line_start + 1, idx - 1);
-          const int64_t string_start = string_end - key_len;
-          bool match = string_start >= line_start && byte_at(c, string_start) == '"';
-          for (int32_t i = 0; match && i < key_len; ++i)
-            match = ir_at(c, key_pos + i) == byte_at(c, string_start + i + 1);
-          if (match) break;
-          ++idx;
-          idx = next_structural(c, level_end.get(lvl), idx, lvl);
-          if (byte_at(c, idx) == '}') goto stop;
-        }
-        ++idx;
-        break;
+      ++s.count;
+      break;
+    }
+    case kMoveUp:
+      if (s.lvl <= 0) { fail = true; break; }
+      s.idx = s.level_end.get(s.lvl);
+      s.level_end.set(s.lvl, -1);
+      --s.lvl;
+      break;
+    case kMoveDown:
+      if (s.lvl + 1 >= MAXLV) { fail = true; break; }
+      ++s.lvl;
+      if (s.level_end.get(s.lvl) == -1) {
+        s.level_end.set(s.lvl,
+                        next_structural(c, s.level_end.get(s.lvl - 1), s.idx + 1, s.lvl - 1));
+        while (byte_at(c, s.idx) == ' ') ++s.idx;
       }
-      case kMoveToIndex: {
-        const int32_t target = read_varint(c, qp);
-        const uint8_t ch = byte_at(c, idx);
-        if (ch != '[' && ch != ',' && ch != ']') goto stop;
-        const int64_t lend = level_end.get(lvl);
-        int32_t ci = (ch == '[') ? 0 : curr_index.get(lvl) + 1;
-        for (;;) {
-          if (ci < target) {
-            ++idx;
-            idx = next_structural(c, lend, idx, lvl);
-            if (byte_at(c, idx) == ',') { ++ci; continue; }
-            curr_index.set(lvl, ci);
-            goto stop;
+      break;
+    case kMoveToKey: {
+      if (arg < 0 || byte_at(c, s.idx) != '{') { fail = true; break; }
+      const int64_t lend = s.level_end.get(s.lvl);
+      const uint64_t key0 = ir.u64(key_pos);
+      const uint64_t mask0 = arg >= 8 ? ~0ull : ((1ull << (8 * arg)) - 1);
+      int64_t idx = s.idx;
+      for (;;) {
+        idx = next_structural(c, lend, idx + 1, s.lvl);
+        if (byte_at(c, idx) != ':') break;
+        const int64_t string_end = last_string_bit(c, line_start + 1, idx - 1);
+        const int64_t ss = string_end - arg;
+        bool match = ss >= line_start && byte_at(c, ss) == '"';
+        if (match && arg > 0) {
+          match = ((text8(c, ss + 1) ^ key0) & mask0) == 0;
+          for (int32_t k = 8; match && k < arg; k += 8) {
+            const int32_t rest = arg - k;
+            const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
+            match = ((text8(c, ss + 1 + k) ^ ir.u64(key_pos + k)) & m) == 0;
           }
-          if (ci > target) {
-            --idx;
-            idx = prev_structural(c, 0, idx, lvl);
-            --ci;
-            continue;
-          }
+        }
+        if (match) break;
+        idx = next_structural(c, lend, idx + 1, s.lvl);
+        if (byte_at(c, idx) == '}') { fail = true; break; }
+      }
+      s.idx = idx + 1;
+      break;
+    }
+    case kMoveToIndex: {
+      const int32_t target = arg;
+      const uint8_t ch = byte_at(c, s.idx);
+      if (ch != '[' && ch != ',' && ch != ']') { fail = true; break; }
+      const int64_t lend = s.level_end.get(s.lvl);
+      int32_t ci = (ch == '[') ? 0 : s.curr_index.get(s.lvl) + 1;
+      int64_t idx = s.idx;
+      for (;;) {
+        if (ci < target) {
+          idx = next_structural(c, lend, idx + 1, s.lvl);
+          if (byte_at(c, idx) == ',') { ++ci; continue; }
+          fail = true;
           break;
         }
-        curr_index.set(lvl, ci);
-        if (byte_at(c, idx) == ']') goto stop;
-        ++idx;
+        if (ci > target) {
+          idx = prev_structural(c, 0, idx - 1, s.lvl);
+          --ci;
+          continue;
+        }
         break;
       }
-      case kMoveToIndexReverse: {
-        const int32_t target = read_varint(c, qp);
-        const int64_t lend = level_end.get(lvl);
-        idx = lend - 1;
-        while (idx >= 0 && byte_at(c, idx) == ' ') --idx;
-        const uint8_t ch = byte_at(c, idx);
-        if (ch != ']' && ch != ',' && ch != '[') goto stop;
-        int32_t ci = (ch == ']') ? 0 : curr_index.get(lvl) + 1;
-        for (;;) {
-          if (ci < target) {
-            --idx;
-            idx = prev_structural(c, 0, idx, lvl);
-            if (byte_at(c, idx) == ',' || ci + 1 == target) {
-              ++ci;
-              continue;
-            }
-            curr_index.set(lvl, ci);
-            goto stop;
-          }
-          if (ci > target) {
-            ++idx;
-            idx = next_structural(c, lend, idx, lvl);
-            --ci;
-            continue;
-          }
+      s.curr_index.set(s.lvl, ci);
+      if (!fail && byte_at(c, idx) == ']') fail = true;
+      s.idx = idx + 1;
+      break;
+    }
+    case kMoveToIndexReverse: {
+      const int32_t target = arg;
+      const int64_t lend = s.level_end.get(s.lvl);
+      int64_t idx = lend - 1;
+      while (idx >= 0 && byte_at(c, idx) == ' ') --idx;
+      const uint8_t ch = byte_at(c, idx);
+      if (ch != ']' && ch != ',' && ch != '[') { fail = true; break; }
+      int32_t ci = (ch == ']') ? 0 : s.curr_index.get(s.lvl) + 1;
+      for (;;) {
+        if (ci < target) {
+          idx = prev_structural(c, 0, idx - 1, s.lvl);
+          if (byte_at(c, idx) == ',' || ci + 1 == target) { ++ci; continue; }
+          fail = true;
           break;
         }
-        curr_index.set(lvl, ci);
-        ++idx;
-        break;
-      }
-      case kMarkPosition:
-        if (mi + 1 >= MAXLV) goto stop;
-        ++mi;
-        marked_pos.set(mi, idx);
-        marked_lvl.set(mi, lvl);
-        break;
-      case kResetPosition: {
-        if (mi < 0) goto stop;
-        idx = marked_pos.get(mi);
-        lvl = marked_lvl.get(mi);
-        --mi;
-        // clear the levels above, up to the first one already clear
-        bool clearing = true;
-#pragma unroll
-        for (int i = 1; i < MAXLV; ++i) {
-          if (i > lvl) {
-            if (level_end.v[i] == -1) clearing = false;
-            if (clearing) level_end.v[i] = -1;
-          }
+        if (ci > target) {
+          idx = next_structural(c, lend, idx + 1, s.lvl);
+          --ci;
+          continue;
         }
         break;
       }
-      case kExpressionStringEquals: {
-        const int32_t key_len = read_varint(c, qp);
-        const int64_t key_pos = qp;
-        qp += key_len;
-        while (byte_at(c, idx) == ' ' && idx < level_end.get(lvl)) ++idx;
-        int64_t string_end = level_end.get(lvl) - 1;
-        while (idx < string_end && byte_at(c, string_end) == ' ') --string_end;
-        if (string_end - idx + 1 != key_len) goto stop;
-        bool eq = true;
-        for (int32_t i = 0; eq && i < key_len; ++i)
-          eq = ir_at(c, key_pos + i) == byte_at(c, idx + i);
-        if (!eq) goto stop;
-        break;
-      }
-      case kMoveToEach: {
-        if (!variable || byte_at(c, idx) != '[' || nfr >= kF) goto stop;
-        const int64_t first = next_structural(c, level_end.get(lvl), idx + 1, lvl);
-        int64_t p = idx + 1;
-        while (p < first && byte_at(c, p) == ' ') ++p;
-        if (p == first && byte_at(c, first) == ']') goto stop; // empty array
-        fr_lvl.set(nfr, lvl);
-        fr_sep.set(nfr, idx);
-        fr_qp.set(nfr, (int32_t)qp);
-        fr_mi.set(nfr, mi);
-        ++nfr;
-        ++idx;
-        break;
-      }
-      case kJumpIfNotArray: {
-        const int32_t off = read_varint(c, qp);
-        if (byte_at(c, idx) != '[') qp += off;
-        break;
-      }
-      default:
-        goto stop;
-      }
+      s.curr_index.set(s.lvl, ci);
+      s.idx = idx + 1;
+      break;
     }
-    continue;
-  stop:
-    for (;;) {
-      if (nfr == 0) return count;
-      const int f = nfr - 1;
-      const int flvl = fr_lvl.get(f);
-      const int64_t lend = level_end.get(flvl);
-      const int64_t sep = next_structural(c, lend, fr_sep.get(f) + 1, flvl);
-      if (byte_at(c, sep) == ',' && sep < lend) {
-        fr_sep.set(f, sep);
-        lvl = flvl;
-#pragma unroll
-        for (int i = 1; i < MAXLV; ++i)
-          if (i > lvl) level_end.set(i, -1);
-        mi = fr_mi.get(f);
-        idx = sep + 1;
-        qp = fr_qp.get(f);
+    case kMarkPosition:
+      if (s.mi + 1 >= MAXLV) { fail = true; break; }
+      ++s.mi;
+      s.marked_pos.set(s.mi, s.idx);
+      s.marked_lvl.set(s.mi, (uint8_t)s.lvl);
+      break;
+    case kResetPosition: {
+      if (s.mi < 0) { fail = true; break; }
+      s.idx = s.marked_pos.get(s.mi);
+      s.lvl = s.marked_lvl.get(s.mi);
+      --s.mi;
+      // clear the levels above, up to the first one already clear
+      s.level_end.fill_above_until(s.lvl, -1);
+      break;
+    }
+    case kExpressionStringEquals: {
+      if (arg < 0) { fail = true; break; }
+      const int64_t lend = s.level_end.get(s.lvl);
+      while (byte_at(c, s.idx) == ' ' && s.idx < lend) ++s.idx;
+      int64_t string_end = lend - 1;
+      while (s.idx < string_end && byte_at(c, string_end) == ' ') --string_end;
+      if (string_end - s.idx + 1 != arg) { fail = true; break; }
+      for (int32_t k = 0; k < arg; k += 8) {
+        const int32_t rest = arg - k;
+        const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
+        if (((text8(c, s.idx + k) ^ ir.u64(key_pos + k)) & m) != 0) { fail = true; break; }
+      }
+      break;
+    }
+    case kMoveToEach: {
+      if (!variable || byte_at(c, s.idx) != '[' || s.nfr >= LaneState<MAXLV>::kF) {
+        fail = true;
         break;
       }
-      --nfr;
+      const int64_t first = next_structural(c, s.level_end.get(s.lvl), s.idx + 1, s.lvl);
+      int64_t p = s.idx + 1;
+      while (p < first && byte_at(c, p) == ' ') ++p;
+      if (p == first && byte_at(c, first) == ']') { fail = true; break; } // empty array
+      s.fr_pack.set(s.nfr, (uint32_t)s.qp | ((uint32_t)s.lvl << 16) |
+                               ((uint32_t)(s.mi + 1) << 21));
+      s.fr_sep.set(s.nfr, s.idx);
+      ++s.nfr;
+      ++s.idx;
+      break;
     }
+    case kJumpIfNotArray:
+      if (arg < 0) { fail = true; break; }
+      if (byte_at(c, s.idx) != '[') s.qp += arg;
+      break;
+    default: // END, unknown opcodes, running off the bytecode
+      fail = true;
+      break;
+    }
+    if (fail) live = resume<MAXLV>(c, s);
   }
+  return s.count;
 }
 
 } // namespace
 
 // All queries of a batch over every line: each lane takes one record and runs
 // the batch's bytecodes over it in turn, so the record's bytes and bitmap
-// words are fetched once for the whole batch.
+// words are fetched once for the whole batch.  Every wave runs the same
+// number of line iterations so the interpreter's wave-wide steps see all
+// lanes.  One wave per block: the block's LDS holds that wave's arrays.
 template <int MAXLV>
-__global__ __launch_bounds__(256) void query_batch_kernel(
+__global__ __launch_bounds__(64) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
     const int64_t *__restrict__ newline, const int64_t *__restrict__ num_lines,
     int64_t max_lines, const uint64_t *__restrict__ string,
     const uint64_t *__restrict__ leveled, int64_t level_size, int32_t num_levels,
     QueryBatch batch) {
-  __shared__ uint8_t s_ir[kBatchIrBytes];
-  {
-    uint32_t total = 0;
-    for (int q = 0; q < batch.count; ++q) {
-      for (uint32_t i = threadIdx.x; i < batch.q[q].ir_len; i += blockDim.x)
-        s_ir[total + i] = batch.q[q].ir[i];
-      total += batch.q[q].ir_len;
-    }
-  }
-  __syncthreads();
   const int64_t all = *num_lines;
   const int64_t lines = all < max_lines ? all : max_lines;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; line < lines;
-       line += stride) {
-    Ctx c{file, n, string, leveled, level_size, num_levels, nullptr, 0, -1, 0,
-          nullptr, nullptr, 0, 0, false};
-    int64_t line_start = newline[line];
-    int64_t line_end = (line + 1 < all) ? newline[line + 1] : n - 1;
-    bool record = line_start >= 0 && line_start < n && line_end >= 0;
-    if (record) {
-      if (line_end >= n) line_end = n - 1;
-      while (line_end > line_start && byte_at(c, line_end) != '}') --line_end;
-      while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
-      record = line_start != line_end;
+  const int64_t wave_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  const Text text = make_text(file, n);
+  __shared__ WaveArrays<MAXLV> arrays;
+  for (int64_t base = wave_base; base < lines; base += stride) {
+    const int64_t line = base + (threadIdx.x & 63);
+    const bool valid = line < lines;
+    Ctx c{text, n, string, leveled, level_size, num_levels, -1, 0, -1, 0, -1, 0};
+    int64_t line_start = 0, line_end = 0;
+    bool record = false;
+    if (valid) {
+      line_start = newline[line];
+      line_end = (line + 1 < all) ? newline[line + 1] : n - 1;
+      record = line_start >= 0 && line_start < n && line_end >= 0;
+      if (record) {
+        if (line_end >= n) line_end = n - 1;
+        while (line_end > line_start && byte_at(c, line_end) != '}') --line_end;
+        while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
+        record = line_start != line_end;
+      }
     }
-    uint32_t ir_off = 0;
     for (int q = 0; q < batch.count; ++q) {
       const QuerySlot &qs = batch.q[q];
       int64_t *res = qs.spans + line * 2 * (int64_t)qs.cap;
-      if (!qs.variable)
+      if (valid && !qs.variable)
         for (int i = 0; i < 2 * qs.cap; ++i) res[i] = -1;
-      int64_t found = 0;
-      if (record) {
-        c.ir = s_ir + ir_off;
-        c.ir_len = qs.ir_len;
-        found = run_record<MAXLV>(c, line_start, line_end, qs.cap, qs.variable != 0, res);
-      }
-      if (qs.counts) qs.counts[line] = found;
-      ir_off += qs.ir_len;
+      const IrReader ir{reinterpret_cast<uintptr_t>(qs.ir), qs.ir_len};
+      const int64_t found =
+          run_query<MAXLV>(c, arrays, record, line_start, line_end, ir, qs.cap,
+                           qs.variable != 0, res);
+      if (valid && qs.counts) qs.counts[line] = found;
     }
   }
 }

@@ -176,14 +176,10 @@ def main():
             hip.record(ev[2], stream)
 
     def single_query(qi):
-        c = compiled[qi]
-        if variable[qi]:
-            gpj.check(L.gpj_query_run_ext(d_in.ptr, n, ctypes.byref(idx), d_irs[qi].ptr, len(c[0]),
-                                          ext_cap, grid_lines, d_spans[qi].ptr, d_counts[qi].ptr,
-                                          stream), "query")
-        else:
-            gpj.check(L.gpj_query_run(d_in.ptr, n, ctypes.byref(idx), d_irs[qi].ptr, len(c[0]),
-                                      c[2], grid_lines, d_spans[qi].ptr, stream), "query")
+        # one query alone, through the same batched entry point
+        gpj.check(L.gpj_query_run_batch(d_in.ptr, n, ctypes.byref(idx),
+                                        ctypes.byref(descs[qi]), 1, grid_lines, stream),
+                  "query")
 
     # first build: learn the line count (the reference reads it back the same
     # way, orig_index_builder.cu:325-330) and size the query grids to it

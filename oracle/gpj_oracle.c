@@ -174,7 +174,8 @@ void gpjo_build_index(const uint8_t *file, uint64_t n, int levels,
 /* line, result[line][slot] = {start, end} or -1).  Differences, all on     */
 /* inputs the reference reads out of bounds or on IR the compiler never     */
 /* emits: bytes outside [0, n) read as 0, an unknown opcode or running off  */
-/* the IR ends the line.                                                     */
+/* the IR ends the path, and so does a negative key length or jump offset   */
+/* (it would move the IR position backwards and never end).                 */
 /* ------------------------------------------------------------------------ */
 
 enum {
@@ -360,6 +361,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
       int32_t key_len = read_varint(ir, ir_len, &qp);
       int64_t key_pos = qp;
       qp += key_len;
+      if (key_len < 0) goto stop; /* malformed: would move qp backwards */
       if (qbyte(c, idx) != '{') goto stop;
       for (;;) { /* search_key */
         ++idx;
@@ -453,6 +455,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
       int32_t key_len = read_varint(ir, ir_len, &qp);
       int64_t key_pos = qp;
       qp += key_len;
+      if (key_len < 0) goto stop; /* malformed: would move qp backwards */
       while (qbyte(c, idx) == ' ' && idx < level_end[lvl]) ++idx;
       int64_t string_end = level_end[lvl] - 1;
       while (idx < string_end && qbyte(c, string_end) == ' ') --string_end;
@@ -477,6 +480,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
     }
     case OP_JUMP_IF_NOT_ARRAY: {
       int32_t off = read_varint(ir, ir_len, &qp);
+      if (off < 0) goto stop; /* malformed: would move qp backwards */
       if (qbyte(c, idx) != '[') qp += off;
       break;
     }
