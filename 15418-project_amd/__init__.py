@@ -135,11 +135,16 @@ def lib():
                                           i32, i64, vp]
         L.gpj_query_compile.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint8), u32,
                                         u32p, u32p, u32p, u32]
+        L.gpj_row_bounds_scratch_bytes.argtypes = [i64]
+        L.gpj_row_bounds_scratch_bytes.restype = u64
+        L.gpj_row_bounds.argtypes = [ctypes.POINTER(IndexT), vp, i64, vp, vp, vp]
+        L.gpj_query_run_records.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, vp,
+                                            ctypes.POINTER(QueryDescT), i32, i64, vp]
         for name in ("gpj_alloc", "gpj_free", "gpj_h2d", "gpj_d2h", "gpj_stream_sync",
                      "gpj_device_sync", "gpj_index_build", "gpj_index_free",
                      "gpj_index_build_into", "gpj_debug_escape_quote", "gpj_query_run",
                      "gpj_query_run_ext", "gpj_query_run_batch", "gpj_query_compile",
-                     "gpj_debug_copy"):
+                     "gpj_debug_copy", "gpj_row_bounds", "gpj_query_run_records"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -388,6 +393,69 @@ def run_query_batch(dev: DeviceBuffer | int, n: int, index: BuiltIndex, compiled
         counts = None
         if var:
             counts = ct.to_host(np.int64, lines, stream)
+            spans[np.arange(cap)[None, :] >= counts[:, None]] = -1
+        out.append((spans, counts))
+    return out
+
+
+def compile_element_query(query: str):
+    """Top-level-array mode: a query `$[*]<rest>` runs as `$<rest>` on every
+    element of the document's top-level array.  Returns the compiled rest
+    (ir, max_depth, slots); the index needs max_depth + 1 leveled rows."""
+    q = query.strip()
+    if not q.startswith("$[*]"):
+        raise MalformedQueryError(f"array-element query must start with $[*]: {query!r}")
+    return compile_query("$" + q[4:], extensions=True)
+
+
+def array_bounds(index: BuiltIndex, stream=None):
+    """Record bounds of a top-level array document: (DeviceBuffer of int64
+    positions, count M); there are M - 1 element records."""
+    L = lib()
+    w = index.level_size
+    scratch = DeviceBuffer(L.gpj_row_bounds_scratch_bytes(w))
+    d_count = DeviceBuffer(8)
+    # first pass learns M, second writes (capacity known)
+    cap = 1
+    while True:
+        bounds = DeviceBuffer(max(cap * 8, 8))
+        check(L.gpj_row_bounds(ctypes.byref(index.raw), bounds.ptr, cap, d_count.ptr,
+                               scratch.ptr, stream), "row bounds")
+        check(L.gpj_stream_sync(stream))
+        m = int(d_count.to_host(np.int64, 1, stream)[0])
+        if m <= cap:
+            return bounds, m
+        cap = m
+
+
+def run_query_records(dev: DeviceBuffer | int, n: int, index: BuiltIndex, bounds, num_bounds: int,
+                      compiled, capacity: int = 8, stream=None):
+    """Run compiled element queries (see compile_element_query) over the
+    top-level array's elements in one launch.  Returns, per query, spans
+    [records, cap, 2] (-1 = none) and counts (None for fixed-slot bytecode)."""
+    ptr = dev.ptr if isinstance(dev, DeviceBuffer) else int(dev)
+    records = max(num_bounds - 1, 0)
+    keep, descs = [], (QueryDescT * max(len(compiled), 1))()
+    for i, (ir, depth, slots) in enumerate(compiled):
+        var = uses_extension_opcodes(ir)
+        cap = capacity if var else slots
+        d_ir = DeviceBuffer.from_host(np.frombuffer(ir, dtype=np.uint8), stream)
+        sp = DeviceBuffer(max(records * cap * 16, 16))
+        ct = DeviceBuffer(max(records * 8, 8)) if var else None
+        keep.append((d_ir, sp, ct, cap, var))
+        descs[i] = QueryDescT(d_ir.ptr, len(ir), cap, int(var), int(depth), sp.ptr,
+                              ct.ptr if ct else None)
+    d_bounds = bounds.ptr if isinstance(bounds, DeviceBuffer) else int(bounds)
+    d_m = DeviceBuffer.from_host(np.array([num_bounds], dtype=np.int64), stream)
+    check(lib().gpj_query_run_records(ptr, int(n), ctypes.byref(index.raw), d_bounds, d_m.ptr,
+                                      descs, len(compiled), records, stream), "query records")
+    check(lib().gpj_stream_sync(stream))
+    out = []
+    for d_ir, sp, ct, cap, var in keep:
+        spans = sp.to_host(np.int64, records * cap * 2, stream).reshape(records, cap, 2)
+        counts = None
+        if var:
+            counts = ct.to_host(np.int64, records, stream)
             spans[np.arange(cap)[None, :] >= counts[:, None]] = -1
         out.append((spans, counts))
     return out

@@ -267,7 +267,7 @@ gpj_status gpj_index_build(const uint8_t *d_in, uint64_t n, int32_t levels,
     hipLaunchKernelGGL(gpj::newline_count_kernel, dim3((uint32_t)tiles), dim3(gpj::kTileWords),
                        0, st, d_in, n, aligned, counts);
     hipLaunchKernelGGL(gpj::newline_scan_kernel, dim3(1), dim3(1024), 0, st, counts, tiles,
-                       r.d_num_lines);
+                       r.d_num_lines, (int64_t)1);
     hipLaunchKernelGGL(gpj::newline_write_kernel, dim3((uint32_t)tiles), dim3(gpj::kTileWords),
                        0, st, d_in, n, aligned, counts, r.d_newline);
     e = hipStreamSynchronize(st);
@@ -293,10 +293,13 @@ gpj_status gpj_debug_escape_quote(const uint8_t *d_in, uint64_t n, uint64_t *d_e
   return GPJ_OK;
 }
 
-gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
-                               const gpj_query_desc *queries, int32_t num_queries,
-                               int64_t max_lines, void *stream) {
-  if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null index");
+namespace {
+
+// Shared by the line and the top-level-array record entry points.
+gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                     const int64_t *d_starts, const int64_t *d_num_starts,
+                     gpj::RecordMode mode, const gpj_query_desc *queries,
+                     int32_t num_queries, int64_t max_records, void *stream) {
   if (num_queries < 0 || num_queries > GPJ_MAX_BATCH_QUERIES || (num_queries > 0 && !queries))
     return fail(GPJ_ERR_INVALID_ARGUMENT,
                 "query batch holds 0.." + std::to_string(GPJ_MAX_BATCH_QUERIES) + " queries");
@@ -322,24 +325,75 @@ gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index 
     const int lv = queries[q].max_level > 0 ? queries[q].max_level : 15;
     deepest = std::max(deepest, std::min(lv, 15));
   }
-  if (max_lines <= 0 || n == 0 || num_queries == 0) return GPJ_OK;
-  if (!d_in || !index->d_newline || !index->d_num_lines || !index->d_string || !index->d_leveled)
+  if (max_records <= 0 || n == 0 || num_queries == 0) return GPJ_OK;
+  if (!d_in || !d_starts || !d_num_starts || !index->d_string || !index->d_leveled)
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "Query execution requires GPU-resident newline, string, and bitmap indices");
   // one wave per block (the kernel keeps each wave's interpreter arrays in
   // its block's LDS)
-  const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_lines, 64), 1u << 22);
+  const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_records, 64), 1u << 22);
 #define GPJ_QLAUNCH(LV)                                                                   \
   hipLaunchKernelGGL(gpj::query_batch_kernel<LV>, dim3((uint32_t)blocks), dim3(64), 0,       \
-                     as_stream(stream), d_in, (int64_t)n, index->d_newline,                  \
-                     index->d_num_lines, max_lines, index->d_string, index->d_leveled,       \
-                     index->level_size, index->num_levels, batch)
+                     as_stream(stream), d_in, (int64_t)n, d_starts, d_num_starts,            \
+                     max_records, index->d_string, index->d_leveled, index->level_size,      \
+                     index->num_levels, mode, batch)
   if (deepest + 1 <= 4) GPJ_QLAUNCH(4);
   else if (deepest + 1 <= 8) GPJ_QLAUNCH(8);
   else GPJ_QLAUNCH(16);
 #undef GPJ_QLAUNCH
   GPJ_HIP(hipGetLastError(), "query_batch_kernel launch");
   return GPJ_OK;
+}
+
+} // namespace
+
+gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                               const gpj_query_desc *queries, int32_t num_queries,
+                               int64_t max_lines, void *stream) {
+  if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null index");
+  return run_batch(d_in, n, index, index->d_newline, index->d_num_lines,
+                   gpj::RecordMode{0, 0}, queries, num_queries, max_lines, stream);
+}
+
+uint64_t gpj_row_bounds_scratch_bytes(int64_t level_size) {
+  return level_size > 0 ? div_up((uint64_t)level_size, gpj::kTileWords) * 8 : 8;
+}
+
+gpj_status gpj_row_bounds(const gpj_index *index, int64_t *d_bounds, int64_t capacity,
+                          int64_t *d_num_bounds, void *d_scratch, void *stream) {
+  if (!index || !d_num_bounds || (capacity > 0 && !d_bounds) || !d_scratch)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_row_bounds: null pointer");
+  if (!index->d_leveled || index->num_levels < 1 || index->level_size < 0)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_row_bounds: the index has no leveled rows");
+  hipStream_t st = as_stream(stream);
+  const uint64_t words = (uint64_t)index->level_size;
+  const uint64_t tiles = div_up(words, gpj::kTileWords);
+  uint64_t *counts = static_cast<uint64_t *>(d_scratch);
+  if (tiles == 0) {
+    hipLaunchKernelGGL(gpj::newline_scan_kernel, dim3(1), dim3(1024), 0, st, counts,
+                       (uint64_t)0, d_num_bounds, (int64_t)0);
+  } else {
+    hipLaunchKernelGGL(gpj::row_count_kernel, dim3((uint32_t)tiles), dim3(gpj::kTileWords), 0,
+                       st, index->d_leveled, words, counts);
+    hipLaunchKernelGGL(gpj::newline_scan_kernel, dim3(1), dim3(1024), 0, st, counts, tiles,
+                       d_num_bounds, (int64_t)0);
+    hipLaunchKernelGGL(gpj::row_write_kernel, dim3((uint32_t)tiles), dim3(gpj::kTileWords), 0,
+                       st, index->d_leveled, words, counts, d_bounds, capacity);
+  }
+  GPJ_HIP(hipGetLastError(), "row bounds launch");
+  return GPJ_OK;
+}
+
+gpj_status gpj_query_run_records(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                                 const int64_t *d_bounds, const int64_t *d_num_bounds,
+                                 const gpj_query_desc *queries, int32_t num_queries,
+                                 int64_t max_records, void *stream) {
+  if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null index");
+  if (index->num_levels < 2)
+    return fail(GPJ_ERR_QUERY_EXECUTION,
+                "array records need an index with at least 2 leveled rows");
+  return run_batch(d_in, n, index, d_bounds, d_num_bounds, gpj::RecordMode{1, 1}, queries,
+                   num_queries, max_records, stream);
 }
 
 gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,

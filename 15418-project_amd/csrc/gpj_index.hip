@@ -543,10 +543,11 @@ __global__ __launch_bounds__(kTileWords) void newline_count_kernel(
   }
 }
 
-// Exclusive scan of tile counts in one workgroup; writes the total line count.
+// Exclusive scan of tile counts in one workgroup; writes total + add (the
+// line count for the newline index: add = 1 for the synthetic first entry).
 __global__ __launch_bounds__(1024) void newline_scan_kernel(
     uint64_t *__restrict__ tile_counts, uint64_t tiles,
-    int64_t *__restrict__ num_lines_out) {
+    int64_t *__restrict__ num_lines_out, int64_t add) {
   __shared__ uint64_t s_part[1024];
   const uint32_t t = threadIdx.x;
   const uint64_t per = (tiles + 1023) / 1024;
@@ -562,7 +563,7 @@ __global__ __launch_bounds__(1024) void newline_scan_kernel(
       s_part[i] = run;
       run += v;
     }
-    *num_lines_out = (int64_t)(1 + run);
+    *num_lines_out = (int64_t)run + add;
   }
   __syncthreads();
   uint64_t run = s_part[t];
@@ -600,6 +601,47 @@ __global__ __launch_bounds__(kTileWords) void newline_write_kernel(
     const int b = __builtin_ctzll(nlm);
     nlm &= nlm - 1;
     newline_out[o++] = (int64_t)(pos + (uint64_t)b);
+  }
+}
+
+// ---- set-bit positions of one bitmap row ----------------------------------
+// Record bounds of a top-level array document are the set bits of leveled
+// row 0 (its '[', the ',' between elements and its ']').
+
+__global__ __launch_bounds__(kTileWords) void row_count_kernel(
+    const uint64_t *__restrict__ row, uint64_t words, uint64_t *__restrict__ tile_counts) {
+  __shared__ uint64_t s_sum[kTileWords / 64];
+  const uint64_t w = (uint64_t)blockIdx.x * kTileWords + threadIdx.x;
+  uint64_t c = w < words ? (uint64_t)__popcll(row[w]) : 0;
+  c = wave_reduce_add64(c);
+  if ((threadIdx.x & 63u) == 0) s_sum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t s = 0;
+    for (int k = 0; k < kTileWords / 64; ++k) s += s_sum[k];
+    tile_counts[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kTileWords) void row_write_kernel(
+    const uint64_t *__restrict__ row, uint64_t words,
+    const uint64_t *__restrict__ tile_offsets, int64_t *__restrict__ out, int64_t capacity) {
+  __shared__ uint32_t s_wscan[kTileWords / 64];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  const uint64_t w = (uint64_t)blockIdx.x * kTileWords + t;
+  uint64_t m = w < words ? row[w] : 0;
+  const uint32_t c = (uint32_t)__popcll(m);
+  const uint32_t incl = wave_incl_scan(c);
+  if (lane == 63) s_wscan[wv] = incl;
+  __syncthreads();
+  int64_t o = (int64_t)(incl - c);
+  for (uint32_t k = 0; k < wv; ++k) o += s_wscan[k];
+  o += (int64_t)tile_offsets[blockIdx.x];
+  while (m) {
+    const int b = __builtin_ctzll(m);
+    m &= m - 1;
+    if (o < capacity) out[o] = (int64_t)(w * 64 + (uint64_t)b);
+    ++o;
   }
 }
 

@@ -20,8 +20,9 @@ __global__ void index_kernel(const uint8_t *in, uint64_t n, uint64_t words,
 
 __global__ void newline_count_kernel(const uint8_t *in, uint64_t n, int aligned,
                                      uint64_t *tile_counts);
+// Exclusive scan of per-tile counts in one workgroup; writes total + add.
 __global__ void newline_scan_kernel(uint64_t *tile_counts, uint64_t tiles,
-                                    int64_t *num_lines_out);
+                                    int64_t *num_lines_out, int64_t add);
 __global__ void newline_write_kernel(const uint8_t *in, uint64_t n, int aligned,
                                      const uint64_t *tile_offsets,
                                      int64_t *newline_out);
@@ -44,10 +45,25 @@ struct QueryBatch {
   int32_t count;
 };
 
+// How records are cut: lines of a newline index (level_base 0, drop_last 0)
+// or top-level array elements between row-0 bounds (level_base 1,
+// drop_last 1).
+struct RecordMode {
+  int32_t level_base;
+  int32_t drop_last;
+};
+
 template <int MAXLV>
 __global__ void query_batch_kernel(const uint8_t *file, int64_t n, const int64_t *newline,
                                    const int64_t *num_lines, int64_t max_lines,
                                    const uint64_t *string, const uint64_t *leveled,
-                                   int64_t level_size, int32_t num_levels, QueryBatch batch);
+                                   int64_t level_size, int32_t num_levels, RecordMode mode,
+                                   QueryBatch batch);
+
+// Record bounds of a top-level array: positions of the set bits of one
+// bitmap row (count per 256-word tile, scan, write).
+__global__ void row_count_kernel(const uint64_t *row, uint64_t words, uint64_t *tile_counts);
+__global__ void row_write_kernel(const uint64_t *row, uint64_t words,
+                                 const uint64_t *tile_offsets, int64_t *out, int64_t capacity);
 
 } // namespace gpj

@@ -123,6 +123,7 @@ struct Ctx {
   const uint64_t *__restrict__ leveled;
   int64_t level_size;
   int num_levels;
+  int base;        // leveled row of interpreter level 0
   int64_t text_at; // block held in text_val (-1: none)
   uint64_t text_val;
   int64_t row_at;  // word index (into the leveled rows) held in row_val
@@ -218,9 +219,9 @@ __device__ __forceinline__ int64_t last_row_bit(Ctx &c, int64_t row, int64_t lo,
 // (find_next_structural_char, query.cu:79-115).
 __device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, int64_t idx, int level) {
   if (idx >= level_end) return level_end;
-  if (level < 0 || level >= c.num_levels) return level_end;
+  if (level < 0 || level + c.base >= c.num_levels) return level_end;
   if (idx < 0) idx = 0;
-  const int64_t row = c.level_size * (int64_t)level;
+  const int64_t row = c.level_size * (int64_t)(level + c.base);
   int64_t wi = idx >> 6, wb = wi << 6;
   uint64_t word = row_word(c, row + wi) & (~0ull << (idx - wb));
   for (;;) {
@@ -238,8 +239,8 @@ __device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, in
 // find_previous_structural_char, query.cu:117-128
 __device__ __forceinline__ int64_t prev_structural(Ctx &c, int64_t level_start, int64_t idx, int level) {
   if (idx <= level_start) return level_start;
-  if (level < 0 || level >= c.num_levels) return level_start;
-  const int64_t row = c.level_size * (int64_t)level;
+  if (level < 0 || level + c.base >= c.num_levels) return level_start;
+  const int64_t row = c.level_size * (int64_t)(level + c.base);
   const int64_t s = last_row_bit(c, row, level_start, idx);
   return s == -1 ? level_start : s;
 }
@@ -559,20 +560,26 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
 
 } // namespace
 
-// All queries of a batch over every line: each lane takes one record and runs
+// All queries of a batch over every record: each lane takes one record and runs
 // the batch's bytecodes over it in turn, so the record's bytes and bitmap
 // words are fetched once for the whole batch.  Every wave runs the same
 // number of line iterations so the interpreter's wave-wide steps see all
 // lanes.  One wave per block: the block's LDS holds that wave's arrays.
+// Records are lines (record r spans newline[r] .. newline[r + 1], the last
+// one runs to the end of the input) or, in top-level-array mode, the spans
+// between consecutive leveled-row-0 characters, with interpreter level k on
+// leveled row k + 1 (mode.level_base = 1, mode.drop_last = 1: the last bound
+// opens no record).
 template <int MAXLV>
 __global__ __launch_bounds__(64) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
     const int64_t *__restrict__ newline, const int64_t *__restrict__ num_lines,
     int64_t max_lines, const uint64_t *__restrict__ string,
     const uint64_t *__restrict__ leveled, int64_t level_size, int32_t num_levels,
-    QueryBatch batch) {
+    RecordMode mode, QueryBatch batch) {
   const int64_t all = *num_lines;
-  const int64_t lines = all < max_lines ? all : max_lines;
+  const int64_t avail = all - mode.drop_last;
+  const int64_t lines = avail < max_lines ? avail : max_lines;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t wave_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
   const Text text = make_text(file, n);
@@ -580,7 +587,7 @@ __global__ __launch_bounds__(64) void query_batch_kernel(
   for (int64_t base = wave_base; base < lines; base += stride) {
     const int64_t line = base + (threadIdx.x & 63);
     const bool valid = line < lines;
-    Ctx c{text, n, string, leveled, level_size, num_levels, -1, 0, -1, 0, -1, 0};
+    Ctx c{text, n, string, leveled, level_size, num_levels, mode.level_base, -1, 0, -1, 0, -1, 0};
     int64_t line_start = 0, line_end = 0;
     bool record = false;
     if (valid) {
@@ -608,8 +615,8 @@ __global__ __launch_bounds__(64) void query_batch_kernel(
   }
 }
 
-template __global__ void query_batch_kernel<4>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, QueryBatch);
-template __global__ void query_batch_kernel<8>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, QueryBatch);
-template __global__ void query_batch_kernel<16>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, QueryBatch);
+template __global__ void query_batch_kernel<4>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
+template __global__ void query_batch_kernel<8>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
+template __global__ void query_batch_kernel<16>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
 
 } // namespace gpj

@@ -39,6 +39,9 @@ WORKLOADS = {
     "ndjson_1GiB_4q": (2, 1 << 30, 0x15418002,
                        ["$.user.lang", "$.e[0].f", "$.h[*].t", '$.h[?(@.k == "v")].id']),
     "ndjson_64MiB_ab": (0, 64 << 20, 0x15418000, ["$.a.b"]),
+    # one 1 GiB document: a top-level array of 2M objects; its elements are
+    # the records (queries start with $[*])
+    "array_1GiB_items_sku": (1, 2_000_000, 0x15418001, ["$[*].items[*].sku"]),
 }
 
 
@@ -124,11 +127,15 @@ def main():
     n = int(data.size)
     print(f"[bench] generated {n} bytes in {gen_s:.1f}s", file=sys.stderr, flush=True)
 
-    compiled = [gpj.compile_query(q, extensions=True) for q in queries]
+    array_mode = schema == 1
+    if array_mode:
+        compiled = [gpj.compile_element_query(q) for q in queries]
+    else:
+        compiled = [gpj.compile_query(q, extensions=True) for q in queries]
     # bytecode with extension opcodes ([*], array filters) runs in variable mode
     variable = [gpj.uses_extension_opcodes(c[0]) for c in compiled]
-    ext_cap = 8
-    levels = max(1, max(c[1] for c in compiled))
+    ext_cap = 24 if array_mode else 8
+    levels = max(1, max(c[1] for c in compiled)) + (1 if array_mode else 0)
     words = (n + 63) // 64
     nl_cap = n // 256 + 2
 
@@ -161,6 +168,23 @@ def main():
         descs[qi] = gpj.QueryDescT(d_irs[qi].ptr, len(c[0]), caps[qi], int(variable[qi]), c[1],
                                    d_spans[qi].ptr, d_counts[qi].ptr if variable[qi] else None)
 
+    bounds_cap = nl_cap
+    d_bounds = gpj.DeviceBuffer(bounds_cap * 8) if array_mode else None
+    d_nbounds = gpj.DeviceBuffer(8) if array_mode else None
+    d_bscratch = (gpj.DeviceBuffer(L.gpj_row_bounds_scratch_bytes(words))
+                  if array_mode else None)
+
+    def run_queries(qdescs, nq):
+        if array_mode:
+            gpj.check(L.gpj_row_bounds(ctypes.byref(idx), d_bounds.ptr, bounds_cap,
+                                       d_nbounds.ptr, d_bscratch.ptr, stream), "row bounds")
+            gpj.check(L.gpj_query_run_records(d_in.ptr, n, ctypes.byref(idx), d_bounds.ptr,
+                                              d_nbounds.ptr, qdescs, nq, grid_lines, stream),
+                      "query records")
+        else:
+            gpj.check(L.gpj_query_run_batch(d_in.ptr, n, ctypes.byref(idx), qdescs, nq,
+                                            grid_lines, stream), "query batch")
+
     def step(ev=None):
         if ev is not None:
             hip.record(ev[0], stream)
@@ -170,24 +194,28 @@ def main():
         if ev is not None:
             hip.record(ev[1], stream)
         # all queries of the workload in one launch over the shared index
-        gpj.check(L.gpj_query_run_batch(d_in.ptr, n, ctypes.byref(idx), descs, len(compiled),
-                                        grid_lines, stream), "query batch")
+        run_queries(descs, len(compiled))
         if ev is not None:
             hip.record(ev[2], stream)
 
     def single_query(qi):
         # one query alone, through the same batched entry point
-        gpj.check(L.gpj_query_run_batch(d_in.ptr, n, ctypes.byref(idx),
-                                        ctypes.byref(descs[qi]), 1, grid_lines, stream),
-                  "query")
+        run_queries(ctypes.byref(descs[qi]), 1)
 
     # first build: learn the line count (the reference reads it back the same
     # way, orig_index_builder.cu:325-330) and size the query grids to it
     step()
     gpj.check(L.gpj_stream_sync(stream))
     num_lines = int(d_numlines.to_host(np.int64, 1)[0])
+    nl_entries = num_lines
     if num_lines > nl_cap:
         raise SystemExit("bench.py: newline capacity too small for this workload")
+    if array_mode:
+        # records are the top-level array's elements: bounds - 1 of them
+        m = int(d_nbounds.to_host(np.int64, 1)[0])
+        if m > bounds_cap:
+            raise SystemExit("bench.py: record-bound capacity too small for this workload")
+        num_lines = max(m - 1, 0)
     grid_lines = num_lines
     for _ in range(args.warmup):
         step()
@@ -215,13 +243,12 @@ def main():
         hip.record(b, stream)
         hip.sync_event(b)
         query_ms.append(hip.elapsed_ms(a, b))
-    gpj.check(L.gpj_query_run_batch(d_in.ptr, n, ctypes.byref(idx), descs, len(compiled),
-                                    grid_lines, stream), "query batch")
+    run_queries(descs, len(compiled))
     gpj.check(L.gpj_stream_sync(stream))
 
     # roofline of the dominant kernel (index): algorithmic bytes per launch
     records = num_lines
-    algo_index = n + n / 8 + levels * n / 8 + 8 * (records + 1)
+    algo_index = n + n / 8 + levels * n / 8 + 8 * nl_entries
     achieved = algo_index / (index_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
@@ -246,10 +273,20 @@ def main():
     if not args.no_cpu_baseline:
         orc = ge.load_oracle()
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        sample = cut_at_newline(data, args.cpu_sample_mib << 20)
+        if array_mode:
+            # a prefix of the array: its complete elements are the records
+            sample = data[: min(data.size, args.cpu_sample_mib << 20 >> 3)]
+        else:
+            sample = cut_at_newline(data, args.cpu_sample_mib << 20)
         t0 = time.time()
         ix = orc.build_index(sample, levels, threads)
+        if array_mode:
+            b = orc.row_bounds(ix)
+            for c, v in zip(compiled, variable):
+                orc.query_records(sample, ix, b, c[0], ext_cap if v else c[2], v)
         for c, v in zip(compiled, variable):
+            if array_mode:
+                break
             if v:
                 orc.query_ext(sample, ix, c[0], ext_cap, threads)
             else:
@@ -257,9 +294,12 @@ def main():
         cpu_s = time.time() - t0
         cpu = {"value": round(sample.size / cpu_s / 1e9, 4), "unit": "GB/s", "cores": threads,
                "kind": "port",
-               "sample": f"first {sample.size} bytes of the same data (cut at a newline), "
-                         f"index + all {len(compiled)} queries; multi-threaded CPU oracle "
-                         f"(port of the reference algorithm), {cpu_s:.2f}s wall"}
+               "sample": f"first {sample.size} bytes of the same data"
+                         f"{'' if array_mode else ' (cut at a newline)'}, index + all "
+                         f"{len(compiled)} queries; CPU oracle (port of the reference "
+                         f"algorithm; index multi-threaded"
+                         f"{', element queries on one thread' if array_mode else ''}), "
+                         f"{cpu_s:.2f}s wall"}
 
     out = {
         "metric": "JSON GB/s indexed+queried (fraction of HBM peak); p50/p99 request latency",
@@ -274,7 +314,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": args.workload, "bytes": n, "records": num_lines - 1,
+        "config": {"workload": args.workload, "bytes": n,
+                   "records": num_lines if array_mode else num_lines - 1,
                    "queries": queries, "levels": levels, "matches": matches,
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -166,6 +166,34 @@ gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index 
                                const gpj_query_desc *queries, int32_t num_queries,
                                int64_t max_lines, void *stream);
 
+/* ---- extension: top-level array elements as records (no reference
+ * counterpart; the reference treats a one-line array document as a single
+ * record and, having no [*], cannot iterate it: query_compiler.cpp:287-290).
+ * Record bounds are the structural characters of leveled row 0: for a
+ * document whose top-level value is an array, its '[', the ',' between
+ * elements and its ']'.  gpj_row_bounds writes their positions (the first
+ * `capacity` of them) to d_bounds and their number M to *d_num_bounds (device
+ * int64); there are M - 1 records and record r spans d_bounds[r] ..
+ * d_bounds[r + 1].  Scratch: gpj_row_bounds_scratch_bytes(level_size) bytes.
+ * Stream-ordered; allocates nothing. */
+uint64_t gpj_row_bounds_scratch_bytes(int64_t level_size);
+gpj_status gpj_row_bounds(const gpj_index *index, int64_t *d_bounds, int64_t capacity,
+                          int64_t *d_num_bounds, void *d_scratch, void *stream);
+
+/* Runs a query batch over those records (results [record][capacity][2]).  A
+ * record is trimmed and interpreted exactly as a line, with interpreter level
+ * k reading leveled row k + 1: the results equal line mode over the same
+ * document with its top-level ',' replaced by '\n' and its '[' ']' by ' '.
+ * Queries are compiled without the leading `$[*]` (e.g. `$[*].items[*].sku`
+ * runs as `$.items[*].sku`); the index needs one more leveled row than the
+ * query's max depth.  max_records must not exceed the bounds buffer's
+ * capacity - 1 (record r reads d_bounds[r + 1]).  Stream-ordered; allocates
+ * nothing. */
+gpj_status gpj_query_run_records(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                                 const int64_t *d_bounds, const int64_t *d_num_bounds,
+                                 const gpj_query_desc *queries, int32_t num_queries,
+                                 int64_t max_records, void *stream);
+
 /* ---- query compiler: replaces QueryCompiler::compile
  *      (src/gpjson/query/query_compiler.cpp:467-477).  Host only.
  * Writes the bytecode into ir (capacity ir_cap) and its length, the query's

@@ -192,6 +192,7 @@ typedef struct {
   const uint64_t *leveled;
   int64_t level_size;
   int num_levels;
+  int level_base; /* leveled row of interpreter level 0 (0 for lines) */
 } qctx;
 
 static inline uint8_t qbyte(const qctx *c, int64_t p) {
@@ -224,9 +225,10 @@ static int64_t prev_bit(const uint64_t *bm, int64_t lo, int64_t idx) {
 /* find_next_structural_char, query.cu:79-115 */
 static int64_t next_struct(const qctx *c, int64_t level_end, int64_t idx, int level) {
   if (idx >= level_end) return level_end;
-  if (level < 0 || level >= c->num_levels) return level_end; /* rows not built */
+  if (level < 0 || level + c->level_base >= c->num_levels) return level_end; /* rows not built */
   if (idx < 0) idx = 0;
-  const uint64_t *row = c->leveled + (uint64_t)c->level_size * (uint64_t)level;
+  const uint64_t *row =
+      c->leveled + (uint64_t)c->level_size * (uint64_t)(level + c->level_base);
   int64_t wi = idx / 64, wb = wi * 64;
   uint64_t word = row[wi] & (~0ull << (idx - wb));
   while (wb <= level_end) {
@@ -245,8 +247,9 @@ static int64_t next_struct(const qctx *c, int64_t level_end, int64_t idx, int le
 /* find_previous_structural_char, query.cu:117-128 */
 static int64_t prev_struct(const qctx *c, int64_t level_start, int64_t idx, int level) {
   if (idx <= level_start) return level_start;
-  if (level < 0 || level >= c->num_levels) return level_start; /* rows not built */
-  const uint64_t *row = c->leveled + (uint64_t)c->level_size * (uint64_t)level;
+  if (level < 0 || level + c->level_base >= c->num_levels) return level_start; /* rows not built */
+  const uint64_t *row =
+      c->leveled + (uint64_t)c->level_size * (uint64_t)(level + c->level_base);
   int64_t s = prev_bit(row, level_start, idx);
   return s == -1 ? level_start : s;
 }
@@ -519,7 +522,7 @@ void gpjo_query(const uint8_t *file, int64_t n, const int64_t *newline,
                 const uint64_t *leveled, int64_t level_size, int num_levels,
                 const uint8_t *ir, int64_t ir_len, int num_results,
                 int64_t *result) {
-  qctx c = {file, n, string, leveled, level_size, num_levels};
+  qctx c = {file, n, string, leveled, level_size, num_levels, 0};
   for (int64_t line = 0; line < num_lines; ++line)
     query_line(&c, newline, num_lines, line, ir, ir_len, num_results, 0, result, NULL);
 }
@@ -532,9 +535,42 @@ void gpjo_query_ext(const uint8_t *file, int64_t n, const int64_t *newline,
                     const uint64_t *leveled, int64_t level_size, int num_levels,
                     const uint8_t *ir, int64_t ir_len, int cap, int64_t *result,
                     int64_t *counts) {
-  qctx c = {file, n, string, leveled, level_size, num_levels};
+  qctx c = {file, n, string, leveled, level_size, num_levels, 0};
   for (int64_t line = 0; line < num_lines; ++line)
     query_line(&c, newline, num_lines, line, ir, ir_len, cap, 1, result, counts);
+}
+
+/* Extension: top-level array elements as records.  `bounds` holds the
+ * positions of every structural character of leveled row 0 (for a document
+ * whose top-level value is an array: its '[', each ',' between elements and
+ * its ']'); record r spans bounds[r] .. bounds[r + 1], so there are
+ * num_bounds - 1 records.  Each record is trimmed and interpreted exactly as a
+ * line, with interpreter level k reading leveled row k + 1.  Equivalently:
+ * the result is that of line mode over the document with its top-level
+ * ',' replaced by '\n' and its '[' ']' by ' ' (same byte offsets). */
+int64_t gpjo_row_bounds(const uint64_t *row, int64_t words, int64_t *bounds, int64_t cap) {
+  int64_t k = 0;
+  for (int64_t w = 0; w < words; ++w) {
+    uint64_t m = row[w];
+    while (m) {
+      const int b = __builtin_ctzll(m);
+      m &= m - 1;
+      if (k < cap) bounds[k] = w * 64 + b;
+      ++k;
+    }
+  }
+  return k;
+}
+
+void gpjo_query_records(const uint8_t *file, int64_t n, const int64_t *bounds,
+                        int64_t num_bounds, const uint64_t *string,
+                        const uint64_t *leveled, int64_t level_size, int num_levels,
+                        const uint8_t *ir, int64_t ir_len, int cap, int variable,
+                        int64_t *result, int64_t *counts) {
+  qctx c = {file, n, string, leveled, level_size, num_levels, 1};
+  for (int64_t r = 0; r + 1 < num_bounds; ++r)
+    query_line(&c, bounds, num_bounds, r, ir, ir_len, cap, variable, result,
+               variable ? counts : NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -761,7 +797,7 @@ void gpjo_query_mt(const uint8_t *file, int64_t n, const int64_t *newline,
     int64_t lo = t * per, hi = lo + per;
     if (hi > num_lines) hi = num_lines;
     if (lo > hi) lo = hi;
-    q[t] = (mt_query){{file, n, string, leveled, level_size, num_levels}, newline, num_lines,
+    q[t] = (mt_query){{file, n, string, leveled, level_size, num_levels, 0}, newline, num_lines,
                       lo, hi, ir, ir_len, num_results, 0, result, NULL};
     pthread_create(&tid[t], NULL, mt_query_worker, &q[t]);
   }
@@ -784,7 +820,7 @@ void gpjo_query_ext_mt(const uint8_t *file, int64_t n, const int64_t *newline,
     int64_t lo = t * per, hi = lo + per;
     if (hi > num_lines) hi = num_lines;
     if (lo > hi) lo = hi;
-    q[t] = (mt_query){{file, n, string, leveled, level_size, num_levels}, newline, num_lines,
+    q[t] = (mt_query){{file, n, string, leveled, level_size, num_levels, 0}, newline, num_lines,
                       lo, hi, ir, ir_len, cap, 1, result, counts};
     pthread_create(&tid[t], NULL, mt_query_worker, &q[t]);
   }

@@ -59,6 +59,10 @@ def lib():
         L.gpjo_query_ext_mt.argtypes = [u8p, i64, i64p, i64, u64p, u64p, i64, cint, u8p, i64, cint,
                                         i64p, i64p, cint]
         L.gpjo_partition.argtypes = [u8p, u64, u64, u64p, i64]
+        L.gpjo_row_bounds.argtypes = [u64p, i64, i64p, i64]
+        L.gpjo_row_bounds.restype = i64
+        L.gpjo_query_records.argtypes = [u8p, i64, i64p, i64, u64p, u64p, i64, cint, u8p, i64,
+                                         cint, cint, i64p, i64p]
         L.gpjo_partition.restype = i64
         _lib = L
     return _lib
@@ -191,6 +195,42 @@ def query_ext(data, index, ir: bytes, cap: int = 8, threads: int = 1):
     else:
         lib().gpjo_query_ext_mt(*args, threads)
     return res[: nl.size * cap * 2].reshape(nl.size, cap, 2), counts[: nl.size]
+
+
+def row_bounds(index) -> np.ndarray:
+    """Positions of every structural character of leveled row 0 (the record
+    bounds of a top-level array document: '[', each ',' and ']')."""
+    _, leveled, _ = index
+    row0 = np.ascontiguousarray(np.asarray(leveled)[0])
+    dummy = np.zeros(1, dtype=np.int64)
+    cnt = lib().gpjo_row_bounds(_p(_buf(row0), ctypes.c_uint64), row0.size,
+                                _p(dummy, ctypes.c_int64), 0)
+    out = np.zeros(max(cnt, 1), dtype=np.int64)
+    lib().gpjo_row_bounds(_p(_buf(row0), ctypes.c_uint64), row0.size, _p(out, ctypes.c_int64), cnt)
+    return out[:cnt]
+
+
+def query_records(data, index, bounds, ir: bytes, cap: int, variable: bool):
+    """Top-level array elements as records: record r spans bounds[r] ..
+    bounds[r + 1] and interpreter level k reads leveled row k + 1.  Returns
+    (spans int64 [records, cap, 2], counts int64 [records] or None)."""
+    f = as_bytes(data)
+    string, leveled, _ = index
+    w = words(f.size)
+    b = np.ascontiguousarray(np.asarray(bounds, dtype=np.int64))
+    nrec = max(b.size - 1, 0)
+    res = np.full(max(nrec * cap * 2, 1), -1, dtype=np.int64)
+    counts = np.zeros(max(nrec, 1), dtype=np.int64)
+    irb = np.frombuffer(bytes(ir), dtype=np.uint8).copy()
+    lv = np.asarray(leveled)
+    lib().gpjo_query_records(_p(_buf(f), ctypes.c_uint8), f.size, _p(_buf(b), ctypes.c_int64),
+                             b.size, _p(_buf(np.ascontiguousarray(string)), ctypes.c_uint64),
+                             _p(_buf(np.ascontiguousarray(lv).reshape(-1)), ctypes.c_uint64), w,
+                             int(lv.shape[0]), _p(_buf(irb), ctypes.c_uint8), irb.size, cap,
+                             int(bool(variable)), _p(res, ctypes.c_int64),
+                             _p(counts, ctypes.c_int64))
+    spans = res[: nrec * cap * 2].reshape(nrec, cap, 2)
+    return spans, (counts[:nrec] if variable else None)
 
 
 class PartitionError(Exception):
