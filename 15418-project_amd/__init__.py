@@ -90,6 +90,29 @@ class QueryDescT(ctypes.Structure):
     ]
 
 
+class BatchFileT(ctypes.Structure):
+    _fields_ = [
+        ("offset", ctypes.c_uint64),
+        ("bytes", ctypes.c_uint64),
+        ("newline_offset", ctypes.c_uint64),
+        ("newline_capacity", ctypes.c_uint64),
+        ("query_mask", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class BatchResultsT(ctypes.Structure):
+    _fields_ = [
+        ("d_spans", ctypes.c_void_p * 2),
+        ("d_counts", ctypes.c_void_p * 2),
+        ("capacity", ctypes.c_int32),
+    ]
+
+
+BATCH_ALIGN = 16384
+BATCH_SLOTS = 2
+
+
 class IndexOptsT(ctypes.Structure):
     _fields_ = [
         ("grid_size", ctypes.c_int32),
@@ -140,11 +163,31 @@ def lib():
         L.gpj_row_bounds.argtypes = [ctypes.POINTER(IndexT), vp, i64, vp, vp, vp]
         L.gpj_query_run_records.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, vp,
                                             ctypes.POINTER(QueryDescT), i32, i64, vp]
+        L.gpj_batch_scratch_bytes.argtypes = [u64]
+        L.gpj_batch_scratch_bytes.restype = u64
+        L.gpj_batch_tile_map.argtypes = [ctypes.POINTER(BatchFileT), i32, u64, vp]
+        L.gpj_batch_index.argtypes = [vp, u64, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp]
+        L.gpj_batch_query.argtypes = [vp, u64, vp, i32, i32, vp, vp, vp, vp, i64,
+                                      ctypes.POINTER(QueryDescT), i32,
+                                      ctypes.POINTER(BatchResultsT), vp]
+        L.gpj_batch_per_file.argtypes = [vp, ctypes.POINTER(BatchFileT), i32, i32, vp, vp, vp,
+                                         vp, vp, ctypes.POINTER(QueryDescT), i32,
+                                         ctypes.POINTER(BatchResultsT), vp, vp]
+        L.gpj_stream_create.argtypes = [ctypes.POINTER(vp)]
+        L.gpj_stream_destroy.argtypes = [vp]
+        L.gpj_graph_begin.argtypes = [vp]
+        L.gpj_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
+        L.gpj_graph_launch.argtypes = [vp, vp]
+        L.gpj_graph_destroy.argtypes = [vp]
         for name in ("gpj_alloc", "gpj_free", "gpj_h2d", "gpj_d2h", "gpj_stream_sync",
                      "gpj_device_sync", "gpj_index_build", "gpj_index_free",
                      "gpj_index_build_into", "gpj_debug_escape_quote", "gpj_query_run",
                      "gpj_query_run_ext", "gpj_query_run_batch", "gpj_query_compile",
-                     "gpj_debug_copy", "gpj_row_bounds", "gpj_query_run_records"):
+                     "gpj_debug_copy", "gpj_row_bounds", "gpj_query_run_records",
+                     "gpj_batch_tile_map", "gpj_batch_index", "gpj_batch_query",
+                     "gpj_batch_per_file", "gpj_graph_begin", "gpj_graph_end",
+                     "gpj_graph_launch", "gpj_graph_destroy", "gpj_stream_create",
+                     "gpj_stream_destroy"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -167,6 +210,13 @@ def check(status: int, what: str = "") -> None:
         ERR_INVALID_ARGUMENT: ValueError,
     }.get(status, GpJSONError)
     raise cls(msg)
+
+
+def new_stream():
+    """A new HIP stream (default priority, non-blocking) as an int handle."""
+    s = ctypes.c_void_p()
+    check(lib().gpj_stream_create(ctypes.byref(s)), "stream create")
+    return s.value
 
 
 def device_available() -> bool:
@@ -482,15 +532,27 @@ SCHEMA_NDJSON_1K = 2     # throughput config: ~1 KiB records, depth <= 4
 SCHEMA_TWITTER = 3       # twitter-like records
 
 
+def _gen_lib():
+    global _gen
+    if _gen is None:
+        g = ctypes.CDLL(GEN_PATH)
+        g.gpj_gen.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                              ctypes.c_void_p, ctypes.c_uint64]
+        g.gpj_gen.restype = ctypes.c_uint64
+        g.gpj_gen_files.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                                    ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.POINTER(ctypes.c_void_p)]
+        g.gpj_gen_files.restype = ctypes.c_uint64
+        g.gpj_gen_free.argtypes = [ctypes.c_void_p]
+        _gen = g
+    return _gen
+
+
 def generate(schema: int, target: int, seed: int, threads: int | None = None) -> np.ndarray:
     """Seeded synthetic JSON as a uint8 array.  NDJSON schemas: `target` is the
     byte bound; schema 1: `target` is the element count."""
-    global _gen
-    if _gen is None:
-        _gen = ctypes.CDLL(GEN_PATH)
-        _gen.gpj_gen.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
-                                 ctypes.c_void_p, ctypes.c_uint64]
-        _gen.gpj_gen.restype = ctypes.c_uint64
+    _gen_lib()
     threads = threads or min(16, os.cpu_count() or 4)
     cap = target + (1 << 20) if schema != 1 else target * 1024 + (1 << 20)
     out = np.empty(cap, dtype=np.uint8)
@@ -498,3 +560,199 @@ def generate(schema: int, target: int, seed: int, threads: int | None = None) ->
     if n == 0:
         raise RuntimeError("generator buffer too small")
     return out[:n]
+
+
+def generate_files(schemas, targets, seed: int, threads: int | None = None):
+    """Many seeded NDJSON files (schemas 0, 2 or 3; each at most its target
+    size, at least one record), laid out GPJ_BATCH_ALIGN-aligned in one
+    zero-filled buffer.  Returns (buffer uint8, offsets uint64, sizes uint64)."""
+    g = _gen_lib()
+    sch = np.ascontiguousarray(schemas, dtype=np.int32)
+    tg = np.ascontiguousarray(targets, dtype=np.uint64)
+    f = sch.size
+    offsets = np.zeros(f, dtype=np.uint64)
+    sizes = np.zeros(f, dtype=np.uint64)
+    ptr = ctypes.c_void_p()
+    threads = threads or min(16, os.cpu_count() or 4)
+    total = g.gpj_gen_files(f, sch.ctypes.data, tg.ctypes.data, seed, threads, BATCH_ALIGN,
+                            offsets.ctypes.data, sizes.ctypes.data, ctypes.byref(ptr))
+    if total == 0:
+        raise RuntimeError("gpj_gen_files failed")
+    buf = np.empty(total, dtype=np.uint8)
+    ctypes.memmove(buf.ctypes.data, ptr.value, total)
+    g.gpj_gen_free(ptr)
+    return buf, offsets, sizes
+
+
+class FileBatch:
+    """Many independent NDJSON files in one device buffer, indexed in one
+    launch and queried in one launch (gpj_batch_index + gpj_batch_query,
+    optionally captured in a HIP graph), or run with plain per-file launches
+    (gpj_batch_per_file) for comparison.  Results of both land in the same
+    layout: per slot k, spans [total lines][capacity][2] and counts [total
+    lines]; file f's lines are line_prefix[f] .. line_prefix[f + 1]."""
+
+    def __init__(self, buf: np.ndarray, offsets, sizes, masks, compiled, capacity: int = 8,
+                 stream=None):
+        L = lib()
+        self.stream = stream
+        self.n = int(buf.size)
+        self.num_files = f = len(offsets)
+        self.offsets = np.asarray(offsets, dtype=np.uint64)
+        self.sizes = np.asarray(sizes, dtype=np.uint64)
+        self.masks = np.asarray(masks, dtype=np.uint32)
+        self.compiled = compiled
+        self.capacity = capacity
+        self.levels = max(1, max(c[1] for c in compiled))
+        # exact newline slices (count + 1 entries per file)
+        nl = np.array([np.count_nonzero(buf[int(o):int(o) + int(z)] == 10)
+                       for o, z in zip(self.offsets, self.sizes)], dtype=np.int64)
+        self.line_counts = nl + 1
+        self.line_prefix = np.zeros(f + 1, dtype=np.int64)
+        np.cumsum(self.line_counts, out=self.line_prefix[1:])
+        self.total_lines = int(self.line_prefix[-1])
+        files = (BatchFileT * f)()
+        for i in range(f):
+            files[i] = BatchFileT(int(self.offsets[i]), int(self.sizes[i]),
+                                  int(self.line_prefix[i]), int(self.line_counts[i]),
+                                  int(self.masks[i]), 0)
+        self.files = files
+        self.words = (self.n + 63) // 64
+        tiles = (self.n + BATCH_ALIGN - 1) // BATCH_ALIGN
+        tile_file = np.zeros(max(tiles, 1), dtype=np.uint32)
+        check(L.gpj_batch_tile_map(files, f, self.n, tile_file.ctypes.data), "tile map")
+        self.d_in = DeviceBuffer.from_host(buf, stream)
+        self.d_files = DeviceBuffer.from_host(
+            np.frombuffer(bytes(files), dtype=np.uint8), stream)
+        self.d_tile_file = DeviceBuffer.from_host(tile_file, stream)
+        self.d_string = DeviceBuffer(self.words * 8)
+        self.d_leveled = DeviceBuffer(self.words * 8 * self.levels)
+        self.d_newline = DeviceBuffer(self.total_lines * 8)
+        self.d_prefix = DeviceBuffer((f + 1) * 8)
+        self.d_scratch = DeviceBuffer(L.gpj_batch_scratch_bytes(self.n))
+        self.descs = (QueryDescT * len(compiled))()
+        self._irs = []
+        for i, (ir, depth, slots) in enumerate(compiled):
+            if slots > capacity:
+                raise ValueError("result capacity below a query's slot count")
+            d_ir = DeviceBuffer.from_host(np.frombuffer(ir, dtype=np.uint8), stream)
+            self._irs.append(d_ir)
+            self.descs[i] = QueryDescT(d_ir.ptr, len(ir), capacity,
+                                       int(uses_extension_opcodes(ir)), int(depth), None, None)
+        self.results = self._alloc_results()
+        self.grid_records = self.total_lines
+        self.graph = None
+
+    def _alloc_results(self):
+        sp = [DeviceBuffer(max(self.total_lines * self.capacity * 16, 16)) for _ in range(2)]
+        ct = [DeviceBuffer(max(self.total_lines * 8, 8)) for _ in range(2)]
+        r = BatchResultsT()
+        for k in range(2):
+            r.d_spans[k] = sp[k].ptr
+            r.d_counts[k] = ct[k].ptr
+        r.capacity = self.capacity
+        return (r, sp, ct)
+
+    def launch(self):
+        """Batched index + query, stream-ordered (capturable)."""
+        L = lib()
+        check(L.gpj_batch_index(self.d_in.ptr, self.n, self.d_files.ptr, self.num_files,
+                                self.d_tile_file.ptr, self.levels, self.d_string.ptr,
+                                self.d_leveled.ptr, self.d_newline.ptr, self.d_prefix.ptr,
+                                self.d_scratch.ptr, self.stream), "batch index")
+        check(L.gpj_batch_query(self.d_in.ptr, self.n, self.d_files.ptr, self.num_files,
+                                self.levels, self.d_string.ptr, self.d_leveled.ptr,
+                                self.d_newline.ptr, self.d_prefix.ptr, self.grid_records,
+                                self.descs, len(self.compiled), ctypes.byref(self.results[0]),
+                                self.stream), "batch query")
+
+    def capture(self):
+        """Capture launch() in a HIP graph (needs a non-default stream)."""
+        L = lib()
+        self.release_graph()
+        check(L.gpj_graph_begin(self.stream), "graph begin")
+        try:
+            self.launch()
+        finally:
+            g = ctypes.c_void_p()
+            st = L.gpj_graph_end(self.stream, ctypes.byref(g))
+        check(st, "graph end")
+        self.graph = g.value
+
+    def launch_graph(self):
+        check(lib().gpj_graph_launch(self.graph, self.stream), "graph launch")
+
+    def release_graph(self):
+        if self.graph:
+            lib().gpj_graph_destroy(self.graph)
+            self.graph = None
+
+    def per_file_setup(self):
+        L = lib()
+        total_file_words = int(sum((int(z) + 63) // 64 for z in self.sizes))
+        self.pf_string = DeviceBuffer(self.words * 8)
+        self.pf_leveled = DeviceBuffer(max(total_file_words * 8 * self.levels, 8))
+        self.pf_newline = DeviceBuffer(self.total_lines * 8)
+        self.pf_counts = DeviceBuffer(self.num_files * 8)
+        self.pf_scratch = DeviceBuffer(L.gpj_index_scratch_bytes(int(self.sizes.max())))
+        self.pf_results = self._alloc_results()
+
+    def launch_per_file(self):
+        """Plain per-file launches (one index + one query launch per file)."""
+        check(lib().gpj_batch_per_file(self.d_in.ptr, self.files, self.num_files, self.levels,
+                                       self.pf_string.ptr, self.pf_leveled.ptr,
+                                       self.pf_newline.ptr, self.pf_counts.ptr,
+                                       self.line_prefix.ctypes.data, self.descs,
+                                       len(self.compiled), ctypes.byref(self.pf_results[0]),
+                                       self.pf_scratch.ptr, self.stream), "per-file launches")
+
+    def fetch(self, per_file: bool = False):
+        """Host copies: (slot spans list, slot counts list), entries past a
+        record's count set to -1."""
+        check(lib().gpj_stream_sync(self.stream))
+        r = self.pf_results if per_file else self.results
+        out_sp, out_ct = [], []
+        for k in range(2):
+            sp = r[1][k].to_host(np.int64, self.total_lines * self.capacity * 2, self.stream)
+            sp = sp.reshape(self.total_lines, self.capacity, 2)
+            ct = r[2][k].to_host(np.int64, self.total_lines, self.stream)
+            sp[np.arange(self.capacity)[None, :] >= np.minimum(ct, self.capacity)[:, None]] = -1
+            out_sp.append(sp)
+            out_ct.append(ct)
+        return out_sp, out_ct
+
+    def device_line_prefix(self) -> np.ndarray:
+        check(lib().gpj_stream_sync(self.stream))
+        return self.d_prefix.to_host(np.int64, self.num_files + 1, self.stream)
+
+
+def autotune_batch(batch: FileBatch, time_fn, cache_path: str | None = None):
+    """Pick the query launch size for a file batch by measurement.
+    Candidates: one grid pass over all records, or 4..32 waves per CU (256
+    CUs) looping over the records.  `time_fn(batch)` returns the time of one
+    batched run in ms.  The choice is cached by (record-count bucket, levels,
+    query count) in cache_path (JSON) when given."""
+    import json as _json
+    key = f"lines~2^{max(batch.total_lines, 1).bit_length()}/levels{batch.levels}/q{len(batch.compiled)}"
+    cache = {}
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as fh:
+            cache = _json.load(fh)
+    if key in cache:
+        batch.grid_records = int(cache[key]["grid_records"])
+        return cache[key]
+    cands = [batch.total_lines] + [64 * 256 * w for w in (4, 8, 16, 32)]
+    trials = []
+    for g in cands:
+        batch.grid_records = max(64, min(g, batch.total_lines))
+        trials.append((time_fn(batch), batch.grid_records))
+    best = min(trials)
+    batch.grid_records = best[1]
+    choice = {"grid_records": best[1], "ms": round(best[0], 4),
+              "trials": [{"grid_records": g, "ms": round(t, 4)} for t, g in trials]}
+    if cache_path:
+        cache[key] = choice
+        with open(cache_path, "w") as fh:
+            _json.dump(cache, fh, indent=1)
+    return choice
+

@@ -50,17 +50,24 @@ __global__ void empty_index_kernel(int64_t *newline, int64_t cap, int64_t *num_l
   *num_lines = 1;
 }
 
-template <int LMAX>
+template <int LMAX, bool SEG = false>
 gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int levels,
                         uint64_t *d_string, uint64_t *d_leveled, int64_t *d_newline,
                         int64_t cap, int64_t *d_num_lines, uint64_t *desc,
-                        uint32_t *counter, uint64_t tiles, hipStream_t st) {
+                        uint32_t *counter, uint64_t tiles, hipStream_t st,
+                        gpj::BatchSeg seg = gpj::BatchSeg{nullptr, nullptr, nullptr}) {
   const int aligned = (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
-  hipLaunchKernelGGL(gpj::index_kernel<LMAX>, dim3((uint32_t)tiles), dim3(gpj::kTileWords),
-                     0, st, d_in, n, words, levels, aligned, d_string, d_leveled,
-                     d_newline, cap, d_num_lines, desc, counter, tiles);
+  hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG>), dim3((uint32_t)tiles),
+                     dim3(gpj::kTileWords), 0, st, d_in, n, words, levels, aligned, d_string,
+                     d_leveled, d_newline, cap, d_num_lines, desc, counter, tiles, seg);
   GPJ_HIP(hipGetLastError(), "index_kernel launch");
   return GPJ_OK;
+}
+
+__global__ void fill_i64_kernel(int64_t *p, int64_t count, int64_t v) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+    p[i] = v;
 }
 
 // Plain 16-byte-per-lane grid-stride copy; the calibration kernel for HBM
@@ -295,6 +302,16 @@ gpj_status gpj_debug_escape_quote(const uint8_t *d_in, uint64_t n, uint64_t *d_e
 
 namespace {
 
+// Deepest interpreter level of a query batch (0 = unknown counts as 15).
+int deepest_level(const gpj_query_desc *queries, int32_t num_queries) {
+  int deepest = 0;
+  for (int32_t q = 0; q < num_queries; ++q) {
+    const int lv = queries[q].max_level > 0 ? queries[q].max_level : 15;
+    deepest = std::max(deepest, std::min(lv, 15));
+  }
+  return deepest;
+}
+
 // Shared by the line and the top-level-array record entry points.
 gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                      const int64_t *d_starts, const int64_t *d_num_starts,
@@ -314,17 +331,12 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
       return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null bytecode or result buffer");
     total_ir += d.ir_len;
     batch.q[q] = gpj::QuerySlot{d.d_ir, d.ir_len, d.capacity, d.variable ? 1 : 0, d.d_spans,
-                                d.variable ? d.d_counts : nullptr};
+                                d.d_counts};
   }
   if (total_ir > gpj::kBatchIrBytes)
     return fail(GPJ_ERR_QUERY_EXECUTION, "query batch bytecode exceeds 8192 bytes; split it");
   batch.count = num_queries;
-  // smallest interpreter variant that holds every query's deepest level
-  int deepest = 0;
-  for (int32_t q = 0; q < num_queries; ++q) {
-    const int lv = queries[q].max_level > 0 ? queries[q].max_level : 15;
-    deepest = std::max(deepest, std::min(lv, 15));
-  }
+  const int deepest = deepest_level(queries, num_queries);
   if (max_records <= 0 || n == 0 || num_queries == 0) return GPJ_OK;
   if (!d_in || !d_starts || !d_num_starts || !index->d_string || !index->d_leveled)
     return fail(GPJ_ERR_QUERY_EXECUTION,
@@ -338,6 +350,7 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                      max_records, index->d_string, index->d_leveled, index->level_size,      \
                      index->num_levels, mode, batch)
   if (deepest + 1 <= 4) GPJ_QLAUNCH(4);
+  else if (deepest + 1 <= 6) GPJ_QLAUNCH(6);
   else if (deepest + 1 <= 8) GPJ_QLAUNCH(8);
   else GPJ_QLAUNCH(16);
 #undef GPJ_QLAUNCH
@@ -409,6 +422,236 @@ gpj_status gpj_query_run_ext(const uint8_t *d_in, uint64_t n, const gpj_index *i
                              void *stream) {
   const gpj_query_desc d{d_ir, ir_len, capacity, 1, 0, d_spans, d_counts};
   return gpj_query_run_batch(d_in, n, index, &d, 1, max_lines, stream);
+}
+
+static_assert(sizeof(gpj_batch_file) == sizeof(gpj::BatchFile), "batch file layout");
+static_assert(GPJ_BATCH_ALIGN == gpj::kTileBytes, "files start on index tiles");
+static_assert(GPJ_BATCH_SLOTS == gpj::kBatchSlots, "result slots");
+
+uint64_t gpj_batch_scratch_bytes(uint64_t total_bytes) { return scratch_bytes(total_bytes); }
+
+gpj_status gpj_batch_tile_map(const gpj_batch_file *files, int32_t num_files,
+                              uint64_t total_bytes, uint32_t *tile_file) {
+  if (num_files <= 0 || !files || !tile_file)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_tile_map: empty batch or null pointer");
+  const uint64_t tiles = div_up(total_bytes, gpj::kTileBytes);
+  uint64_t next_free = 0;
+  for (int32_t f = 0; f < num_files; ++f) {
+    const gpj_batch_file &b = files[f];
+    if (b.bytes == 0)
+      return fail(GPJ_ERR_INVALID_ARGUMENT, "batch file " + std::to_string(f) + " is empty");
+    if (b.offset % gpj::kTileBytes != 0 || b.offset < next_free ||
+        b.offset + b.bytes > total_bytes)
+      return fail(GPJ_ERR_INVALID_ARGUMENT,
+                  "batch file " + std::to_string(f) +
+                      ": offsets must be GPJ_BATCH_ALIGN multiples, increasing, inside the "
+                      "buffer and not overlapping");
+    if (b.newline_capacity < 1)
+      return fail(GPJ_ERR_INVALID_ARGUMENT, "batch file newline capacity must be >= 1");
+    if (__builtin_popcount(b.query_mask) > GPJ_BATCH_SLOTS)
+      return fail(GPJ_ERR_INVALID_ARGUMENT, "a batch file runs at most 2 queries");
+    next_free = div_up(b.offset + b.bytes, gpj::kTileBytes) * gpj::kTileBytes;
+  }
+  int32_t f = 0;
+  for (uint64_t t = 0; t < tiles; ++t) {
+    while (f + 1 < num_files && files[f + 1].offset / gpj::kTileBytes <= t) ++f;
+    tile_file[t] = (uint32_t)f;
+  }
+  return GPJ_OK;
+}
+
+gpj_status gpj_batch_index(const uint8_t *d_in, uint64_t total_bytes,
+                           const gpj_batch_file *d_files, int32_t num_files,
+                           const uint32_t *d_tile_file, int32_t levels, uint64_t *d_string,
+                           uint64_t *d_leveled, int64_t *d_newline, int64_t *d_line_prefix,
+                           void *d_scratch, void *stream) {
+  if (levels < 1 || levels > GPJ_MAX_INDEX_LEVELS)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "batch index levels must be in [1, 22]");
+  if (num_files <= 0 || total_bytes == 0 || !d_in || !d_files || !d_tile_file || !d_string ||
+      !d_leveled || !d_newline || !d_line_prefix || !d_scratch)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_index: empty batch or null pointer");
+  if (reinterpret_cast<uintptr_t>(d_in) % 64 != 0)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_index: batch buffer must be 64-byte aligned");
+  hipStream_t st = as_stream(stream);
+  const uint64_t words = div_up(total_bytes, 64);
+  const uint64_t tiles = tiles_for(total_bytes);
+  if (tiles > 0xFFFFFFFFull)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "batch too large for one index launch");
+  uint64_t *desc = static_cast<uint64_t *>(d_scratch);
+  uint32_t *counter = reinterpret_cast<uint32_t *>(desc + tiles);
+  const uint64_t clear_blocks = std::min<uint64_t>(div_up(tiles, 256), 2048);
+  hipLaunchKernelGGL(gpj::index_clear_kernel, dim3((uint32_t)clear_blocks), dim3(256), 0, st,
+                     desc, tiles, counter);
+  GPJ_HIP(hipGetLastError(), "index_clear_kernel launch");
+  const gpj::BatchSeg seg{d_tile_file, reinterpret_cast<const gpj::BatchFile *>(d_files),
+                          d_line_prefix};
+  gpj_status r;
+#define GPJ_SEG(L)                                                                         \
+  r = launch_index<L, true>(d_in, total_bytes, words, levels, d_string, d_leveled, d_newline, \
+                            0, nullptr, desc, counter, tiles, st, seg)
+  if (levels <= 2) GPJ_SEG(2);
+  else if (levels <= 4) GPJ_SEG(4);
+  else if (levels <= 8) GPJ_SEG(8);
+  else if (levels <= 16) GPJ_SEG(16);
+  else GPJ_SEG(22);
+#undef GPJ_SEG
+  if (r != GPJ_OK) return r;
+  // line counts -> exclusive prefix, total in entry num_files
+  hipLaunchKernelGGL(gpj::newline_scan_kernel, dim3(1), dim3(1024), 0, st,
+                     reinterpret_cast<uint64_t *>(d_line_prefix), (uint64_t)num_files,
+                     d_line_prefix + num_files, (int64_t)0);
+  GPJ_HIP(hipGetLastError(), "line prefix launch");
+  return GPJ_OK;
+}
+
+gpj_status gpj_batch_query(const uint8_t *d_in, uint64_t total_bytes,
+                           const gpj_batch_file *d_files, int32_t num_files, int32_t levels,
+                           const uint64_t *d_string, const uint64_t *d_leveled,
+                           const int64_t *d_newline, const int64_t *d_line_prefix,
+                           int64_t max_records, const gpj_query_desc *queries,
+                           int32_t num_queries, const gpj_batch_results *results,
+                           void *stream) {
+  if (num_files <= 0 || !d_in || !d_files || !d_string || !d_leveled || !d_newline ||
+      !d_line_prefix || !results)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_query: empty batch or null pointer");
+  if (num_queries < 1 || num_queries > GPJ_MAX_BATCH_QUERIES || !queries)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_query: 1..8 queries");
+  if (results->capacity < 1)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_query: result capacity must be >= 1");
+  for (int k = 0; k < GPJ_BATCH_SLOTS; ++k)
+    if (!results->d_spans[k] || !results->d_counts[k])
+      return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_query: null result slot");
+  gpj::QueryBatch batch{};
+  uint32_t total_ir = 0;
+  for (int32_t q = 0; q < num_queries; ++q) {
+    const gpj_query_desc &d = queries[q];
+    if (d.ir_len == 0 || d.ir_len > GPJ_MAX_QUERY_IR_BYTES || !d.d_ir)
+      return fail(GPJ_ERR_QUERY_EXECUTION, "Compiled query IR exceeds the 4096-byte IR buffer");
+    total_ir += d.ir_len;
+    batch.q[q] = gpj::QuerySlot{d.d_ir, d.ir_len, results->capacity, d.variable ? 1 : 0,
+                                nullptr, nullptr};
+  }
+  if (total_ir > gpj::kBatchIrBytes)
+    return fail(GPJ_ERR_QUERY_EXECUTION, "query batch bytecode exceeds 8192 bytes; split it");
+  batch.count = num_queries;
+  if (max_records <= 0) return GPJ_OK;
+  const gpj::FilesMode fm{reinterpret_cast<const gpj::BatchFile *>(d_files), num_files,
+                          results->capacity, d_line_prefix,
+                          {results->d_spans[0], results->d_spans[1]},
+                          {results->d_counts[0], results->d_counts[1]}};
+  const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_records, 64), 1u << 22);
+  const int64_t stride = (int64_t)div_up(total_bytes, 64);
+  const int deepest = deepest_level(queries, num_queries);
+#define GPJ_FLAUNCH(LV)                                                                  \
+  hipLaunchKernelGGL(gpj::query_files_kernel<LV>, dim3((uint32_t)blocks), dim3(64), 0,    \
+                     as_stream(stream), d_in, d_newline, d_string, d_leveled, stride,    \
+                     levels, fm, batch)
+  if (deepest + 1 <= 4) GPJ_FLAUNCH(4);
+  else if (deepest + 1 <= 6) GPJ_FLAUNCH(6);
+  else if (deepest + 1 <= 8) GPJ_FLAUNCH(8);
+  else GPJ_FLAUNCH(16);
+#undef GPJ_FLAUNCH
+  GPJ_HIP(hipGetLastError(), "query_files_kernel launch");
+  return GPJ_OK;
+}
+
+gpj_status gpj_batch_per_file(const uint8_t *d_in, const gpj_batch_file *files,
+                              int32_t num_files, int32_t levels, uint64_t *d_string,
+                              uint64_t *d_leveled_per_file, int64_t *d_newline,
+                              int64_t *d_line_counts, const int64_t *line_prefix,
+                              const gpj_query_desc *queries, int32_t num_queries,
+                              const gpj_batch_results *results, void *d_scratch,
+                              void *stream) {
+  if (num_files <= 0 || !d_in || !files || !d_string || !d_leveled_per_file || !d_newline ||
+      !d_line_counts || !line_prefix || !queries || !results || !d_scratch)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_per_file: empty batch or null pointer");
+  if (num_queries < 1 || num_queries > GPJ_MAX_BATCH_QUERIES)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_per_file: 1..8 queries");
+  hipStream_t st = as_stream(stream);
+  const int32_t cap = results->capacity;
+  uint64_t lv_words = 0;
+  for (int32_t f = 0; f < num_files; ++f) {
+    const gpj_batch_file &b = files[f];
+    const uint64_t w = div_up(b.bytes, 64);
+    uint64_t *lv = d_leveled_per_file + lv_words * (uint64_t)levels;
+    lv_words += w;
+    int64_t *nl = d_newline + b.newline_offset;
+    gpj_status r = gpj_index_build_into(d_in + b.offset, b.bytes, levels,
+                                        d_string + b.offset / 64, lv, nl,
+                                        (int64_t)b.newline_capacity, d_line_counts + f,
+                                        d_scratch, stream);
+    if (r != GPJ_OK) return r;
+    gpj_query_desc descs[GPJ_BATCH_SLOTS];
+    int k = 0;
+    for (int32_t q = 0; q < num_queries && k < GPJ_BATCH_SLOTS; ++q) {
+      if (!((b.query_mask >> q) & 1u)) continue;
+      gpj_query_desc d = queries[q];
+      d.capacity = cap;
+      d.d_spans = results->d_spans[k] + line_prefix[f] * 2 * (int64_t)cap;
+      d.d_counts = results->d_counts[k] + line_prefix[f];
+      descs[k++] = d;
+    }
+    const int64_t lines = line_prefix[f + 1] - line_prefix[f];
+    for (int kk = k; kk < GPJ_BATCH_SLOTS; ++kk) {
+      hipLaunchKernelGGL(fill_i64_kernel, dim3((uint32_t)std::min<uint64_t>(div_up(lines, 256), 1024)),
+                         dim3(256), 0, st, results->d_counts[kk] + line_prefix[f], lines,
+                         (int64_t)0);
+      GPJ_HIP(hipGetLastError(), "fill launch");
+    }
+    gpj_index idx{nl, d_line_counts + f, (int64_t)b.newline_capacity, d_string + b.offset / 64,
+                  lv, (int64_t)w, levels, 0};
+    if (k > 0) {
+      r = gpj_query_run_batch(d_in + b.offset, b.bytes, &idx, descs, k, lines, stream);
+      if (r != GPJ_OK) return r;
+    }
+  }
+  return GPJ_OK;
+}
+
+gpj_status gpj_stream_create(void **stream) {
+  if (!stream) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_stream_create: null");
+  hipStream_t s = nullptr;
+  GPJ_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  *stream = s;
+  return GPJ_OK;
+}
+
+gpj_status gpj_stream_destroy(void *stream) {
+  if (stream) GPJ_HIP(hipStreamDestroy(as_stream(stream)), "hipStreamDestroy");
+  return GPJ_OK;
+}
+
+gpj_status gpj_graph_begin(void *stream) {
+  if (!stream) return fail(GPJ_ERR_INVALID_ARGUMENT, "graph capture needs a non-default stream");
+  GPJ_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal),
+          "hipStreamBeginCapture");
+  return GPJ_OK;
+}
+
+gpj_status gpj_graph_end(void *stream, void **graph_exec) {
+  if (!stream || !graph_exec) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_graph_end: null");
+  hipGraph_t g = nullptr;
+  GPJ_HIP(hipStreamEndCapture(as_stream(stream), &g), "hipStreamEndCapture");
+  hipGraphExec_t e = nullptr;
+  const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  GPJ_HIP(err, "hipGraphInstantiate");
+  *graph_exec = e;
+  return GPJ_OK;
+}
+
+gpj_status gpj_graph_launch(void *graph_exec, void *stream) {
+  if (!graph_exec) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_graph_launch: null graph");
+  GPJ_HIP(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph_exec), as_stream(stream)),
+          "hipGraphLaunch");
+  return GPJ_OK;
+}
+
+gpj_status gpj_graph_destroy(void *graph_exec) {
+  if (graph_exec)
+    GPJ_HIP(hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graph_exec)),
+            "hipGraphExecDestroy");
+  return GPJ_OK;
 }
 
 gpj_status gpj_query_compile(const char *query, uint8_t *ir, uint32_t ir_cap,

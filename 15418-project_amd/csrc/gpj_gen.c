@@ -436,3 +436,84 @@ uint64_t gpj_gen(int schema, uint64_t target, uint64_t seed, int threads, uint8_
   for (int s = 0; s < kStreams; ++s) free(jobs[s].out.p);
   return total <= cap ? total : 0;
 }
+
+/* ---- many files (the batched configuration) ------------------------------
+ * File f: NDJSON of schema schemas[f] (0, 2 or 3) from its own generator
+ * stream (seed, stream 1000 + f), records appended while the file stays
+ * within targets[f] bytes, but at least one record.  The files are laid out
+ * in one zero-filled buffer, each at an offset that is a multiple of
+ * `align`; offsets_out / sizes_out receive the layout.  Returns the buffer
+ * size (0 on allocation failure); free the buffer with gpj_gen_free. */
+typedef struct {
+  const int32_t *schemas;
+  const uint64_t *targets;
+  uint64_t seed;
+  int32_t num_files;
+  int32_t next; /* work queue */
+  pthread_mutex_t mu;
+  buf_t *bufs;
+} files_job_t;
+
+static void *files_worker(void *arg) {
+  files_job_t *j = (files_job_t *)arg;
+  buf_t tmp = {0, 0, 0};
+  for (;;) {
+    pthread_mutex_lock(&j->mu);
+    const int32_t f = j->next++;
+    pthread_mutex_unlock(&j->mu);
+    if (f >= j->num_files) break;
+    rng_t r;
+    seed_rng(&r, j->seed, 1000u + (uint64_t)f);
+    buf_t *out = &j->bufs[f];
+    for (;;) {
+      tmp.len = 0;
+      if (j->schemas[f] == 0) record0(&tmp, &r);
+      else if (j->schemas[f] == 2) record2(&tmp, &r);
+      else record3(&tmp, &r);
+      if (out->len > 0 && out->len + tmp.len > j->targets[f]) break;
+      put(out, (const char *)tmp.p, tmp.len);
+    }
+  }
+  free(tmp.p);
+  return NULL;
+}
+
+uint64_t gpj_gen_files(int32_t num_files, const int32_t *schemas, const uint64_t *targets,
+                       uint64_t seed, int threads, uint64_t align, uint64_t *offsets_out,
+                       uint64_t *sizes_out, uint8_t **buffer_out) {
+  *buffer_out = NULL;
+  if (num_files <= 0 || align == 0) return 0;
+  files_job_t j;
+  j.schemas = schemas;
+  j.targets = targets;
+  j.seed = seed;
+  j.num_files = num_files;
+  j.next = 0;
+  pthread_mutex_init(&j.mu, NULL);
+  j.bufs = (buf_t *)calloc((size_t)num_files, sizeof(buf_t));
+  if (!j.bufs) return 0;
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  pthread_t tid[256];
+  for (int t = 0; t < threads; ++t) pthread_create(&tid[t], NULL, files_worker, &j);
+  for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+  pthread_mutex_destroy(&j.mu);
+  uint64_t total = 0;
+  for (int32_t f = 0; f < num_files; ++f) {
+    offsets_out[f] = total;
+    sizes_out[f] = j.bufs[f].len;
+    total += (j.bufs[f].len + align - 1) / align * align;
+  }
+  uint8_t *buf = (uint8_t *)calloc(total ? total : 1, 1);
+  if (buf) {
+    for (int32_t f = 0; f < num_files; ++f)
+      memcpy(buf + offsets_out[f], j.bufs[f].p, j.bufs[f].len);
+  }
+  for (int32_t f = 0; f < num_files; ++f) free(j.bufs[f].p);
+  free(j.bufs);
+  *buffer_out = buf;
+  return buf ? total : 0;
+}
+
+void gpj_gen_free(uint8_t *buffer) { free(buffer); }
+

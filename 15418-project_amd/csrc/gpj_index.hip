@@ -173,6 +173,9 @@ constexpr uint32_t kMaxSpins = 1u << 24;
 // (esc, str) -> (esc, str), depth delta 0, for all four inputs
 constexpr uint64_t kIdentityTable =
     (0ull << 0) | (2ull << 10) | (1ull << 20) | (3ull << 30);
+// The state before an input's first byte as an inclusive descriptor: no
+// escape, outside strings, depth -1, no newlines.
+constexpr uint64_t kInitialInclusive = (0xFFull << 2) | (kFlagInclusive << kFlagShift);
 
 struct Carry {
   uint32_t esc, str;
@@ -245,15 +248,17 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
 
 // The fused single-pass index kernel.  One workgroup per 16 KiB tile; tile
 // ids come from an atomic counter so every predecessor a tile waits on has
-// already started.
-template <int LMAX>
+// already started.  SEG: multi-file batch (see BatchSeg); the look-back of a
+// tile never reads before its file's first tile, whose carry-in is the
+// initial state.
+template <int LMAX, bool SEG>
 __global__ __launch_bounds__(kTileWords) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
     int aligned, uint64_t *__restrict__ string_out,
     uint64_t *__restrict__ leveled_out, int64_t *__restrict__ newline_out,
     int64_t newline_cap, int64_t *__restrict__ num_lines_out,
     uint64_t *__restrict__ desc, uint32_t *__restrict__ tile_counter,
-    uint64_t num_tiles) {
+    uint64_t num_tiles, BatchSeg seg) {
   uint32_t *const error_flag = tile_counter + 1;
   __shared__ uint32_t s_tile;
   __shared__ uint8_t s_bsinfo[kTileWords];  // bit0 trailing parity, bit1 all-backslash
@@ -276,12 +281,27 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   const uint64_t tile = s_tile;
   const uint64_t w = tile * kTileWords + t;
   const uint64_t pos = w * 64;
+  // the file this tile belongs to (the whole input when !SEG)
+  uint64_t first_tile = 0, last_tile = num_tiles - 1, file_base = 0, file_end = n;
+  int64_t *nl_out = newline_out;
+  int64_t nl_cap = newline_cap;
+  int64_t *lines_out = num_lines_out;
+  if (SEG) {
+    const BatchFile bf = seg.files[seg.tile_file[tile]];
+    first_tile = bf.offset / kTileBytes;
+    last_tile = (bf.offset + bf.bytes - 1) / kTileBytes;
+    file_base = bf.offset;
+    file_end = bf.offset + bf.bytes; // bytes past it (padding) read as 0
+    nl_out = newline_out + bf.nl_offset;
+    nl_cap = (int64_t)bf.nl_capacity;
+    lines_out = seg.line_counts + seg.tile_file[tile];
+  }
 
   // 1. classify this lane's 64 bytes
   WordClasses m = {0, 0, 0, 0, 0, 0};
-  if (pos < n) {
+  if (pos < file_end) {
     uint32_t x[16];
-    load_word(in, n, pos, aligned != 0, x);
+    load_word(in, file_end, pos, aligned != 0, x);
     m = classify(x);
   }
 
@@ -377,7 +397,7 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
     excl.str = 0;
     excl.depth = -1;
     excl.nl = 0;
-    if (tile != 0) {
+    if (tile != first_tile) {
       if (lane == 0) desc_store(&desc[tile], tbl | (NL << 40) | (kFlagAggregate << kFlagShift));
       // Look back 64 predecessors at a time: lane i holds tile-1-i.  Compose
       // the transfer tables newer-first up to the nearest inclusive prefix.
@@ -390,7 +410,7 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
         uint64_t d = 0;
         uint32_t first_incl = 64;
         for (;;) {
-          d = (p >= 0) ? desc_load(&desc[p]) : (kFlagInclusive << kFlagShift);
+          d = (p >= (int64_t)first_tile) ? desc_load(&desc[p]) : kInitialInclusive;
           const uint64_t incl = __ballot((d >> kFlagShift) == kFlagInclusive);
           const uint64_t zero = __ballot((d >> kFlagShift) == 0);
           first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
@@ -433,7 +453,7 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
     if (lane == 0) {
       const Carry incl = tbl_apply(tbl, NL, excl);
       desc_store(&desc[tile], encode_inclusive(incl));
-      if (tile == num_tiles - 1) *num_lines_out = (int64_t)(1 + incl.nl);
+      if (tile == last_tile) *lines_out = (int64_t)(1 + incl.nl);
       s_carry = excl;
     }
   }
@@ -499,23 +519,28 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
       if (k < levels) leveled_out[(uint64_t)k * words + w] = s_rows[k][t];
 
     // 9. newline positions
-    if (tile == 0 && t == 0 && newline_cap > 0) newline_out[0] = 0;
+    if (tile == first_tile && t == 0 && nl_cap > 0) nl_out[0] = 0;
     uint64_t nlm = m.newline;
     int64_t o = (int64_t)nl_base;
     while (nlm) {
       const int b = __builtin_ctzll(nlm);
       nlm &= nlm - 1;
-      if (o < newline_cap) newline_out[o] = (int64_t)(pos + (uint64_t)b);
+      if (o < nl_cap) nl_out[o] = (int64_t)(pos - file_base + (uint64_t)b);
       ++o;
     }
   }
 }
 
-template __global__ void index_kernel<2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
-template __global__ void index_kernel<4>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
-template __global__ void index_kernel<8>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
-template __global__ void index_kernel<16>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
-template __global__ void index_kernel<22>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t);
+template __global__ void index_kernel<2, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<2, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<4, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<4, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<8, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<8, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<16, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<16, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<22, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<22, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
 
 // ---- exact newline index (fallback when the capacity hint was too small) ---
 

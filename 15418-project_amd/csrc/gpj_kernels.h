@@ -10,13 +10,37 @@ constexpr uint64_t kTileBytes = 64ull * kTileWords;
 
 __global__ void index_clear_kernel(uint64_t *desc, uint64_t count, uint32_t *counter);
 
-template <int LMAX>
+// One file of a multi-file batch.  Files sit in one buffer, each starting on
+// a tile boundary (offset % kTileBytes == 0) with zero bytes up to the next
+// file; a file's newline index is the slice [nl_offset, nl_offset +
+// nl_capacity) of the batch's newline array.  query_mask: bit q set = batch
+// query q runs on this file (at most kBatchSlots bits).
+struct BatchFile {
+  uint64_t offset;
+  uint64_t bytes;
+  uint64_t nl_offset;
+  uint64_t nl_capacity;
+  uint32_t query_mask;
+  uint32_t reserved;
+};
+constexpr int kBatchSlots = 2;
+
+// Multi-file mode of the index kernel: carries restart at every file's first
+// tile, newline positions are file-relative and go to the file's slice, and
+// each file's last tile writes its line count.
+struct BatchSeg {
+  const uint32_t *tile_file; // [tiles] file of each tile
+  const BatchFile *files;    // [num_files]
+  int64_t *line_counts;      // [num_files] lines per file
+};
+
+template <int LMAX, bool SEG>
 __global__ void index_kernel(const uint8_t *in, uint64_t n, uint64_t words,
                              int levels, int aligned, uint64_t *string_out,
                              uint64_t *leveled_out, int64_t *newline_out,
                              int64_t newline_cap, int64_t *num_lines_out,
                              uint64_t *desc, uint32_t *tile_counter,
-                             uint64_t num_tiles);
+                             uint64_t num_tiles, BatchSeg seg);
 
 __global__ void newline_count_kernel(const uint8_t *in, uint64_t n, int aligned,
                                      uint64_t *tile_counts);
@@ -58,6 +82,22 @@ __global__ void query_batch_kernel(const uint8_t *file, int64_t n, const int64_t
                                    const int64_t *num_lines, int64_t max_lines,
                                    const uint64_t *string, const uint64_t *leveled,
                                    int64_t level_size, int32_t num_levels, RecordMode mode,
+                                   QueryBatch batch);
+
+// Multi-file mode of the query kernel (see BatchFile).
+struct FilesMode {
+  const BatchFile *files;
+  int32_t num_files;
+  int32_t cap;                // entries per record in each slot's spans
+  const int64_t *line_prefix; // [num_files + 1] exclusive prefix of lines
+  int64_t *spans[kBatchSlots];
+  int64_t *counts[kBatchSlots];
+};
+
+template <int MAXLV>
+__global__ void query_files_kernel(const uint8_t *batch_in, const int64_t *newline,
+                                   const uint64_t *string, const uint64_t *leveled,
+                                   int64_t level_stride, int32_t num_levels, FilesMode fm,
                                    QueryBatch batch);
 
 // Record bounds of a top-level array: positions of the set bits of one

@@ -615,7 +615,89 @@ __global__ __launch_bounds__(64) void query_batch_kernel(
   }
 }
 
+// A multi-file batch (see BatchFile): record g of the batch is line
+// g - line_prefix[f] of the file f with line_prefix[f] <= g < line_prefix[f +
+// 1]; each file runs the batch queries its query_mask selects, its k-th one
+// writing result slot k ([total lines][cap][2] spans, [total lines] counts;
+// a slot no query of the file uses gets count 0).  Per file, the results are
+// those of a single-file launch over that file alone.
+template <int MAXLV>
+__global__ __launch_bounds__(64) void query_files_kernel(
+    const uint8_t *__restrict__ batch_in, const int64_t *__restrict__ newline,
+    const uint64_t *__restrict__ string, const uint64_t *__restrict__ leveled,
+    int64_t level_stride, int32_t num_levels, FilesMode fm, QueryBatch batch) {
+  const int64_t total = fm.line_prefix[fm.num_files];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t wave_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  __shared__ WaveArrays<MAXLV> arrays;
+  for (int64_t base = wave_base; base < total; base += stride) {
+    const int64_t g = base + (threadIdx.x & 63);
+    const bool valid = g < total;
+    // the file holding record g: last f with line_prefix[f] <= g
+    int32_t lo = 0, hi = fm.num_files - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (fm.line_prefix[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchFile bf = fm.files[lo];
+    const int64_t n = (int64_t)bf.bytes;
+    const int64_t line = g - fm.line_prefix[lo];
+    int64_t all = fm.line_prefix[lo + 1] - fm.line_prefix[lo];
+    if (all > (int64_t)bf.nl_capacity) all = (int64_t)bf.nl_capacity; // stay in the slice
+    const uint8_t *file = batch_in + bf.offset;
+    Text text;
+    text.blocks = reinterpret_cast<const uint64_t *>(file); // files are tile aligned
+    text.head = 0;
+    text.last = (n - 1) >> 3;
+    text.first_val = text.blocks[0];
+    const int64_t tail_bytes = n - 8 * text.last; // 1..8
+    text.last_val = text.blocks[text.last] &
+                    (tail_bytes >= 8 ? ~0ull : ((1ull << (8 * tail_bytes)) - 1));
+    if (text.last == 0) text.first_val = text.last_val;
+    const uint64_t word_off = bf.offset >> 6;
+    Ctx c{text, n, string + word_off, leveled + word_off, level_stride, num_levels, 0,
+          -1, 0, -1, 0, -1, 0};
+    const int64_t *starts = newline + bf.nl_offset;
+    int64_t line_start = 0, line_end = 0;
+    bool record = false;
+    if (valid && line < all) {
+      line_start = starts[line];
+      line_end = (line + 1 < all) ? starts[line + 1] : n - 1;
+      record = line_start >= 0 && line_start < n && line_end >= 0;
+      if (record) {
+        if (line_end >= n) line_end = n - 1;
+        while (line_end > line_start && byte_at(c, line_end) != '}') --line_end;
+        while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
+        record = line_start != line_end;
+      }
+    }
+    if (valid)
+      for (int k = 0; k < kBatchSlots; ++k) fm.counts[k][g] = 0;
+    for (int q = 0; q < batch.count; ++q) {
+      const QuerySlot &qs = batch.q[q];
+      const bool uses = valid && ((bf.query_mask >> q) & 1u);
+      const int slot = __popc(bf.query_mask & ((1u << q) - 1u));
+      const bool mine = uses && slot < kBatchSlots;
+      int64_t *res = (slot == 0 ? fm.spans[0] : fm.spans[1]) + g * 2 * (int64_t)fm.cap;
+      if (mine && !qs.variable)
+        for (int i = 0; i < 2 * fm.cap; ++i) res[i] = -1;
+      const IrReader ir{reinterpret_cast<uintptr_t>(qs.ir), qs.ir_len};
+      const int64_t found = run_query<MAXLV>(c, arrays, record && mine, line_start, line_end,
+                                             ir, qs.variable ? fm.cap : qs.cap,
+                                             qs.variable != 0, res);
+      if (mine) (slot == 0 ? fm.counts[0] : fm.counts[1])[g] = found;
+    }
+  }
+}
+
+template __global__ void query_files_kernel<4>(const uint8_t *, const int64_t *, const uint64_t *, const uint64_t *, int64_t, int32_t, FilesMode, QueryBatch);
+template __global__ void query_files_kernel<6>(const uint8_t *, const int64_t *, const uint64_t *, const uint64_t *, int64_t, int32_t, FilesMode, QueryBatch);
+template __global__ void query_files_kernel<8>(const uint8_t *, const int64_t *, const uint64_t *, const uint64_t *, int64_t, int32_t, FilesMode, QueryBatch);
+template __global__ void query_files_kernel<16>(const uint8_t *, const int64_t *, const uint64_t *, const uint64_t *, int64_t, int32_t, FilesMode, QueryBatch);
+
 template __global__ void query_batch_kernel<4>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
+template __global__ void query_batch_kernel<6>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
 template __global__ void query_batch_kernel<8>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
 template __global__ void query_batch_kernel<16>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
 

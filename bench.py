@@ -42,7 +42,12 @@ WORKLOADS = {
     # one 1 GiB document: a top-level array of 2M objects; its elements are
     # the records (queries start with $[*])
     "array_1GiB_items_sku": (1, 2_000_000, 0x15418001, ["$[*].items[*].sku"]),
+    # 10,000 NDJSON files, log-normal sizes (median 256 KiB, sigma 1.5,
+    # clamped to 1 KiB .. 256 MiB), schemas 0/2/3 mixed, 2 queries per file
+    "batch_10k_files": (-1, 10_000, 0x15418003, []),
 }
+BATCH_QUERIES = ["$.a.b", "$.a", "$.user.lang", "$.h[*].t", "$.entities.hashtags[*].text"]
+BATCH_SCHEMA_MASKS = {0: 0b00011, 2: 0b01100, 3: 0b10100}
 
 
 class Hip:
@@ -120,6 +125,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nb, "launches": 3}))
         return
     schema, nbytes, seed, queries = WORKLOADS[args.workload]
+    if schema == -1:
+        return run_file_batch(args, gpj, hip, nbytes, seed)
 
     t0 = time.time()
     data = gpj.generate(schema, nbytes, seed)
@@ -329,6 +336,129 @@ def main():
         "end_to_end_hbm_frac": round(
             (algo_index + sum(8 * (records + 1) + n / 8 + 8 * m for m in matches))
             / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_file_batch(args, gpj, hip, num_files: int, seed: int):
+    """Many files: one graph launch (batched index + batched query) per step,
+    against plain per-file launches over the same device-resident files."""
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.exp(rng.normal(np.log(256 << 10), 1.5, num_files)), 1 << 10, 256 << 20)
+    schemas = rng.choice([0, 2, 3], num_files).astype(np.int32)
+    masks = np.array([BATCH_SCHEMA_MASKS[int(x)] for x in schemas], dtype=np.uint32)
+    t0 = time.time()
+    buf, off, sz = gpj.generate_files(schemas, sizes.astype(np.uint64), seed)
+    print(f"[bench] generated {num_files} files, {int(sz.sum())} bytes in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    compiled = [gpj.compile_query(q, extensions=True) for q in BATCH_QUERIES]
+    stream = gpj.new_stream()
+    b = gpj.FileBatch(buf, off, sz, masks, compiled, capacity=8, stream=stream)
+    L = gpj.lib()
+    total = int(sz.sum())
+
+    def timed(fn, reps):
+        a, z = hip.event(), hip.event()
+        gpj.check(L.gpj_stream_sync(stream))
+        hip.record(a, stream)
+        for _ in range(reps):
+            fn()
+        hip.record(z, stream)
+        hip.sync_event(z)
+        return hip.elapsed_ms(a, z) / reps
+
+    def one_batched(batch):
+        batch.launch()
+        return timed(batch.launch, 2)
+
+    tune = gpj.autotune_batch(b, one_batched,
+                              os.path.join(ROOT, "15418-project_amd", "autotune_cache.json"))
+    # per-kernel split of one eager batched run (roofline of the index kernel)
+    e = [hip.event() for _ in range(3)]
+    gpj.check(L.gpj_stream_sync(stream))
+    hip.record(e[0], stream)
+    gpj.check(L.gpj_batch_index(b.d_in.ptr, b.n, b.d_files.ptr, b.num_files, b.d_tile_file.ptr,
+                                b.levels, b.d_string.ptr, b.d_leveled.ptr, b.d_newline.ptr,
+                                b.d_prefix.ptr, b.d_scratch.ptr, stream), "batch index")
+    hip.record(e[1], stream)
+    gpj.check(L.gpj_batch_query(b.d_in.ptr, b.n, b.d_files.ptr, b.num_files, b.levels,
+                                b.d_string.ptr, b.d_leveled.ptr, b.d_newline.ptr, b.d_prefix.ptr,
+                                b.grid_records, b.descs, len(compiled),
+                                ctypes.byref(b.results[0]), stream), "batch query")
+    hip.record(e[2], stream)
+    hip.sync_event(e[2])
+    index_ms, query_ms = hip.elapsed_ms(e[0], e[1]), hip.elapsed_ms(e[1], e[2])
+
+    b.capture()
+    for _ in range(args.warmup):
+        b.launch_graph()
+    batched_ms = timed(b.launch_graph, args.steps)
+    b.per_file_setup()
+    b.launch_per_file()
+    per_file_ms = timed(b.launch_per_file, max(1, args.steps // 4))
+    bs, bc = b.fetch()
+    ps, pc = b.fetch(per_file=True)
+    identical = all(np.array_equal(bs[k], ps[k]) and np.array_equal(bc[k], pc[k])
+                    for k in range(2))
+    if not identical:
+        raise SystemExit("bench.py: batched and per-file results differ")
+    b.release_graph()
+
+    lines = b.total_lines
+    algo_index = b.n + b.n / 8 + b.levels * b.n / 8 + 8 * lines
+    achieved = algo_index / (index_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = ge.load_oracle()
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        budget, done, t0 = args.cpu_sample_mib << 19, 0, time.time()
+        nfiles = 0
+        for f in range(num_files):
+            if done >= budget:
+                break
+            data = buf[int(off[f]):int(off[f] + sz[f])]
+            ix = orc.build_index(data, b.levels, threads)
+            for q in range(len(compiled)):
+                if (int(masks[f]) >> q) & 1:
+                    ir, _d, slots = compiled[q]
+                    if gpj.uses_extension_opcodes(ir):
+                        orc.query_ext(data, ix, ir, 8, threads)
+                    else:
+                        orc.query(data, ix, ir, 8, threads)
+            done += int(sz[f])
+            nfiles += 1
+        cpu_s = time.time() - t0
+        cpu = {"value": round(done / cpu_s / 1e9, 4), "unit": "GB/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {nfiles} files ({done} bytes), index + their 2 queries each; "
+                         f"multi-threaded CPU oracle, {cpu_s:.2f}s wall"}
+    out = {
+        "metric": "JSON GB/s indexed+queried (fraction of HBM peak); p50/p99 request latency",
+        "value": round(total / (batched_ms * 1e-3) / 1e9, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batched_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "files": num_files, "bytes": total,
+                   "buffer_bytes": b.n, "records": lines - num_files, "queries": BATCH_QUERIES,
+                   "levels": b.levels, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "gpj::index_kernel (multi-file)",
+                     "kernel_ms": round(index_ms, 4), "algorithmic_bytes": int(algo_index)},
+        "cpu_baseline": cpu,
+        "kernels_ms": {"batch_index": round(index_ms, 4), "batch_query": round(query_ms, 4)},
+        "per_file_launches": {"ms_per_step": round(per_file_ms, 4),
+                              "GBps": round(total / (per_file_ms * 1e-3) / 1e9, 3)},
+        "batched_over_per_file": round(per_file_ms / batched_ms, 2),
+        "identical_results": identical,
+        "autotune": tune,
     }
     print(json.dumps(out), flush=True)
 

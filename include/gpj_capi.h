@@ -160,7 +160,8 @@ typedef struct {
   int32_t max_level; /* the compiler's max_depth (deepest level the bytecode
                         reaches); 0 = unknown.  Picks a smaller interpreter. */
   int64_t *d_spans;  /* [max_lines][capacity][2] */
-  int64_t *d_counts; /* [max_lines], variable mode only */
+  int64_t *d_counts; /* [max_lines]: variable mode, the full count per line;
+                        fixed mode, optional (may be null): results stored */
 } gpj_query_desc;
 gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                                const gpj_query_desc *queries, int32_t num_queries,
@@ -193,6 +194,88 @@ gpj_status gpj_query_run_records(const uint8_t *d_in, uint64_t n, const gpj_inde
                                  const int64_t *d_bounds, const int64_t *d_num_bounds,
                                  const gpj_query_desc *queries, int32_t num_queries,
                                  int64_t max_records, void *stream);
+
+/* ---- extension: multi-file batches (no reference counterpart; the
+ * reference engine loops over partitions one launch sequence at a time,
+ * src/gpjson/engine.cpp:131-166).  Many independent files are indexed in one
+ * launch and queried in one launch (and the pair can be captured in a HIP
+ * graph with gpj_graph_*).  Layout: the files sit in one device buffer, file
+ * f at files[f].offset (a multiple of GPJ_BATCH_ALIGN) with files[f].bytes
+ * >= 1 bytes; bytes between files are ignored.  Per file, every index
+ * structure and query result equals that of a single-file run over the file
+ * alone (offsets are file-relative).
+ *   string mask / leveled rows: in the batch's word space (W = total/64
+ *     words; row r of the leveled rows starts at word r * W); file f's words
+ *     start at files[f].offset / 64.
+ *   newline index: file f's entries at [newline_offset, + newline_capacity)
+ *     of d_newline; newline_capacity must be >= the file's newline count + 1.
+ *   d_line_prefix [num_files + 1]: exclusive prefix of the files' line
+ *     counts; record g of the batch is line g - prefix[f] of file f.
+ *   query_mask: bit q = batch query q runs on this file, at most
+ *     GPJ_BATCH_SLOTS bits; the file's k-th query writes result slot k. */
+#define GPJ_BATCH_ALIGN 16384
+#define GPJ_BATCH_SLOTS 2
+typedef struct {
+  uint64_t offset;
+  uint64_t bytes;
+  uint64_t newline_offset;
+  uint64_t newline_capacity;
+  uint32_t query_mask;
+  uint32_t reserved;
+} gpj_batch_file;
+
+/* Scratch for gpj_batch_index: tile descriptors and counters. */
+uint64_t gpj_batch_scratch_bytes(uint64_t total_bytes);
+/* Host helper: tile_file[t] = the file holding tile t (tiles = total_bytes /
+ * GPJ_BATCH_ALIGN rounded up); tiles no file covers get the preceding file. */
+gpj_status gpj_batch_tile_map(const gpj_batch_file *files, int32_t num_files,
+                              uint64_t total_bytes, uint32_t *tile_file);
+/* One index launch over every file (d_files, d_tile_file: device arrays). */
+gpj_status gpj_batch_index(const uint8_t *d_in, uint64_t total_bytes,
+                           const gpj_batch_file *d_files, int32_t num_files,
+                           const uint32_t *d_tile_file, int32_t levels, uint64_t *d_string,
+                           uint64_t *d_leveled, int64_t *d_newline, int64_t *d_line_prefix,
+                           void *d_scratch, void *stream);
+/* One query launch over every record of every file.  queries: host array
+ * (d_spans / d_counts of the descriptors are ignored); results: slot k
+ * spans [total records][capacity][2] and counts [total records] (count 0
+ * where the file has no k-th query).  Fixed-slot queries run with
+ * `capacity` slots (>= their compiled slot count).  max_records sizes the
+ * launch (records per pass of the grid); the launch covers every record. */
+typedef struct {
+  int64_t *d_spans[GPJ_BATCH_SLOTS];
+  int64_t *d_counts[GPJ_BATCH_SLOTS];
+  int32_t capacity;
+} gpj_batch_results;
+gpj_status gpj_batch_query(const uint8_t *d_in, uint64_t total_bytes,
+                           const gpj_batch_file *d_files, int32_t num_files, int32_t levels,
+                           const uint64_t *d_string, const uint64_t *d_leveled,
+                           const int64_t *d_newline, const int64_t *d_line_prefix,
+                           int64_t max_records, const gpj_query_desc *queries,
+                           int32_t num_queries, const gpj_batch_results *results,
+                           void *stream);
+/* The same work as plain per-file launches (one index launch and one query
+ * launch per file, like the reference's per-partition loop), writing the
+ * same result layout; files: host array; line_prefix: host array of the
+ * files' exact line-count prefix.  d_leveled_per_file holds levels * (sum of
+ * the files' words) words; scratch: gpj_index_scratch_bytes(largest file). */
+gpj_status gpj_batch_per_file(const uint8_t *d_in, const gpj_batch_file *files,
+                              int32_t num_files, int32_t levels, uint64_t *d_string,
+                              uint64_t *d_leveled_per_file, int64_t *d_newline,
+                              int64_t *d_line_counts, const int64_t *line_prefix,
+                              const gpj_query_desc *queries, int32_t num_queries,
+                              const gpj_batch_results *results, void *d_scratch,
+                              void *stream);
+
+/* ---- HIP streams and graph capture of stream-ordered calls (for C callers
+ * without HIP headers).  Streams have the default priority.  graph begin/end
+ * bracket calls on `stream`; end instantiates. */
+gpj_status gpj_stream_create(void **stream);
+gpj_status gpj_stream_destroy(void *stream);
+gpj_status gpj_graph_begin(void *stream);
+gpj_status gpj_graph_end(void *stream, void **graph_exec);
+gpj_status gpj_graph_launch(void *graph_exec, void *stream);
+gpj_status gpj_graph_destroy(void *graph_exec);
 
 /* ---- query compiler: replaces QueryCompiler::compile
  *      (src/gpjson/query/query_compiler.cpp:467-477).  Host only.
