@@ -124,6 +124,97 @@ gpj_status gpj_d2h(void *h_dst, const void *d_src, uint64_t bytes, void *stream)
   return GPJ_OK;
 }
 
+gpj_status gpj_host_alloc(void **h_ptr, uint64_t bytes) {
+  if (!h_ptr) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_host_alloc: null output pointer");
+  *h_ptr = nullptr;
+  if (bytes == 0) return GPJ_OK;
+  GPJ_HIP(hipHostMalloc(h_ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+  return GPJ_OK;
+}
+
+gpj_status gpj_host_free(void *h_ptr) {
+  if (h_ptr) GPJ_HIP(hipHostFree(h_ptr), "hipHostFree");
+  return GPJ_OK;
+}
+
+gpj_status gpj_alloc_async(void **d_ptr, uint64_t bytes, void *stream) {
+  if (!d_ptr) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_alloc_async: null output pointer");
+  *d_ptr = nullptr;
+  if (bytes == 0) return GPJ_OK;
+  static bool pool_ready = false;
+  if (!pool_ready) {
+    // keep freed blocks in the pool instead of returning them at each sync
+    int dev = 0;
+    hipMemPool_t pool = nullptr;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t keep = ~0ull;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    (void)hipGetLastError();
+    pool_ready = true;
+  }
+  GPJ_HIP(hipMallocAsync(d_ptr, bytes, as_stream(stream)), "hipMallocAsync");
+  return GPJ_OK;
+}
+
+gpj_status gpj_free_async(void *d_ptr, void *stream) {
+  if (d_ptr) GPJ_HIP(hipFreeAsync(d_ptr, as_stream(stream)), "hipFreeAsync");
+  return GPJ_OK;
+}
+
+gpj_status gpj_d2h_async(void *h_dst, const void *d_src, uint64_t bytes, void *stream) {
+  if (bytes == 0) return GPJ_OK;
+  if (!h_dst || !d_src) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_d2h_async: null pointer");
+  GPJ_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, as_stream(stream)),
+          "hipMemcpyAsync D2H");
+  return GPJ_OK;
+}
+
+gpj_status gpj_event_create(void **event) {
+  if (!event) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_event_create: null");
+  hipEvent_t e = nullptr;
+  GPJ_HIP(hipEventCreate(&e), "hipEventCreate");
+  *event = e;
+  return GPJ_OK;
+}
+
+gpj_status gpj_event_record(void *event, void *stream) {
+  GPJ_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(event), as_stream(stream)),
+          "hipEventRecord");
+  return GPJ_OK;
+}
+
+gpj_status gpj_event_query(void *event) {
+  const hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(event));
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return hip_fail(e, "hipEventQuery");
+}
+
+gpj_status gpj_event_sync(void *event) {
+  GPJ_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)), "hipEventSynchronize");
+  return GPJ_OK;
+}
+
+gpj_status gpj_event_elapsed_ms(void *start, void *end, float *ms) {
+  if (!ms) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_event_elapsed_ms: null");
+  GPJ_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),
+                              reinterpret_cast<hipEvent_t>(end)),
+          "hipEventElapsedTime");
+  return GPJ_OK;
+}
+
+gpj_status gpj_event_destroy(void *event) {
+  if (event) GPJ_HIP(hipEventDestroy(reinterpret_cast<hipEvent_t>(event)), "hipEventDestroy");
+  return GPJ_OK;
+}
+
+gpj_status gpj_stream_wait_event(void *stream, void *event) {
+  GPJ_HIP(hipStreamWaitEvent(as_stream(stream), reinterpret_cast<hipEvent_t>(event), 0),
+          "hipStreamWaitEvent");
+  return GPJ_OK;
+}
+
 gpj_status gpj_stream_sync(void *stream) {
   GPJ_HIP(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
   return GPJ_OK;

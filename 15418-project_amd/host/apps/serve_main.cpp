@@ -1,0 +1,238 @@
+// gpjson_serve: Poisson request replay against gpjson::serve::Server.
+//
+// Requests: documents of 4 KiB .. 64 MiB (log-uniform) from the NDJSON
+// generator (record schemas of the 64 MiB plain and the 8 GiB throughput
+// configurations), 1..8 queries each drawn from those configurations' query
+// sets, a share of requests (default 30%) re-querying an earlier document.
+// For every arrival rate it prints one JSON line: p50/p99 latency (arrival to
+// results on the host), sustained GB/s, resident hits, evictions, and whether
+// every verified request's results equal the same request served alone.
+#include "gpjson/serve/server.hpp"
+
+#include "gpj_capi.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+extern "C" {
+uint64_t gpj_gen_files(int32_t num_files, const int32_t *schemas, const uint64_t *targets,
+                       uint64_t seed, int threads, uint64_t align, uint64_t *offsets_out,
+                       uint64_t *sizes_out, uint8_t **buffer_out);
+void gpj_gen_free(uint8_t *buffer);
+}
+
+namespace {
+
+using gpjson::serve::Request;
+using gpjson::serve::RequestResult;
+
+struct Args {
+  int requests = 1000;
+  std::vector<double> rates{500, 1000, 2000, 5000};
+  uint64_t seed = 0x15418004;
+  uint64_t min_doc = 4096, max_doc = 64ull << 20;
+  double requery = 0.3;
+  double budget_gib = 80;
+  int slots = 8;
+  int capacity = 8;
+  int verify = -1; // requests of the last rate checked against run_alone (-1 = all)
+  int threads = 16;
+};
+
+std::vector<double> parse_list(const std::string &s) {
+  std::vector<double> v;
+  std::stringstream ss(s);
+  std::string tok;
+  while (std::getline(ss, tok, ',')) v.push_back(std::stod(tok));
+  return v;
+}
+
+uint64_t parse_size(const std::string &s) {
+  char *end = nullptr;
+  double v = std::strtod(s.c_str(), &end);
+  std::string u(end);
+  if (u == "KiB" || u == "K") v *= 1024;
+  else if (u == "MiB" || u == "M") v *= 1024 * 1024;
+  else if (u == "GiB" || u == "G") v *= 1024.0 * 1024 * 1024;
+  return (uint64_t)v;
+}
+
+Args parse(int argc, char **argv) {
+  Args a;
+  for (int i = 1; i < argc; ++i) {
+    std::string k = argv[i];
+    auto val = [&]() -> std::string {
+      if (i + 1 >= argc) throw std::invalid_argument("missing value for " + k);
+      return argv[++i];
+    };
+    if (k == "--requests") a.requests = std::stoi(val());
+    else if (k == "--rates") a.rates = parse_list(val());
+    else if (k == "--seed") a.seed = std::stoull(val(), nullptr, 0);
+    else if (k == "--min-doc") a.min_doc = parse_size(val());
+    else if (k == "--max-doc") a.max_doc = parse_size(val());
+    else if (k == "--requery") a.requery = std::stod(val());
+    else if (k == "--budget-gib") a.budget_gib = std::stod(val());
+    else if (k == "--slots") a.slots = std::stoi(val());
+    else if (k == "--capacity") a.capacity = std::stoi(val());
+    else if (k == "--verify") a.verify = std::stoi(val());
+    else if (k == "--threads") a.threads = std::stoi(val());
+    else throw std::invalid_argument("unknown option " + k);
+  }
+  return a;
+}
+
+// query sets of the 64 MiB plain (schema 0) and 8 GiB throughput (schema 2)
+// configurations; indices into the server's table
+const std::vector<std::string> kQueries = {"$.a.b", "$.user.lang", "$.e[0].f", "$.h[*].t",
+                                           "$.h[?(@.k == \"v\")].id"};
+const std::vector<int> kSchema0 = {0};
+const std::vector<int> kSchema2 = {1, 2, 3, 4};
+
+double pct(std::vector<double> v, double p) {
+  if (v.empty()) return 0;
+  std::sort(v.begin(), v.end());
+  const double r = p / 100.0 * (double)(v.size() - 1);
+  const size_t lo = (size_t)r, hi = std::min(v.size() - 1, lo + 1);
+  return v[lo] + (v[hi] - v[lo]) * (r - (double)lo);
+}
+
+bool same(const RequestResult &a, const RequestResult &b) {
+  if (a.num_lines != b.num_lines || a.outputs.size() != b.outputs.size()) return false;
+  for (size_t q = 0; q < a.outputs.size(); ++q)
+    if (a.outputs[q].spans != b.outputs[q].spans || a.outputs[q].counts != b.outputs[q].counts)
+      return false;
+  return true;
+}
+
+} // namespace
+
+int main(int argc, char **argv) {
+  try {
+    const Args a = parse(argc, argv);
+    if (!gpj_device_available()) throw std::runtime_error("no HIP device");
+    std::vector<gpjson::serve::QuerySpec> table;
+    for (const auto &q : kQueries) table.push_back(gpjson::serve::compile_query_spec(q));
+
+    // the request shapes (documents and query sets) are shared by all rates
+    std::mt19937_64 rng(a.seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    struct Shape {
+      int doc;
+      std::vector<int> queries;
+    };
+    std::vector<Shape> shapes;
+    std::vector<int32_t> schemas;
+    std::vector<uint64_t> targets;
+    for (int i = 0; i < a.requests; ++i) {
+      Shape s;
+      if (!schemas.empty() && U(rng) < a.requery) {
+        s.doc = (int)(U(rng) * (double)schemas.size()) % (int)schemas.size();
+      } else {
+        s.doc = (int)schemas.size();
+        schemas.push_back(U(rng) < 0.5 ? 0 : 2);
+        const double lg = std::log((double)a.min_doc) +
+                          U(rng) * (std::log((double)a.max_doc) - std::log((double)a.min_doc));
+        targets.push_back((uint64_t)std::exp(lg));
+      }
+      const std::vector<int> &pool = schemas[s.doc] == 0 ? kSchema0 : kSchema2;
+      const int k = 1 + (int)(U(rng) * 8) % 8;
+      for (int j = 0; j < k; ++j) s.queries.push_back(pool[(size_t)(U(rng) * pool.size()) % pool.size()]);
+      shapes.push_back(std::move(s));
+    }
+    std::vector<uint64_t> off(schemas.size()), sz(schemas.size());
+    uint8_t *gen = nullptr;
+    const uint64_t total = gpj_gen_files((int32_t)schemas.size(), schemas.data(), targets.data(),
+                                         a.seed, a.threads, 64, off.data(), sz.data(), &gen);
+    if (!total) throw std::runtime_error("document generation failed");
+    uint8_t *docs = nullptr; // pinned, so uploads overlap other slots' kernels
+    if (gpj_host_alloc(reinterpret_cast<void **>(&docs), total) < 0)
+      throw std::runtime_error(gpj_last_error());
+    std::memcpy(docs, gen, total);
+    gpj_gen_free(gen);
+    uint64_t doc_bytes_total = 0;
+    for (auto v : sz) doc_bytes_total += v;
+    std::fprintf(stderr, "[serve] %d requests over %zu documents (%llu bytes)\n", a.requests,
+                 schemas.size(), (unsigned long long)doc_bytes_total);
+
+    for (size_t ri = 0; ri < a.rates.size(); ++ri) {
+      const double rate = a.rates[ri];
+      std::mt19937_64 arng(a.seed ^ (uint64_t)(rate * 1000));
+      std::exponential_distribution<double> gap(rate);
+      std::vector<Request> reqs(a.requests);
+      double t = 0;
+      uint64_t req_bytes = 0, new_bytes = 0;
+      std::vector<char> seen(schemas.size(), 0);
+      for (int i = 0; i < a.requests; ++i) {
+        t += gap(arng);
+        Request &r = reqs[i];
+        r.id = (uint64_t)i;
+        r.doc_id = (uint64_t)shapes[i].doc;
+        r.doc = docs + off[shapes[i].doc];
+        r.doc_bytes = sz[shapes[i].doc];
+        r.queries = shapes[i].queries;
+        r.arrival_s = t;
+        req_bytes += r.doc_bytes;
+        if (!seen[shapes[i].doc]) new_bytes += r.doc_bytes;
+        seen[shapes[i].doc] = 1;
+      }
+      gpjson::serve::ServerOptions so;
+      so.hbm_budget_bytes = (uint64_t)(a.budget_gib * (double)(1ull << 30));
+      so.slots = a.slots;
+      so.result_capacity = a.capacity;
+      gpjson::serve::Server server(table, so);
+      const std::vector<RequestResult> res = server.run(reqs);
+      std::vector<double> lat;
+      double first = 1e300, last = 0;
+      uint64_t resident = 0;
+      for (const auto &r : res) {
+        lat.push_back((r.done_s - r.arrival_s) * 1e3);
+        first = std::min(first, r.arrival_s);
+        last = std::max(last, r.done_s);
+        resident += r.resident ? 1 : 0;
+      }
+      const double span = last - first;
+      int checked = 0, identical = 1;
+      const bool last_rate = ri + 1 == a.rates.size();
+      const int want = last_rate ? (a.verify < 0 ? a.requests : std::min(a.verify, a.requests))
+                                 : std::min(a.requests, 50);
+      server.clear_resident();
+      for (int i = 0; i < want; ++i) {
+        const RequestResult alone = server.run_alone(reqs[i]);
+        if (!same(alone, res[i])) {
+          identical = 0;
+          std::fprintf(stderr, "[serve] request %d differs from its solo run\n", i);
+        }
+        ++checked;
+      }
+      const auto &st = server.stats();
+      std::printf(
+          "{\"metric\": \"serving p50/p99 request latency and sustained JSON GB/s\", "
+          "\"rate_per_s\": %.0f, \"requests\": %d, \"documents\": %zu, \"p50_ms\": %.3f, "
+          "\"p99_ms\": %.3f, \"mean_ms\": %.3f, \"sustained_GBps\": %.3f, "
+          "\"new_document_GBps\": %.3f, \"makespan_s\": %.4f, \"resident_hits\": %llu, "
+          "\"evictions\": %llu, \"retries\": %llu, \"peak_device_bytes\": %llu, "
+          "\"slots\": %d, \"hbm_budget_bytes\": %llu, \"verified\": %d, \"identical\": %s}\n",
+          rate, a.requests, schemas.size(), pct(lat, 50), pct(lat, 99),
+          [&] { double m = 0; for (double x : lat) m += x; return lat.empty() ? 0 : m / lat.size(); }(),
+          (double)req_bytes / span / 1e9, (double)new_bytes / span / 1e9, span,
+          (unsigned long long)resident, (unsigned long long)st.evictions,
+          (unsigned long long)st.retries, (unsigned long long)st.peak_bytes, a.slots,
+          (unsigned long long)so.hbm_budget_bytes, checked, identical ? "true" : "false");
+      std::fflush(stdout);
+      if (!identical) return 3;
+    }
+    gpj_host_free(docs);
+    return 0;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "gpjson_serve: %s\n", e.what());
+    return 2;
+  }
+}

@@ -51,6 +51,27 @@ gpj_status gpj_d2h(void *h_dst, const void *d_src, uint64_t bytes, void *stream)
 gpj_status gpj_stream_sync(void *stream);
 gpj_status gpj_device_sync(void);
 
+/* ---- asynchronous host runtime (no reference counterpart: the reference
+ * copies synchronously from pageable memory, src/gpjson/file/file.cpp:30-36).
+ * Pinned host memory makes gpj_h2d / gpj_d2h_async truly asynchronous, so
+ * copies overlap kernels on other streams.  gpj_alloc_async / gpj_free_async
+ * are stream-ordered (a device memory pool that keeps freed memory cached).
+ * gpj_event_query returns 1 when the event has completed, 0 while pending,
+ * or a negative status. */
+gpj_status gpj_host_alloc(void **h_ptr, uint64_t bytes);
+gpj_status gpj_host_free(void *h_ptr);
+gpj_status gpj_alloc_async(void **d_ptr, uint64_t bytes, void *stream);
+gpj_status gpj_free_async(void *d_ptr, void *stream);
+gpj_status gpj_d2h_async(void *h_dst, const void *d_src, uint64_t bytes, void *stream);
+gpj_status gpj_event_create(void **event);
+gpj_status gpj_event_record(void *event, void *stream);
+gpj_status gpj_event_query(void *event);
+gpj_status gpj_event_sync(void *event);
+gpj_status gpj_event_elapsed_ms(void *start, void *end, float *ms);
+gpj_status gpj_event_destroy(void *event);
+/* Later work on `stream` waits for `event` (recorded on another stream). */
+gpj_status gpj_stream_wait_event(void *stream, void *event);
+
 /* ---- structural index: replaces IndexBuilder::build
  *      (include/gpjson/index/index_builder.hpp:55-63; kernels of
  *      src/gpjson/index/orig_index_builder.cu:224-395) */
