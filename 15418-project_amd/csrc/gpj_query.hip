@@ -134,15 +134,15 @@ struct Ctx {
 
 // Block b of the input (blocks outside the input read as 0).
 __device__ __forceinline__ uint64_t text_block(const Ctx &c, int64_t b) {
-  const int64_t bc = b < 0 ? 0 : (b > c.t.last ? c.t.last : b);
-  uint64_t v = c.t.blocks[bc < 0 ? 0 : bc];
-  v = (b == 0) ? c.t.first_val : v;
-  v = (b == c.t.last) ? c.t.last_val : v;
-  return (b < 0 || b > c.t.last) ? 0 : v;
+  if (b > 0 && b < c.t.last) return c.t.blocks[b]; // interior block
+  if (b == 0) return c.t.first_val;
+  if (b == c.t.last) return c.t.last_val;
+  return 0;
 }
 
+// Byte p of the input; bytes outside [0, n) read as 0 (the first and last
+// blocks are stored with those bytes cleared, blocks beyond them read as 0).
 __device__ __forceinline__ uint8_t byte_at(Ctx &c, int64_t p) {
-  if ((uint64_t)p >= (uint64_t)c.n) return 0;
   const int64_t a = p + c.t.head;
   const int64_t b = a >> 3;
   if (b != c.text_at) {
