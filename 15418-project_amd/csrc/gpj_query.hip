@@ -85,29 +85,49 @@ struct IrReader {
   }
 };
 
-// The input as aligned 8-byte blocks.  Block b holds file offsets
-// [8b - head, 8b - head + 8), where head is the input address modulo 8.  The
-// first and last blocks may hold bytes outside [0, n); they are assembled
-// once (those bytes read as 0), every block between is loaded whole.
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+// The input as aligned 16-byte granules.  Granule g holds file offsets
+// [16g - head, 16g - head + 16), where head is the input address modulo 16.
+// The first and last granules may hold bytes outside [0, n); they are
+// assembled once with those bytes cleared, every granule between is loaded
+// whole.
 struct Text {
-  const uint64_t *__restrict__ blocks;
+  const u64x2 *__restrict__ blocks;
   int64_t head;
-  int64_t last; // index of the last block that holds a byte of the input
-  uint64_t first_val, last_val;
+  int64_t last; // index of the last granule that holds a byte of the input
+  u64x2 first_val, last_val;
 };
+
+__device__ __forceinline__ u64x2 clear_past(u64x2 v, int64_t valid) {
+  // keep the first `valid` bytes (0..16) of a granule
+  const uint64_t m0 = valid >= 8 ? ~0ull : (valid <= 0 ? 0ull : (1ull << (8 * valid)) - 1);
+  const int64_t v1 = valid - 8;
+  const uint64_t m1 = v1 >= 8 ? ~0ull : (v1 <= 0 ? 0ull : (1ull << (8 * v1)) - 1);
+  v.x &= m0;
+  v.y &= m1;
+  return v;
+}
 
 __device__ __forceinline__ Text make_text(const uint8_t *__restrict__ file, int64_t n) {
   Text t;
-  const int64_t head = (int64_t)(reinterpret_cast<uintptr_t>(file) & 7);
+  const int64_t head = (int64_t)(reinterpret_cast<uintptr_t>(file) & 15);
   t.head = head;
-  t.blocks = reinterpret_cast<const uint64_t *>(file - head);
-  t.last = n > 0 ? (n - 1 + head) >> 3 : -1;
-  t.first_val = 0;
-  t.last_val = 0;
-  for (int64_t k = 0; k < 8; ++k) {
-    const int64_t p0 = k - head, p1 = 8 * t.last + k - head;
-    if (p0 >= 0 && p0 < n) t.first_val |= (uint64_t)file[p0] << (8 * k);
-    if (p1 >= 0 && p1 < n) t.last_val |= (uint64_t)file[p1] << (8 * k);
+  t.blocks = reinterpret_cast<const u64x2 *>(file - head);
+  t.last = n > 0 ? (n - 1 + head) >> 4 : -1;
+  t.first_val = u64x2{0, 0};
+  t.last_val = u64x2{0, 0};
+  for (int64_t k = 0; k < 16; ++k) {
+    const int64_t p0 = k - head, p1 = 16 * t.last + k - head;
+    const uint64_t sh = 8 * (k & 7);
+    if (p0 >= 0 && p0 < n) {
+      if (k < 8) t.first_val.x |= (uint64_t)file[p0] << sh;
+      else t.first_val.y |= (uint64_t)file[p0] << sh;
+    }
+    if (p1 >= 0 && p1 < n) {
+      if (k < 8) t.last_val.x |= (uint64_t)file[p1] << sh;
+      else t.last_val.y |= (uint64_t)file[p1] << sh;
+    }
   }
   return t;
 }
@@ -124,57 +144,70 @@ struct Ctx {
   int64_t level_size;
   int num_levels;
   int base;        // leveled row of interpreter level 0
-  int64_t text_at; // block held in text_val (-1: none)
-  uint64_t text_val;
-  int64_t row_at;  // word index (into the leveled rows) held in row_val
-  uint64_t row_val;
-  int64_t str_at;  // word index held in str_val
-  uint64_t str_val;
+  int64_t text_at; // granule held in text_val (-1: none)
+  u64x2 text_val;
+  int64_t row_at;  // first word index of the 16-byte granule in row_val
+  u64x2 row_val;
+  int64_t str_at;  // same for the string mask
+  u64x2 str_val;
 };
 
-// Block b of the input (blocks outside the input read as 0).
-__device__ __forceinline__ uint64_t text_block(const Ctx &c, int64_t b) {
-  if (b > 0 && b < c.t.last) return c.t.blocks[b]; // interior block
-  if (b == 0) return c.t.first_val;
-  if (b == c.t.last) return c.t.last_val;
-  return 0;
+// Granule g of the input (granules outside the input read as 0).
+__device__ __forceinline__ u64x2 text_granule(const Ctx &c, int64_t g) {
+  if (g > 0 && g < c.t.last) return c.t.blocks[g]; // interior granule
+  if (g == 0) return c.t.first_val;
+  if (g == c.t.last) return c.t.last_val;
+  return u64x2{0, 0};
 }
 
 // Byte p of the input; bytes outside [0, n) read as 0 (the first and last
-// blocks are stored with those bytes cleared, blocks beyond them read as 0).
+// granules are stored with those bytes cleared, granules beyond them read
+// as 0).  The lane keeps the last granule read.
 __device__ __forceinline__ uint8_t byte_at(Ctx &c, int64_t p) {
   const int64_t a = p + c.t.head;
-  const int64_t b = a >> 3;
-  if (b != c.text_at) {
-    c.text_val = text_block(c, b);
-    c.text_at = b;
+  const int64_t g = a >> 4;
+  if (g != c.text_at) {
+    c.text_val = text_granule(c, g);
+    c.text_at = g;
   }
-  return (uint8_t)(c.text_val >> (8 * (a & 7)));
+  const uint64_t w = (a & 8) ? c.text_val.y : c.text_val.x;
+  return (uint8_t)(w >> (8 * (a & 7)));
 }
 
-// Little-endian 8 text bytes from p (bytes outside [0, n) read as 0).
+// Little-endian 8 text bytes from p (bytes outside [0, n) read as 0), from
+// the lane's granule when it holds them.
 __device__ __forceinline__ uint64_t text8(const Ctx &c, int64_t p) {
   const int64_t a = p + c.t.head;
-  const int64_t b = a >> 3;
-  const uint32_t sh = 8 * (uint32_t)(a & 7);
-  const uint64_t lo = text_block(c, b);
-  const uint64_t hi = text_block(c, b + 1);
-  return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+  const int64_t g = a >> 4;
+  const int64_t o = a & 15;
+  u64x2 v0 = (g == c.text_at) ? c.text_val : text_granule(c, g);
+  if (o == 0) return v0.x;
+  if (o == 8) return v0.y;
+  const uint32_t sh = 8 * (uint32_t)(o & 7);
+  if (o < 8) return (v0.x >> sh) | (v0.y << (64 - sh));
+  const u64x2 v1 = (g + 1 == c.text_at) ? c.text_val : text_granule(c, g + 1);
+  return (v0.y >> sh) | (v1.x << (64 - sh));
 }
 
-__device__ __forceinline__ uint64_t row_word(Ctx &c, int64_t wi) {
-  if (wi != c.row_at) {
-    c.row_val = c.leveled[wi];
-    c.row_at = wi;
+// Word wi of an 8-byte-aligned bitmap, read as part of the 16-byte aligned
+// granule that holds it (the granule's other word may lie just outside the
+// bitmap, inside the same aligned 16 bytes); the lane keeps one granule per
+// bitmap.
+__device__ __forceinline__ uint64_t granule_word(const uint64_t *__restrict__ base, int64_t wi,
+                                                 int64_t &at, u64x2 &val) {
+  const int64_t odd = (int64_t)((reinterpret_cast<uintptr_t>(base + wi) >> 3) & 1);
+  const int64_t g = wi - odd;
+  if (g != at) {
+    val = *reinterpret_cast<const u64x2 *>(base + g);
+    at = g;
   }
-  return c.row_val;
+  return odd ? val.y : val.x;
+}
+__device__ __forceinline__ uint64_t row_word(Ctx &c, int64_t wi) {
+  return granule_word(c.leveled, wi, c.row_at, c.row_val);
 }
 __device__ __forceinline__ uint64_t str_word(Ctx &c, int64_t wi) {
-  if (wi != c.str_at) {
-    c.str_val = c.string[wi];
-    c.str_at = wi;
-  }
-  return c.str_val;
+  return granule_word(c.string, wi, c.str_at, c.str_val);
 }
 
 __device__ __forceinline__ uint64_t through(int b) {
@@ -587,7 +620,8 @@ __global__ __launch_bounds__(64) void query_batch_kernel(
   for (int64_t base = wave_base; base < lines; base += stride) {
     const int64_t line = base + (threadIdx.x & 63);
     const bool valid = line < lines;
-    Ctx c{text, n, string, leveled, level_size, num_levels, mode.level_base, -1, 0, -1, 0, -1, 0};
+    Ctx c{text, n, string, leveled, level_size, num_levels, mode.level_base, -1, u64x2{0, 0},
+          INT64_MIN, u64x2{0, 0}, INT64_MIN, u64x2{0, 0}};
     int64_t line_start = 0, line_end = 0;
     bool record = false;
     if (valid) {
@@ -647,17 +681,15 @@ __global__ __launch_bounds__(64) void query_files_kernel(
     if (all > (int64_t)bf.nl_capacity) all = (int64_t)bf.nl_capacity; // stay in the slice
     const uint8_t *file = batch_in + bf.offset;
     Text text;
-    text.blocks = reinterpret_cast<const uint64_t *>(file); // files are tile aligned
+    text.blocks = reinterpret_cast<const u64x2 *>(file); // files are tile aligned
     text.head = 0;
-    text.last = (n - 1) >> 3;
+    text.last = (n - 1) >> 4;
     text.first_val = text.blocks[0];
-    const int64_t tail_bytes = n - 8 * text.last; // 1..8
-    text.last_val = text.blocks[text.last] &
-                    (tail_bytes >= 8 ? ~0ull : ((1ull << (8 * tail_bytes)) - 1));
+    text.last_val = clear_past(text.blocks[text.last], n - 16 * text.last);
     if (text.last == 0) text.first_val = text.last_val;
     const uint64_t word_off = bf.offset >> 6;
     Ctx c{text, n, string + word_off, leveled + word_off, level_stride, num_levels, 0,
-          -1, 0, -1, 0, -1, 0};
+          -1, u64x2{0, 0}, INT64_MIN, u64x2{0, 0}, INT64_MIN, u64x2{0, 0}};
     const int64_t *starts = newline + bf.nl_offset;
     int64_t line_start = 0, line_end = 0;
     bool record = false;
