@@ -429,6 +429,10 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
   batch.count = num_queries;
   const int deepest = deepest_level(queries, num_queries);
   if (max_records <= 0 || n == 0 || num_queries == 0) return GPJ_OK;
+  if (n >= (1ull << 35) ||
+      (uint64_t)std::max(index->num_levels, 1) * (uint64_t)index->level_size >= 0xFFFFFFF0ull)
+    return fail(GPJ_ERR_QUERY_EXECUTION,
+                "query input above 32 GiB per launch; split it into partitions");
   if (!d_in || !d_starts || !d_num_starts || !index->d_string || !index->d_leveled)
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "Query execution requires GPU-resident newline, string, and bitmap indices");
@@ -626,6 +630,10 @@ gpj_status gpj_batch_query(const uint8_t *d_in, uint64_t total_bytes,
     return fail(GPJ_ERR_QUERY_EXECUTION, "query batch bytecode exceeds 8192 bytes; split it");
   batch.count = num_queries;
   if (max_records <= 0) return GPJ_OK;
+  if (total_bytes >= (1ull << 35) ||
+      (uint64_t)levels * div_up(total_bytes, 64) >= 0xFFFFFFF0ull)
+    return fail(GPJ_ERR_QUERY_EXECUTION,
+                "batch above 32 GiB per launch; split it into several batches");
   const gpj::FilesMode fm{reinterpret_cast<const gpj::BatchFile *>(d_files), num_files,
                           results->capacity, d_line_prefix,
                           {results->d_spans[0], results->d_spans[1]},

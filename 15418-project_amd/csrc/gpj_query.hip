@@ -109,6 +109,12 @@ __device__ __forceinline__ u64x2 clear_past(u64x2 v, int64_t valid) {
   return v;
 }
 
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ Text make_text(const uint8_t *__restrict__ file, int64_t n) {
   Text t;
   const int64_t head = (int64_t)(reinterpret_cast<uintptr_t>(file) & 15);
@@ -129,8 +135,20 @@ __device__ __forceinline__ Text make_text(const uint8_t *__restrict__ file, int6
       else t.last_val.y |= (uint64_t)file[p1] << sh;
     }
   }
+  // the same for every lane: keep in scalar registers
+  t.first_val.x = uniform64(t.first_val.x);
+  t.first_val.y = uniform64(t.first_val.y);
+  t.last_val.x = uniform64(t.last_val.x);
+  t.last_val.y = uniform64(t.last_val.y);
   return t;
 }
+
+// Cache tags keep the low 32 bits of granule and word indices.  Word indices
+// are >= -1 and below 2^32 - 2 (the host keeps the leveled rows below that),
+// so the initial tag 2^32 - 2 never names a real word; granule indices of real
+// bytes stay below 2^32 - 2^27 (inputs below 2^35 bytes), and a negative
+// position that aliases a tag only ever meets zeros.
+constexpr uint32_t kNoTag = 0xFFFFFFFEu;
 
 // Per-lane view of the record's text and bitmaps, with one-entry caches for
 // the last text block, the last leveled-row word and the last string-mask
@@ -144,11 +162,11 @@ struct Ctx {
   int64_t level_size;
   int num_levels;
   int base;        // leveled row of interpreter level 0
-  int64_t text_at; // granule held in text_val (-1: none)
+  uint32_t text_at; // granule held in text_val (low 32 bits of its index)
   u64x2 text_val;
-  int64_t row_at;  // first word index of the 16-byte granule in row_val
+  uint32_t row_at; // first word index (low 32 bits) of the granule in row_val
   u64x2 row_val;
-  int64_t str_at;  // same for the string mask
+  uint32_t str_at; // same for the string mask
   u64x2 str_val;
 };
 
@@ -166,9 +184,9 @@ __device__ __forceinline__ u64x2 text_granule(const Ctx &c, int64_t g) {
 __device__ __forceinline__ uint8_t byte_at(Ctx &c, int64_t p) {
   const int64_t a = p + c.t.head;
   const int64_t g = a >> 4;
-  if (g != c.text_at) {
+  if ((uint32_t)g != c.text_at) {
     c.text_val = text_granule(c, g);
-    c.text_at = g;
+    c.text_at = (uint32_t)g;
   }
   const uint64_t w = (a & 8) ? c.text_val.y : c.text_val.x;
   return (uint8_t)(w >> (8 * (a & 7)));
@@ -180,12 +198,12 @@ __device__ __forceinline__ uint64_t text8(const Ctx &c, int64_t p) {
   const int64_t a = p + c.t.head;
   const int64_t g = a >> 4;
   const int64_t o = a & 15;
-  u64x2 v0 = (g == c.text_at) ? c.text_val : text_granule(c, g);
+  u64x2 v0 = ((uint32_t)g == c.text_at) ? c.text_val : text_granule(c, g);
   if (o == 0) return v0.x;
   if (o == 8) return v0.y;
   const uint32_t sh = 8 * (uint32_t)(o & 7);
   if (o < 8) return (v0.x >> sh) | (v0.y << (64 - sh));
-  const u64x2 v1 = (g + 1 == c.text_at) ? c.text_val : text_granule(c, g + 1);
+  const u64x2 v1 = ((uint32_t)(g + 1) == c.text_at) ? c.text_val : text_granule(c, g + 1);
   return (v0.y >> sh) | (v1.x << (64 - sh));
 }
 
@@ -194,12 +212,12 @@ __device__ __forceinline__ uint64_t text8(const Ctx &c, int64_t p) {
 // bitmap, inside the same aligned 16 bytes); the lane keeps one granule per
 // bitmap.
 __device__ __forceinline__ uint64_t granule_word(const uint64_t *__restrict__ base, int64_t wi,
-                                                 int64_t &at, u64x2 &val) {
+                                                 uint32_t &at, u64x2 &val) {
   const int64_t odd = (int64_t)((reinterpret_cast<uintptr_t>(base + wi) >> 3) & 1);
   const int64_t g = wi - odd;
-  if (g != at) {
+  if ((uint32_t)g != at) {
     val = *reinterpret_cast<const u64x2 *>(base + g);
-    at = g;
+    at = (uint32_t)g;
   }
   return odd ? val.y : val.x;
 }
@@ -604,7 +622,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
 // leveled row k + 1 (mode.level_base = 1, mode.drop_last = 1: the last bound
 // opens no record).
 template <int MAXLV>
-__global__ __launch_bounds__(64) void query_batch_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ? 5 : 1))) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
     const int64_t *__restrict__ newline, const int64_t *__restrict__ num_lines,
     int64_t max_lines, const uint64_t *__restrict__ string,
@@ -620,8 +638,8 @@ __global__ __launch_bounds__(64) void query_batch_kernel(
   for (int64_t base = wave_base; base < lines; base += stride) {
     const int64_t line = base + (threadIdx.x & 63);
     const bool valid = line < lines;
-    Ctx c{text, n, string, leveled, level_size, num_levels, mode.level_base, -1, u64x2{0, 0},
-          INT64_MIN, u64x2{0, 0}, INT64_MIN, u64x2{0, 0}};
+    Ctx c{text, n, string, leveled, level_size, num_levels, mode.level_base, kNoTag,
+          u64x2{0, 0}, kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}};
     int64_t line_start = 0, line_end = 0;
     bool record = false;
     if (valid) {
@@ -689,7 +707,7 @@ __global__ __launch_bounds__(64) void query_files_kernel(
     if (text.last == 0) text.first_val = text.last_val;
     const uint64_t word_off = bf.offset >> 6;
     Ctx c{text, n, string + word_off, leveled + word_off, level_stride, num_levels, 0,
-          -1, u64x2{0, 0}, INT64_MIN, u64x2{0, 0}, INT64_MIN, u64x2{0, 0}};
+          kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}};
     const int64_t *starts = newline + bf.nl_offset;
     int64_t line_start = 0, line_end = 0;
     bool record = false;
