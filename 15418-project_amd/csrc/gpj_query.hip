@@ -42,37 +42,38 @@ constexpr int kMaxFrames = 8; // element iterations per record (as the oracle)
 
 typedef const __attribute__((address_space(4))) uint32_t *ConstU32;
 
-// Wave-uniform reads of one query's bytecode through the scalar data cache.
-// Only aligned dwords that hold at least one byte of the buffer are loaded;
-// bytes at or past `len` read as 0.
+// Wave-uniform reads of one query's bytecode through the scalar data cache,
+// in 32-bit arithmetic (bytecode positions stay below 2^32).  Only aligned
+// dwords that hold at least one byte of the buffer are loaded; bytes at or
+// past `len` read as 0.
 struct IrReader {
-  uintptr_t base;
+  ConstU32 words; // aligned dwords covering the bytecode
+  uint32_t head;  // bytecode address modulo 4
   uint32_t len;
-  __device__ __forceinline__ uint32_t dword_at(uintptr_t a) const {
-    // a is 4-byte aligned; loaded only when it overlaps [base, base + len)
-    if (a + 4 <= base || a >= base + len) return 0;
-    return *reinterpret_cast<ConstU32>(a);
+  __device__ static __forceinline__ IrReader make(const uint8_t *ir, uint32_t len) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(ir);
+    return IrReader{reinterpret_cast<ConstU32>(a & ~(uintptr_t)3), (uint32_t)(a & 3), len};
   }
-  __device__ __forceinline__ uint32_t byte(int64_t p) const {
-    if (p < 0 || p >= (int64_t)len) return 0;
-    const uintptr_t a = base + (uintptr_t)p;
-    return (dword_at(a & ~(uintptr_t)3) >> (8 * (a & 3))) & 0xFFu;
+  __device__ __forceinline__ uint32_t dword(uint32_t w) const {
+    return w < ((head + len + 3) >> 2) ? words[w] : 0u;
   }
-  // Little-endian 8 bytes from p (bytes outside [0, len) read as 0).
-  __device__ __forceinline__ uint64_t u64(int64_t p) const {
-    const uintptr_t a = base + (uintptr_t)p;
-    const uintptr_t a0 = a & ~(uintptr_t)3;
-    const uint32_t sh = 8 * (uint32_t)(a & 3);
-    const uint64_t lo = (uint64_t)dword_at(a0) | ((uint64_t)dword_at(a0 + 4) << 32);
-    const uint64_t hi = dword_at(a0 + 8);
+  __device__ __forceinline__ uint32_t byte(uint32_t p) const {
+    if (p >= len) return 0;
+    const uint32_t o = p + head;
+    return (words[o >> 2] >> (8 * (o & 3))) & 0xFFu;
+  }
+  // Little-endian 8 bytes from p (bytes at or past len read as 0).
+  __device__ __forceinline__ uint64_t u64(uint32_t p) const {
+    if (p >= len) return 0;
+    const uint32_t o = p + head, w = o >> 2, sh = 8 * (o & 3);
+    const uint64_t lo = (uint64_t)dword(w) | ((uint64_t)dword(w + 1) << 32);
+    const uint64_t hi = dword(w + 2);
     uint64_t v = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
-    // clear bytes at or past len
-    const int64_t valid = (int64_t)len - p;
-    if (valid <= 0) return 0;
+    const uint32_t valid = len - p;
     if (valid < 8) v &= (1ull << (8 * valid)) - 1;
     return v;
   }
-  __device__ __forceinline__ int32_t varint(int64_t &qp) const {
+  __device__ __forceinline__ int32_t varint(uint32_t &qp) const {
     uint32_t value = 0, b = 0;
     int shift = 0;
     for (;;) {
@@ -429,15 +430,15 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       if (__ballot(live && s.qp == m)) qpu = m;
     }
     // decode the instruction at qpu (wave-uniform)
-    int64_t qn = qpu;
-    const uint32_t op = qpu >= (int32_t)ir.len ? (uint32_t)kOffEnd : ir.byte(qn++);
+    uint32_t qn = (uint32_t)qpu;
+    const uint32_t op = qn >= ir.len ? (uint32_t)kOffEnd : ir.byte(qn++);
     int32_t arg = 0;
-    int64_t key_pos = 0;
+    uint32_t key_pos = 0;
     if (op == kMoveToKey || op == kMoveToIndex || op == kMoveToIndexReverse ||
         op == kExpressionStringEquals || op == kJumpIfNotArray) {
       arg = ir.varint(qn);
       key_pos = qn;
-      if (op == kMoveToKey || op == kExpressionStringEquals) qn += arg;
+      if (op == kMoveToKey || op == kExpressionStringEquals) qn += (uint32_t)arg;
     }
     if (!(live && s.qp == qpu)) continue;
 
@@ -489,7 +490,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
           for (int32_t k = 8; match && k < arg; k += 8) {
             const int32_t rest = arg - k;
             const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
-            match = ((text8(c, ss + 1 + k) ^ ir.u64(key_pos + k)) & m) == 0;
+            match = ((text8(c, ss + 1 + k) ^ ir.u64(key_pos + (uint32_t)k)) & m) == 0;
           }
         }
         if (match) break;
@@ -576,7 +577,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       for (int32_t k = 0; k < arg; k += 8) {
         const int32_t rest = arg - k;
         const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
-        if (((text8(c, s.idx + k) ^ ir.u64(key_pos + k)) & m) != 0) { fail = true; break; }
+        if (((text8(c, s.idx + k) ^ ir.u64(key_pos + (uint32_t)k)) & m) != 0) { fail = true; break; }
       }
       break;
     }
@@ -658,7 +659,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
       int64_t *res = qs.spans + line * 2 * (int64_t)qs.cap;
       if (valid && !qs.variable)
         for (int i = 0; i < 2 * qs.cap; ++i) res[i] = -1;
-      const IrReader ir{reinterpret_cast<uintptr_t>(qs.ir), qs.ir_len};
+      const IrReader ir = IrReader::make(qs.ir, qs.ir_len);
       const int64_t found =
           run_query<MAXLV>(c, arrays, record, line_start, line_end, ir, qs.cap,
                            qs.variable != 0, res);
@@ -732,7 +733,7 @@ __global__ __launch_bounds__(64) void query_files_kernel(
       int64_t *res = (slot == 0 ? fm.spans[0] : fm.spans[1]) + g * 2 * (int64_t)fm.cap;
       if (mine && !qs.variable)
         for (int i = 0; i < 2 * fm.cap; ++i) res[i] = -1;
-      const IrReader ir{reinterpret_cast<uintptr_t>(qs.ir), qs.ir_len};
+      const IrReader ir = IrReader::make(qs.ir, qs.ir_len);
       const int64_t found = run_query<MAXLV>(c, arrays, record && mine, line_start, line_end,
                                              ir, qs.variable ? fm.cap : qs.cap,
                                              qs.variable != 0, res);
