@@ -48,7 +48,14 @@ __device__ __forceinline__ uint64_t prefix_xor64(uint64_t x) {
 // ---- byte classification ---------------------------------------------------
 // For a 7-bit byte b and ASCII c: ((b ^ (0x7F ^ c)) + 1) has bit 7 set iff
 // b == c, with no carry out of the byte.  `nh` keeps only bit 7 of bytes
-// whose original value is < 0x80.
+// whose original value is < 0x80.  v_xad_u32 computes (a ^ b) + c in one
+// instruction (the compiler splits it into xor + add when an operand is a
+// literal); the class constant sits in a scalar register.
+__device__ __forceinline__ uint32_t xad(uint32_t a, uint32_t k, uint32_t c) {
+  uint32_t r;
+  asm("v_xad_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(k), "v"(c));
+  return r;
+}
 __device__ __forceinline__ uint32_t eq_bytes(uint32_t lo7, uint32_t c) {
   return (lo7 ^ (0x7F7F7F7Fu ^ (c * 0x01010101u))) + 0x01010101u;
 }
@@ -101,27 +108,48 @@ __device__ __forceinline__ void load_word(const uint8_t *in, uint64_t n,
 }
 
 __device__ __forceinline__ WordClasses classify(const uint32_t (&x)[16]) {
-  uint32_t fq[16], fb[16], fn[16], fo[16], fc[16], fs[16];
+  // class constants 0x7F ^ c in every byte ('[' and ']' fold onto '{' and '}'
+  // once bit 5 is set)
+  const uint32_t kq = 0x7F7F7F7Fu ^ ('"' * 0x01010101u);
+  const uint32_t kb = 0x7F7F7F7Fu ^ ('\\' * 0x01010101u);
+  const uint32_t kn = 0x7F7F7F7Fu ^ ('\n' * 0x01010101u);
+  const uint32_t ko = 0x7F7F7F7Fu ^ ('{' * 0x01010101u);
+  const uint32_t kc = 0x7F7F7F7Fu ^ ('}' * 0x01010101u);
+  const uint32_t kl = 0x7F7F7F7Fu ^ (':' * 0x01010101u);
+  const uint32_t km = 0x7F7F7F7Fu ^ (',' * 0x01010101u);
+  const uint32_t ones = 0x01010101u;
+  // per class: bits of the low and high 32 bytes, packed two dwords at a time
+  uint32_t acc[6][2] = {};
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const uint32_t v = x[i];
-    const uint32_t nh = ~v & 0x80808080u;
-    const uint32_t lo = v & 0x7F7F7F7Fu;
-    const uint32_t lo20 = lo | 0x20202020u; // folds '[' onto '{', ']' onto '}'
-    fq[i] = eq_bytes(lo, '"') & nh;
-    fb[i] = eq_bytes(lo, '\\') & nh;
-    fn[i] = eq_bytes(lo, '\n') & nh;
-    fo[i] = eq_bytes(lo20, '{') & nh;
-    fc[i] = eq_bytes(lo20, '}') & nh;
-    fs[i] = (eq_bytes(lo, ':') | eq_bytes(lo, ',')) & nh;
+  for (int j = 0; j < 8; ++j) {
+    uint32_t f[6][2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const uint32_t v = x[2 * j + e];
+      const uint32_t nh = ~v & 0x80808080u;
+      const uint32_t lo = v & 0x7F7F7F7Fu;
+      const uint32_t lo20 = lo | 0x20202020u;
+      f[0][e] = xad(lo, kq, ones) & nh;
+      f[1][e] = xad(lo, kb, ones) & nh;
+      f[2][e] = xad(lo, kn, ones) & nh;
+      f[3][e] = xad(lo20, ko, ones) & nh;
+      f[4][e] = xad(lo20, kc, ones) & nh;
+      f[5][e] = (xad(lo, kl, ones) | xad(lo, km, ones)) & nh;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const uint32_t p = pack2(f[c][0], f[c][1]); // mask byte << 7
+      const int sh = 8 * (j & 3) - 7;
+      acc[c][j >> 2] |= sh < 0 ? (p >> 7) : (p << sh);
+    }
   }
   WordClasses w;
-  w.quote = pack16(fq);
-  w.backslash = pack16(fb);
-  w.newline = pack16(fn);
-  w.open = pack16(fo);
-  w.close = pack16(fc);
-  w.sep = pack16(fs);
+  w.quote = (uint64_t)acc[0][0] | ((uint64_t)acc[0][1] << 32);
+  w.backslash = (uint64_t)acc[1][0] | ((uint64_t)acc[1][1] << 32);
+  w.newline = (uint64_t)acc[2][0] | ((uint64_t)acc[2][1] << 32);
+  w.open = (uint64_t)acc[3][0] | ((uint64_t)acc[3][1] << 32);
+  w.close = (uint64_t)acc[4][0] | ((uint64_t)acc[4][1] << 32);
+  w.sep = (uint64_t)acc[5][0] | ((uint64_t)acc[5][1] << 32);
   return w;
 }
 
@@ -137,27 +165,30 @@ __device__ __forceinline__ uint64_t escaped_from_zero(uint64_t b) {
 }
 
 // ---- wave / block primitives ----------------------------------------------
-__device__ __forceinline__ uint32_t lane_id() {
-  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-}
 __device__ __forceinline__ uint32_t count_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Inclusive add-scan over the wave with DPP row shifts and row broadcasts
+// (six VALU instructions; rows of 16 lanes, then row 0 -> 1, row 2 -> 3,
+// rows 0..1 -> 2..3).  Lanes outside the source range add 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_src(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWMASK, 0xF, false);
+}
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t o = __shfl_up(v, d, 64);
-    if (lane >= (uint32_t)d) v += o;
-  }
+  v += dpp_src<0x111, 0xF>(v); // row_shr:1
+  v += dpp_src<0x112, 0xF>(v); // row_shr:2
+  v += dpp_src<0x114, 0xF>(v); // row_shr:4
+  v += dpp_src<0x118, 0xF>(v); // row_shr:8
+  v += dpp_src<0x142, 0xA>(v); // row_bcast:15
+  v += dpp_src<0x143, 0xC>(v); // row_bcast:31
   return v;
 }
-__device__ __forceinline__ uint64_t wave_reduce_add64(uint64_t v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
+// Sum over the wave (every lane gets it).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), 63);
 }
 
 // ---- tile descriptors -------------------------------------------------------
@@ -268,7 +299,7 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   __shared__ uint64_t s_wsum[kTileWords / 64];
   __shared__ uint32_t s_wscan[kTileWords / 64];
   __shared__ uint32_t s_first;              // k | lead_k << 16 | g << 24
-  __shared__ uint64_t s_rows[LMAX][kTileWords]; // per-lane leveled words
+  __shared__ uint32_t s_rows[LMAX + 1][2 * kTileWords]; // per-lane leveled words (+ sink)
   __shared__ uint32_t s_tail_esc;           // tile escape out (X=0) | all-bs << 1
   __shared__ Carry s_carry;
 
@@ -362,10 +393,10 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   {
     const int d0 = __popcll(m.open & ~s0) - __popcll(m.close & ~s0);
     const int tot = __popcll(m.open) - __popcll(m.close);
-    const uint64_t packed = (uint64_t)(d0 + 64) | ((uint64_t)(tot + 64) << 16) |
-                            ((uint64_t)__popcll(m.newline) << 32);
-    const uint64_t r = wave_reduce_add64(packed);
-    if (lane == 0) s_wsum[wv] = r;
+    // per-wave sums stay below 2^16 in each 16-bit field
+    const uint32_t dsum = wave_sum((uint32_t)(d0 + 64) | ((uint32_t)(tot + 64) << 16));
+    const uint32_t nsum = wave_sum((uint32_t)__popcll(m.newline));
+    if (lane == 0) s_wsum[wv] = (uint64_t)dsum | ((uint64_t)nsum << 32);
   }
   __syncthreads();
 
@@ -489,34 +520,48 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
       (int)(int8_t)(uint8_t)((uint32_t)c.depth + (excl_v & 0xFFFFu) - 64u * t);
   const uint64_t nl_base = 1 + c.nl + (excl_v >> 16);
 
-  // 8. leveled rows: walk this word's depth changes in order; every
-  //    structural bit between two changes belongs to the current depth
-  //    (an opener's own bit to the depth it opens, a closer's to the depth
-  //    it closes).  Rows accumulate in this lane's LDS slots.
+  // 8. leveled rows: walk this word's depth changes in order, one 32-bit half
+  //    at a time; every structural bit between two changes belongs to the
+  //    current depth (an opener's own bit to the depth it opens, a closer's
+  //    to the depth it closes).  Each segment is ORed into this lane's LDS
+  //    slot of its row; row LMAX is a sink for depths outside [0, LMAX).
+  //    Depth needs no 8-bit wrap inside the word: from an entry depth in
+  //    [-128, 127], 64 changes cannot wrap into [0, LMAX).
   const bool live = w < words;
   if (live) {
 #pragma unroll
-    for (int k = 0; k < LMAX; ++k) s_rows[k][t] = 0;
-    const uint64_t E = O | C | P;
-    uint64_t chg = O | C;
-    uint64_t done = 0;
-    int depth = depth_in;
-    while (chg) {
-      const uint64_t low = chg & (0ull - chg);
-      chg ^= low;
-      const bool open = (O & low) != 0;
-      const uint64_t upto = open ? (low - 1) : (low | (low - 1));
-      const uint64_t seg = upto & ~done & E;
-      done = upto;
-      if ((unsigned)depth < (unsigned)LMAX && seg) s_rows[depth][t] |= seg;
-      depth = (int)(int8_t)(uint8_t)(depth + (open ? 1 : -1));
+    for (int k = 0; k < LMAX; ++k) {
+      s_rows[k][2 * t] = 0;
+      s_rows[k][2 * t + 1] = 0;
     }
-    const uint64_t seg = E & ~done;
-    if ((unsigned)depth < (unsigned)LMAX && seg) s_rows[depth][t] |= seg;
+    const uint64_t E = O | C | P;
+    int depth = depth_in;
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+      const uint32_t Eh = (uint32_t)(E >> (32 * h));
+      const uint32_t Oh = (uint32_t)(O >> (32 * h));
+      uint32_t x = Oh | (uint32_t)(C >> (32 * h));
+      uint32_t done = 0;
+      uint32_t *const slot = &s_rows[0][2 * t + h];
+      while (x) {
+        const uint32_t low = x & (0u - x);
+        x ^= low;
+        const uint32_t upto = (low - 1u) | (low & ~Oh);
+        const uint32_t seg = upto & ~done & Eh;
+        done = upto;
+        const uint32_t row = min((uint32_t)depth, (uint32_t)LMAX);
+        atomicOr(slot + row * (2 * kTileWords), seg);
+        depth += (low & Oh) ? 1 : -1;
+      }
+      const uint32_t row = min((uint32_t)depth, (uint32_t)LMAX);
+      atomicOr(slot + row * (2 * kTileWords), Eh & ~done);
+    }
     string_out[w] = S;
 #pragma unroll
     for (int k = 0; k < LMAX; ++k)
-      if (k < levels) leveled_out[(uint64_t)k * words + w] = s_rows[k][t];
+      if (k < levels)
+        leveled_out[(uint64_t)k * words + w] =
+            (uint64_t)s_rows[k][2 * t] | ((uint64_t)s_rows[k][2 * t + 1] << 32);
 
     // 9. newline positions
     if (tile == first_tile && t == 0 && nl_cap > 0) nl_out[0] = 0;
@@ -558,7 +603,7 @@ __global__ __launch_bounds__(kTileWords) void newline_count_kernel(
     for (int i = 0; i < 16; ++i)
       c += __popc(eq_bytes(x[i] & 0x7F7F7F7Fu, '\n') & ~x[i] & 0x80808080u);
   }
-  c = wave_reduce_add64(c);
+  c = wave_sum((uint32_t)c); // at most 64 * 64 per wave
   if ((threadIdx.x & 63u) == 0) s_sum[threadIdx.x >> 6] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -638,7 +683,7 @@ __global__ __launch_bounds__(kTileWords) void row_count_kernel(
   __shared__ uint64_t s_sum[kTileWords / 64];
   const uint64_t w = (uint64_t)blockIdx.x * kTileWords + threadIdx.x;
   uint64_t c = w < words ? (uint64_t)__popcll(row[w]) : 0;
-  c = wave_reduce_add64(c);
+  c = wave_sum((uint32_t)c); // at most 64 * 64 per wave
   if ((threadIdx.x & 63u) == 0) s_sum[threadIdx.x >> 6] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
