@@ -264,6 +264,14 @@ __device__ __forceinline__ void desc_store(uint64_t *p, uint64_t v) {
 
 } // namespace
 
+// Block barrier for LDS traffic only: global loads and stores in flight stay
+// in flight across it (__syncthreads() waits for them as well).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Clears the tile descriptors and the tile counter (kept out of
 // hipMemsetAsync so a captured graph replays it correctly).
 __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
@@ -295,11 +303,13 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   __shared__ uint8_t s_bsinfo[kTileWords];  // bit0 trailing parity, bit1 all-backslash
   __shared__ uint8_t s_lead[kTileWords];    // leading backslash run, bit7 = quote at its end
   __shared__ uint8_t s_cin[kTileWords];     // escape carry-in per word (tile carry-in 0)
-  __shared__ uint32_t s_wpar[kTileWords / 64];
-  __shared__ uint64_t s_wsum[kTileWords / 64];
+  __shared__ uint32_t s_wany[kTileWords / 64]; // per wave: a word of only backslashes
+  __shared__ uint32_t s_wpar[kTileWords / 64]; // per wave: quote parity
+  __shared__ uint32_t s_wsum[kTileWords / 64]; // per wave: sum of (d0a+64) | (tot+64) << 16
+  __shared__ uint32_t s_wnl[kTileWords / 64];  // per wave: newlines
   __shared__ uint32_t s_wscan[kTileWords / 64];
   __shared__ uint32_t s_first;              // k | lead_k << 16 | g << 24
-  __shared__ uint32_t s_rows[LMAX + 1][2 * kTileWords]; // per-lane leveled words (+ sink)
+  __shared__ uint32_t s_rows[LMAX + 1][2][kTileWords]; // per-lane leveled half-words (+ sink)
   __shared__ uint32_t s_tail_esc;           // tile escape out (X=0) | all-bs << 1
   __shared__ Carry s_carry;
 
@@ -307,8 +317,10 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   const uint32_t lane = t & 63u;
   const uint32_t wv = t >> 6;
 
+  // 0. tile id from the counter: ids follow dispatch order, so every tile a
+  //    look-back waits on has started
   if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
-  __syncthreads();
+  lds_barrier();
   const uint64_t tile = s_tile;
   const uint64_t w = tile * kTileWords + t;
   const uint64_t pos = w * 64;
@@ -344,8 +356,16 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   const uint32_t trail = (uint32_t)((~B == 0ull) ? 0 : (__builtin_clzll(~B) & 1));
   s_bsinfo[t] = (uint8_t)(trail | (all_bs ? 2u : 0u));
   s_lead[t] = (uint8_t)(lead | ((lead < 64 && ((m.quote >> lead) & 1ull)) ? 0x80 : 0));
-  const int any_all_bs = __syncthreads_or(all_bs ? 1 : 0);
+  const uint64_t any_ballot = __ballot(all_bs);
+  if (lane == 0) s_wany[wv] = any_ballot != 0;
+  lds_barrier();
+  uint32_t any_all_bs = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kTileWords / 64; ++k) any_all_bs |= s_wany[k];
+  uint32_t cin, tail_esc, first;
   if (any_all_bs) {
+    // rare: a word of only backslashes passes its carry through; one serial
+    // pass over the tile's words
     if (t == 0) {
       uint32_t c = 0;
       for (uint32_t k = 0; k < kTileWords; ++k) {
@@ -364,52 +384,70 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
                   ((uint32_t)(s_lead[k] >> 7) << 24);
       }
     }
-  } else if (t == 0) {
-    s_tail_esc = s_bsinfo[kTileWords - 1] & 1u;
-    s_first = 0u | ((uint32_t)(s_lead[0] & 0x7F) << 16) |
-              ((uint32_t)(s_lead[0] >> 7) << 24);
+    lds_barrier();
+    cin = s_cin[t];
+    tail_esc = s_tail_esc;
+    first = s_first;
+  } else {
+    cin = t ? (s_bsinfo[t - 1] & 1u) : 0u;
+    tail_esc = s_bsinfo[kTileWords - 1] & 1u;
+    first = ((uint32_t)(s_lead[0] & 0x7F) << 16) | ((uint32_t)(s_lead[0] >> 7) << 24);
   }
-  __syncthreads();
-  const uint32_t cin = any_all_bs ? s_cin[t] : (t ? (s_bsinfo[t - 1] & 1u) : 0u);
   const uint64_t escaped = cin ? (e0 ^ mask_upto(lead)) : e0;
 
-  // 3. string mask for string carry-in 0
+  // 3. string mask for string carry-in 0 at this wave's first word (the
+  //    parity the tile's earlier waves leave is applied below)
   const uint64_t q = m.quote & ~escaped;
   const uint64_t px = prefix_xor64(q);
   const uint64_t par_ballot = __ballot((uint32_t)(px >> 63));
-  const uint32_t wave_par_before = count_below(par_ballot) & 1u;
-  if (lane == 0) s_wpar[wv] = (uint32_t)__popcll(par_ballot) & 1u;
+  const uint64_t s0a = px ^ ((count_below(par_ballot) & 1u) ? ~0ull : 0ull);
 
-  // 4. per-word depth deltas for both string polarities, newline counts
-  __syncthreads();
-  uint32_t sin = wave_par_before;
-  uint32_t tile_par = 0;
+  // 4. depth delta of this word under s0a and its total, newline count; the
+  //    in-wave inclusive scans serve both the tile aggregate and, once the
+  //    carry-in is known, this word's depth and newline offsets
+  const int d0a = __popcll(m.open & ~s0a) - __popcll(m.close & ~s0a);
+  const int tot = __popcll(m.open) - __popcll(m.close);
+  const uint32_t nlc = (uint32_t)__popcll(m.newline);
+  // 16-bit fields: in-wave sums stay below 64 * 129
+  const uint32_t dv = (uint32_t)(d0a + 64) | ((uint32_t)(tot + 64) << 16);
+  const uint32_t dincl = wave_incl_scan(dv);
+  const uint32_t nincl = wave_incl_scan(nlc);
+  if (lane == 63) {
+    s_wsum[wv] = dincl;
+    s_wnl[wv] = nincl;
+  }
+  if (lane == 0) s_wpar[wv] = (uint32_t)__popcll(par_ballot) & 1u;
+  lds_barrier();
+
+  // tile sums for string carry-in 0 (wave k's words see the parity of the
+  // waves before it), and the same sums over the waves before this one
+  uint32_t run_par = 0, my_par = 0, NL = 0, nl_before = 0;
+  int D0 = 0, TOT = 0, d_before = 0, tot_before = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kTileWords / 64; ++k) {
-    if (k < wv) sin ^= s_wpar[k];
-    tile_par ^= s_wpar[k];
+    const uint32_t ws = s_wsum[k];
+    const int d0k = (int)(ws & 0xFFFFu) - 64 * 64;
+    const int totk = (int)(ws >> 16) - 64 * 64;
+    const int dk = run_par ? totk - d0k : d0k;
+    if (k < wv) {
+      d_before += dk;
+      tot_before += totk;
+      nl_before += s_wnl[k];
+    }
+    if (k == wv) my_par = run_par;
+    D0 += dk;
+    TOT += totk;
+    NL += s_wnl[k];
+    run_par ^= s_wpar[k];
   }
-  const uint64_t s0 = px ^ (sin ? ~0ull : 0ull);
-  {
-    const int d0 = __popcll(m.open & ~s0) - __popcll(m.close & ~s0);
-    const int tot = __popcll(m.open) - __popcll(m.close);
-    // per-wave sums stay below 2^16 in each 16-bit field
-    const uint32_t dsum = wave_sum((uint32_t)(d0 + 64) | ((uint32_t)(tot + 64) << 16));
-    const uint32_t nsum = wave_sum((uint32_t)__popcll(m.newline));
-    if (lane == 0) s_wsum[wv] = (uint64_t)dsum | ((uint64_t)nsum << 32);
-  }
-  __syncthreads();
+  const uint32_t tile_par = run_par;
+  const uint64_t s0 = s0a ^ (my_par ? ~0ull : 0ull);
 
   // 5. publish the tile aggregate and resolve the exclusive prefix (wave 0)
   if (wv == 0) {
-    uint64_t sum = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kTileWords / 64; ++k) sum += s_wsum[k];
-    const uint32_t D0 = (uint32_t)((sum & 0xFFFFu) - 64u * kTileWords) & 0xFFu;
-    const uint32_t TOT = (uint32_t)(((sum >> 16) & 0xFFFFu) - 64u * kTileWords) & 0xFFu;
-    const uint64_t NL = sum >> 32;
-    const uint32_t g = (s_first >> 24) & 1u;
-    const uint32_t tail = s_tail_esc;
+    const uint32_t g = (first >> 24) & 1u;
+    const uint32_t tail = tail_esc;
+    const uint32_t D0b = (uint32_t)D0 & 0xFFu, TOTb = (uint32_t)TOT & 0xFFu;
     uint64_t tbl = 0;
 #pragma unroll
     for (uint32_t X = 0; X < 2; ++X) {
@@ -418,7 +456,7 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
         const uint32_t eff = Y ^ (X & g);
         const uint32_t esc_out = (tail & 2u) ? X : (tail & 1u);
         const uint32_t str_out = Y ^ tile_par ^ (X & g);
-        const uint32_t dd = eff ? ((TOT - D0) & 0xFFu) : D0;
+        const uint32_t dd = eff ? ((TOTb - D0b) & 0xFFu) : D0b;
         const uint64_t e = esc_out | (str_out << 1) | (dd << 2);
         tbl |= e << ((X * 2 + Y) * 10);
       }
@@ -429,7 +467,8 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
     excl.depth = -1;
     excl.nl = 0;
     if (tile != first_tile) {
-      if (lane == 0) desc_store(&desc[tile], tbl | (NL << 40) | (kFlagAggregate << kFlagShift));
+      if (lane == 0)
+        desc_store(&desc[tile], tbl | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
       // Look back 64 predecessors at a time: lane i holds tile-1-i.  Compose
       // the transfer tables newer-first up to the nearest inclusive prefix.
       uint64_t acc = kIdentityTable;
@@ -488,17 +527,18 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
       s_carry = excl;
     }
   }
-  __syncthreads();
+  lds_barrier();
   const Carry c = s_carry;
 
   // 6. final string mask: string carry-in, plus the flip of everything from
   //    the tile's first non-backslash byte on when an escape carry-in of 1
   //    un-counts a quote there
-  const uint32_t k_first = s_first & 0xFFFFu;
-  const uint32_t lead_k = (s_first >> 16) & 0xFFu;
-  const uint32_t g = (s_first >> 24) & 1u;
+  const uint32_t k_first = first & 0xFFFFu;
+  const uint32_t lead_k = (first >> 16) & 0xFFu;
+  const uint32_t g = (first >> 24) & 1u;
+  const bool esc_flip = (c.esc & g) != 0; // uniform
   uint64_t flip = c.str ? ~0ull : 0ull;
-  if (c.esc & g) {
+  if (esc_flip) {
     if (t > k_first) flip ^= ~0ull;
     else if (t == k_first) flip ^= ~0ull << lead_k;
   }
@@ -507,18 +547,29 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   const uint64_t C = m.close & ~S;
   const uint64_t P = m.sep & ~S;
 
-  // 7. block exclusive scan of (depth delta, newline count)
-  const int d = __popcll(O) - __popcll(C);
-  const uint32_t nlc = (uint32_t)__popcll(m.newline);
-  const uint32_t v = ((uint32_t)(d + 64)) | (nlc << 16);
-  const uint32_t incl_v = wave_incl_scan(v);
-  if (lane == 63) s_wscan[wv] = incl_v;
-  __syncthreads();
-  uint32_t excl_v = incl_v - v;
-  for (uint32_t k = 0; k < wv; ++k) excl_v += s_wscan[k];
-  const int depth_in =
-      (int)(int8_t)(uint8_t)((uint32_t)c.depth + (excl_v & 0xFFFFu) - 64u * t);
-  const uint64_t nl_base = 1 + c.nl + (excl_v >> 16);
+  // 7. depth before this word and its newline offset
+  const uint64_t nl_base = 1 + c.nl + nl_before + (nincl - nlc);
+  int depth_sum;
+  if (!esc_flip) {
+    // every word keeps its polarity relative to s0a: the words of this wave
+    // flip iff my_par ^ c.str, the earlier waves' sums flip iff c.str
+    const uint32_t dex = dincl - dv;
+    const int d0_ex = (int)(dex & 0xFFFFu) - 64 * (int)lane;
+    const int tot_ex = (int)(dex >> 16) - 64 * (int)lane;
+    depth_sum = (c.str ? tot_before - d_before : d_before) +
+                ((my_par ^ c.str) ? tot_ex - d0_ex : d0_ex);
+  } else {
+    // rare: scan the final per-word deltas
+    const int d = __popcll(O) - __popcll(C);
+    const uint32_t v = (uint32_t)(d + 64);
+    const uint32_t incl_v = wave_incl_scan(v);
+    if (lane == 63) s_wscan[wv] = incl_v;
+    lds_barrier();
+    uint32_t excl_v = incl_v - v;
+    for (uint32_t k = 0; k < wv; ++k) excl_v += s_wscan[k];
+    depth_sum = (int)excl_v - 64 * (int)t;
+  }
+  const int depth_in = (int)(int8_t)(uint8_t)((uint32_t)c.depth + (uint32_t)depth_sum);
 
   // 8. leveled rows: walk this word's depth changes in order, one 32-bit half
   //    at a time; every structural bit between two changes belongs to the
@@ -531,8 +582,8 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   if (live) {
 #pragma unroll
     for (int k = 0; k < LMAX; ++k) {
-      s_rows[k][2 * t] = 0;
-      s_rows[k][2 * t + 1] = 0;
+      s_rows[k][0][t] = 0;
+      s_rows[k][1][t] = 0;
     }
     const uint64_t E = O | C | P;
     int depth = depth_in;
@@ -542,7 +593,7 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
       const uint32_t Oh = (uint32_t)(O >> (32 * h));
       uint32_t x = Oh | (uint32_t)(C >> (32 * h));
       uint32_t done = 0;
-      uint32_t *const slot = &s_rows[0][2 * t + h];
+      uint32_t *const slot = &s_rows[0][h][t];
       while (x) {
         const uint32_t low = x & (0u - x);
         x ^= low;
@@ -561,7 +612,7 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
     for (int k = 0; k < LMAX; ++k)
       if (k < levels)
         leveled_out[(uint64_t)k * words + w] =
-            (uint64_t)s_rows[k][2 * t] | ((uint64_t)s_rows[k][2 * t + 1] << 32);
+            (uint64_t)s_rows[k][0][t] | ((uint64_t)s_rows[k][1][t] << 32);
 
     // 9. newline positions
     if (tile == first_tile && t == 0 && nl_cap > 0) nl_out[0] = 0;
