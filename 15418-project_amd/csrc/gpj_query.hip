@@ -152,9 +152,12 @@ __device__ __forceinline__ Text make_text(const uint8_t *__restrict__ file, int6
 constexpr uint32_t kNoTag = 0xFFFFFFFEu;
 
 // Per-lane view of the record's text and bitmaps, with one-entry caches for
-// the last text block, the last leveled-row word and the last string-mask
-// word read.  The caches stay valid across the queries of a batch, which run
-// over the same record.
+// the last text granule and for the last granule read from even and from odd
+// leveled rows (a walk alternates between a level and the one below it, so
+// the two never evict each other).  The caches stay valid across the queries
+// of a batch, which run over the same record.  String-mask words are read
+// directly: a key lookup needs them only when the byte before its ':' is not
+// an unescaped quote.
 struct Ctx {
   Text t;
   int64_t n;
@@ -165,10 +168,10 @@ struct Ctx {
   int base;        // leveled row of interpreter level 0
   uint32_t text_at; // granule held in text_val (low 32 bits of its index)
   u64x2 text_val;
-  uint32_t row_at; // first word index (low 32 bits) of the granule in row_val
-  u64x2 row_val;
-  uint32_t str_at; // same for the string mask
-  u64x2 str_val;
+  uint32_t row_at0; // first word index (low 32 bits) of the granule in row_val0
+  u64x2 row_val0;   // (a granule of an even row)
+  uint32_t row_at1; // same for odd rows
+  u64x2 row_val1;
 };
 
 // Granule g of the input (granules outside the input read as 0).
@@ -222,19 +225,20 @@ __device__ __forceinline__ uint64_t granule_word(const uint64_t *__restrict__ ba
   }
   return odd ? val.y : val.x;
 }
-__device__ __forceinline__ uint64_t row_word(Ctx &c, int64_t wi) {
-  return granule_word(c.leveled, wi, c.row_at, c.row_val);
+// Word wi of the leveled rows (row-major, so wi includes the row offset);
+// `odd` is the parity of the row.
+__device__ __forceinline__ uint64_t row_word(Ctx &c, int64_t wi, bool odd) {
+  return odd ? granule_word(c.leveled, wi, c.row_at1, c.row_val1)
+             : granule_word(c.leveled, wi, c.row_at0, c.row_val0);
 }
-__device__ __forceinline__ uint64_t str_word(Ctx &c, int64_t wi) {
-  return granule_word(c.string, wi, c.str_at, c.str_val);
-}
+__device__ __forceinline__ uint64_t str_word(const Ctx &c, int64_t wi) { return c.string[wi]; }
 
 __device__ __forceinline__ uint64_t through(int b) {
   return b >= 63 ? ~0ull : ((1ull << (b + 1)) - 1);
 }
 
 // Last set bit of the string mask in [lo, idx], or -1.
-__device__ __forceinline__ int64_t last_string_bit(Ctx &c, int64_t lo, int64_t idx) {
+__device__ __forceinline__ int64_t last_string_bit(const Ctx &c, int64_t lo, int64_t idx) {
   if (idx < lo || idx < 0) return -1;
   int64_t wi = idx >> 6, wb = wi << 6;
   uint64_t word = str_word(c, wi) & through((int)(idx - wb));
@@ -251,10 +255,11 @@ __device__ __forceinline__ int64_t last_string_bit(Ctx &c, int64_t lo, int64_t i
 }
 
 // Last set bit of the leveled row starting at word `row` in [lo, idx], or -1.
-__device__ __forceinline__ int64_t last_row_bit(Ctx &c, int64_t row, int64_t lo, int64_t idx) {
+__device__ __forceinline__ int64_t last_row_bit(Ctx &c, int64_t row, bool odd, int64_t lo,
+                                                int64_t idx) {
   if (idx < lo || idx < 0) return -1;
   int64_t wi = idx >> 6, wb = wi << 6;
-  uint64_t word = row_word(c, row + wi) & through((int)(idx - wb));
+  uint64_t word = row_word(c, row + wi, odd) & through((int)(idx - wb));
   for (;;) {
     uint64_t cand = word;
     const int64_t first = lo - wb;
@@ -263,7 +268,7 @@ __device__ __forceinline__ int64_t last_row_bit(Ctx &c, int64_t row, int64_t lo,
     if (wi == 0 || wb - 1 < lo) return -1;
     --wi;
     wb -= 64;
-    word = row_word(c, row + wi);
+    word = row_word(c, row + wi, odd);
   }
 }
 
@@ -274,8 +279,9 @@ __device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, in
   if (level < 0 || level + c.base >= c.num_levels) return level_end;
   if (idx < 0) idx = 0;
   const int64_t row = c.level_size * (int64_t)(level + c.base);
+  const bool odd = ((level + c.base) & 1) != 0;
   int64_t wi = idx >> 6, wb = wi << 6;
-  uint64_t word = row_word(c, row + wi) & (~0ull << (idx - wb));
+  uint64_t word = row_word(c, row + wi, odd) & (~0ull << (idx - wb));
   for (;;) {
     uint64_t cand = word;
     const int64_t last = level_end - wb;
@@ -284,7 +290,7 @@ __device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, in
     ++wi;
     wb += 64;
     if (wb > level_end || wi >= c.level_size) return level_end;
-    word = row_word(c, row + wi);
+    word = row_word(c, row + wi, odd);
   }
 }
 
@@ -293,7 +299,7 @@ __device__ __forceinline__ int64_t prev_structural(Ctx &c, int64_t level_start, 
   if (idx <= level_start) return level_start;
   if (level < 0 || level + c.base >= c.num_levels) return level_start;
   const int64_t row = c.level_size * (int64_t)(level + c.base);
-  const int64_t s = last_row_bit(c, row, level_start, idx);
+  const int64_t s = last_row_bit(c, row, ((level + c.base) & 1) != 0, level_start, idx);
   return s == -1 ? level_start : s;
 }
 
@@ -482,7 +488,14 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       for (;;) {
         idx = next_structural(c, lend, idx + 1, s.lvl);
         if (byte_at(c, idx) != ':') break;
-        const int64_t string_end = last_string_bit(c, line_start + 1, idx - 1);
+        // idx is outside strings, so an unescaped quote right before it closes
+        // a string whose last in-string byte (body or opening quote) is
+        // idx - 2: the string mask's last set bit in [line_start + 1, idx - 1]
+        int64_t string_end;
+        if (idx - 2 >= line_start + 1 && byte_at(c, idx - 1) == '"' && byte_at(c, idx - 2) != '\\')
+          string_end = idx - 2;
+        else
+          string_end = last_string_bit(c, line_start + 1, idx - 1);
         const int64_t ss = string_end - arg;
         bool match = ss >= line_start && byte_at(c, ss) == '"';
         if (match && arg > 0) {
