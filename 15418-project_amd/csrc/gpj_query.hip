@@ -148,11 +148,14 @@ __device__ __forceinline__ Text make_text(const uint8_t *__restrict__ file, int6
 // are >= -1 and below 2^32 - 2 (the host keeps the leveled rows below that),
 // so the initial tag 2^32 - 2 never names a real word; granule indices of real
 // bytes stay below 2^32 - 2^27 (inputs below 2^35 bytes), and a negative
-// position that aliases a tag only ever meets zeros.
+// position that aliases a tag (the text pair's initial tags name granules -2
+// and -1) only ever meets zeros.
 constexpr uint32_t kNoTag = 0xFFFFFFFEu;
 
-// Per-lane view of the record's text and bitmaps, with one-entry caches for
-// the last text granule and for the last granule read from even and from odd
+// Per-lane view of the record's text and bitmaps, with caches for the last
+// two consecutive text granules (filled together: a forward walk usually
+// moves on into the next granule) and for the last granule read from even and
+// from odd
 // leveled rows (a walk alternates between a level and the one below it, so
 // the two never evict each other).  The caches stay valid across the queries
 // of a batch, which run over the same record.  String-mask words are read
@@ -166,8 +169,9 @@ struct Ctx {
   int64_t level_size;
   int num_levels;
   int base;        // leveled row of interpreter level 0
-  uint32_t text_at; // granule held in text_val (low 32 bits of its index)
+  uint32_t text_at; // granules text_at, text_at + 1 held in text_val, text_nxt
   u64x2 text_val;
+  u64x2 text_nxt;
   uint32_t row_at0; // first word index (low 32 bits) of the granule in row_val0
   u64x2 row_val0;   // (a granule of an even row)
   uint32_t row_at1; // same for odd rows
@@ -182,32 +186,47 @@ __device__ __forceinline__ u64x2 text_granule(const Ctx &c, int64_t g) {
   return u64x2{0, 0};
 }
 
+// Fills the lane's granule pair with granules g and g + 1 (one round trip).
+__device__ __forceinline__ void text_fill(Ctx &c, int64_t g) {
+  if (g > 0 && g + 1 < c.t.last) {
+    c.text_val = c.t.blocks[g];
+    c.text_nxt = c.t.blocks[g + 1];
+  } else {
+    c.text_val = text_granule(c, g);
+    c.text_nxt = text_granule(c, g + 1);
+  }
+  c.text_at = (uint32_t)g;
+}
+// Granule g from the lane's pair, filling the pair from g on a miss.
+__device__ __forceinline__ u64x2 text_get(Ctx &c, int64_t g) {
+  const uint32_t d = (uint32_t)g - c.text_at;
+  if (d > 1u) {
+    text_fill(c, g);
+    return c.text_val;
+  }
+  return d ? c.text_nxt : c.text_val;
+}
+
 // Byte p of the input; bytes outside [0, n) read as 0 (the first and last
 // granules are stored with those bytes cleared, granules beyond them read
-// as 0).  The lane keeps the last granule read.
+// as 0).
 __device__ __forceinline__ uint8_t byte_at(Ctx &c, int64_t p) {
   const int64_t a = p + c.t.head;
-  const int64_t g = a >> 4;
-  if ((uint32_t)g != c.text_at) {
-    c.text_val = text_granule(c, g);
-    c.text_at = (uint32_t)g;
-  }
-  const uint64_t w = (a & 8) ? c.text_val.y : c.text_val.x;
+  const u64x2 v = text_get(c, a >> 4);
+  const uint64_t w = (a & 8) ? v.y : v.x;
   return (uint8_t)(w >> (8 * (a & 7)));
 }
 
-// Little-endian 8 text bytes from p (bytes outside [0, n) read as 0), from
-// the lane's granule when it holds them.
-__device__ __forceinline__ uint64_t text8(const Ctx &c, int64_t p) {
+__device__ __forceinline__ uint64_t text8(Ctx &c, int64_t p) {
   const int64_t a = p + c.t.head;
   const int64_t g = a >> 4;
   const int64_t o = a & 15;
-  u64x2 v0 = ((uint32_t)g == c.text_at) ? c.text_val : text_granule(c, g);
+  const u64x2 v0 = text_get(c, g);
   if (o == 0) return v0.x;
   if (o == 8) return v0.y;
   const uint32_t sh = 8 * (uint32_t)(o & 7);
   if (o < 8) return (v0.x >> sh) | (v0.y << (64 - sh));
-  const u64x2 v1 = ((uint32_t)(g + 1) == c.text_at) ? c.text_val : text_granule(c, g + 1);
+  const u64x2 v1 = text_get(c, g + 1);
   return (v0.y >> sh) | (v1.x << (64 - sh));
 }
 
@@ -636,7 +655,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
 // leveled row k + 1 (mode.level_base = 1, mode.drop_last = 1: the last bound
 // opens no record).
 template <int MAXLV>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ? 5 : 1))) void query_batch_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ? 4 : 1))) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
     const int64_t *__restrict__ newline, const int64_t *__restrict__ num_lines,
     int64_t max_lines, const uint64_t *__restrict__ string,
@@ -653,7 +672,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
     const int64_t line = base + (threadIdx.x & 63);
     const bool valid = line < lines;
     Ctx c{text, n, string, leveled, level_size, num_levels, mode.level_base, kNoTag,
-          u64x2{0, 0}, kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}};
+          u64x2{0, 0}, u64x2{0, 0}, kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}};
     int64_t line_start = 0, line_end = 0;
     bool record = false;
     if (valid) {
@@ -721,7 +740,7 @@ __global__ __launch_bounds__(64) void query_files_kernel(
     if (text.last == 0) text.first_val = text.last_val;
     const uint64_t word_off = bf.offset >> 6;
     Ctx c{text, n, string + word_off, leveled + word_off, level_stride, num_levels, 0,
-          kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}};
+          kNoTag, u64x2{0, 0}, u64x2{0, 0}, kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}};
     const int64_t *starts = newline + bf.nl_offset;
     int64_t line_start = 0, line_end = 0;
     bool record = false;
