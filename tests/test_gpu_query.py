@@ -92,3 +92,26 @@ def test_byte_soup_queries_do_not_fault(gpu, orc):
     for q in ("$.a", "$.a[2]", "$.a[-1]", "$.a.b[?(@ == \"x\")]"):
         got, want = run_both(gpu, orc, data, q)
         np.testing.assert_array_equal(got, want)
+
+
+def test_key_lookup_forms(gpu, orc):
+    # the byte before ':' decides how a key lookup finds the key's last byte:
+    # an unescaped quote (compact keys), an escaped quote, a quote after an
+    # escaped backslash, spaces, or no string at all
+    lines = [
+        b'{"a":1,"b":2,"ab":3}',
+        b'{"k\\\\":1,"a":2}',
+        b'{"a\\"":1,"a":2}',
+        b'{x\\":1,"a":2}',
+        b'{"a" :1,"b"  :2}',
+        b'{"":1,"a":2}',
+        b'{"b":{"a":[1,{"a":2}]},"a":3}',
+        b'{ "a" : { "b" : 4 } }',
+        b'{"\\\\":1,"a\\\\b":2}',
+        b'{"x":"\\":","a":5}',
+    ]
+    data = b"\n".join(lines) + b"\n"
+    for q in ("$.a", "$.b", "$.ab", "$.b.a", "$.a.b", "$['k\\\\']", "$.x"):
+        print(f"[query] q={q}", flush=True)
+        got, want = run_both(gpu, orc, data, q, levels=4)
+        np.testing.assert_array_equal(got, want)
