@@ -40,9 +40,11 @@ uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 uint64_t tiles_for(uint64_t n) { return div_up(div_up(n, 64), gpj::kTileWords); }
 
-// Scratch layout of the index build: tile descriptors, then the tile counter.
+// Scratch layout of the index build: tile aggregates (multi-file: tile
+// descriptors), tile inclusive states (one uint64 per tile each), then the
+// tile counter and the failure flag.
 uint64_t scratch_bytes(uint64_t n) {
-  return tiles_for(n) * 8 + 256;
+  return tiles_for(n) * 16 + 256;
 }
 
 __global__ void empty_index_kernel(int64_t *newline, int64_t cap, int64_t *num_lines) {
@@ -57,7 +59,8 @@ gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int lev
                         uint32_t *counter, uint64_t tiles, hipStream_t st,
                         gpj::BatchSeg seg = gpj::BatchSeg{nullptr, nullptr, nullptr}) {
   const int aligned = (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
-  hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG>), dim3((uint32_t)tiles),
+  // single input: one extra workgroup (block 0) runs the prefix scan
+  hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG>), dim3((uint32_t)(SEG ? tiles : tiles + 1)),
                      dim3(gpj::kTileWords), 0, st, d_in, n, words, levels, aligned, d_string,
                      d_leveled, d_newline, cap, d_num_lines, desc, counter, tiles, seg);
   GPJ_HIP(hipGetLastError(), "index_kernel launch");
@@ -261,13 +264,13 @@ gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
     return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_build_into: null buffer");
   const uint64_t words = div_up(n, 64);
   const uint64_t tiles = tiles_for(n);
-  if (tiles > 0xFFFFFFFFull)
+  if (tiles >= 0xFFFFFFFFull) // the grid holds tiles + 1 workgroups
     return fail(GPJ_ERR_INVALID_ARGUMENT, "input too large for one index build");
   uint64_t *desc = static_cast<uint64_t *>(d_scratch);
-  uint32_t *counter = reinterpret_cast<uint32_t *>(desc + tiles);
+  uint32_t *counter = reinterpret_cast<uint32_t *>(desc + 2 * tiles);
   const uint64_t clear_blocks = std::min<uint64_t>(div_up(tiles, 256), 2048);
   hipLaunchKernelGGL(gpj::index_clear_kernel, dim3((uint32_t)clear_blocks), dim3(256), 0, st,
-                     desc, tiles, counter);
+                     desc, 2 * tiles, counter);
   GPJ_HIP(hipGetLastError(), "index_clear_kernel launch");
 #define GPJ_DISPATCH(L)                                                            \
   return launch_index<L>(d_in, n, words, levels, d_string, d_leveled, d_newline,    \
@@ -346,7 +349,7 @@ gpj_status gpj_index_build(const uint8_t *d_in, uint64_t n, int32_t levels,
   if (n > 0) {
     uint32_t failed = 0;
     const uint32_t *flag =
-        reinterpret_cast<const uint32_t *>(static_cast<uint64_t *>(scratch) + tiles_for(n)) + 1;
+        reinterpret_cast<const uint32_t *>(static_cast<uint64_t *>(scratch) + 2 * tiles_for(n)) + 1;
     GPJ_TRY(hipMemcpy(&failed, flag, 4, hipMemcpyDeviceToHost), "copy index status");
     if (failed) {
       cleanup();
@@ -573,10 +576,10 @@ gpj_status gpj_batch_index(const uint8_t *d_in, uint64_t total_bytes,
   if (tiles > 0xFFFFFFFFull)
     return fail(GPJ_ERR_INVALID_ARGUMENT, "batch too large for one index launch");
   uint64_t *desc = static_cast<uint64_t *>(d_scratch);
-  uint32_t *counter = reinterpret_cast<uint32_t *>(desc + tiles);
+  uint32_t *counter = reinterpret_cast<uint32_t *>(desc + 2 * tiles);
   const uint64_t clear_blocks = std::min<uint64_t>(div_up(tiles, 256), 2048);
   hipLaunchKernelGGL(gpj::index_clear_kernel, dim3((uint32_t)clear_blocks), dim3(256), 0, st,
-                     desc, tiles, counter);
+                     desc, 2 * tiles, counter);
   GPJ_HIP(hipGetLastError(), "index_clear_kernel launch");
   const gpj::BatchSeg seg{d_tile_file, reinterpret_cast<const gpj::BatchFile *>(d_files),
                           d_line_prefix};

@@ -9,10 +9,15 @@
 // (16 KiB) per workgroup tile.  Each lane classifies its 64 bytes with
 // byte-parallel SWAR compares and packs the per-class flags into 64-bit masks
 // with v_dot4_u32_u8.  The three carried states (escape parity, in-string
-// parity, nesting depth) plus the newline count cross tiles through a
-// decoupled look-back: each tile publishes, before it knows its carry-in,
-// a 4-entry transfer table indexed by (escape-in, string-in), then resolves
-// its exclusive prefix by walking predecessors' descriptors.
+// parity, nesting depth) plus the newline count cross tiles as follows: each
+// tile publishes, before it knows its carry-in, a 4-entry transfer table
+// indexed by (escape-in, string-in).  In a single-input build one extra
+// workgroup scans those tables in tile order, 1,024 tiles per round, and
+// writes every tile's inclusive state; a tile waits for its predecessor's.
+// (One atomic tile counter, or a look-back per tile, each cost about 11 ns
+// per tile in series on MI355X: the chain, not the work, bounded the
+// kernel.)  A multi-file batch resolves its prefixes with a decoupled
+// look-back over predecessors' descriptors, restarted at each file.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -272,6 +277,113 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Tiles a scan round resolves per lane: 4 waves x 64 lanes x 4 = 1,024.
+constexpr uint32_t kScanPer = 4;
+constexpr uint32_t kScanWaves = kTileWords / 64;
+
+// The prefix scanner of a single-input build (one workgroup): resolves tiles
+// in order.  Each round reads the aggregates of the next 1,024 tiles, takes
+// the longest run of them that is published, composes each lane's share, then
+// the lanes' composites with a wave scan and the waves' composites through
+// LDS, applies them to the running carry and writes every resolved tile's
+// inclusive state.  A tile waits only for its predecessor's inclusive state,
+// so the carry chain costs one descriptor round trip per round instead of a
+// look-back per tile.  Every wave takes the same branches (counts come
+// through LDS), so the barriers always match.
+__device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tiles,
+                           uint32_t *error_flag, uint32_t lane, uint32_t wv) {
+  __shared__ uint32_t s_ready[kScanWaves];
+  __shared__ uint64_t s_tab[kScanWaves], s_nl[kScanWaves];
+  Carry carry;
+  carry.esc = 0;
+  carry.str = 0;
+  carry.depth = -1;
+  carry.nl = 0;
+  uint64_t next = 0;
+  uint32_t spins = 0;
+  const uint64_t lo = (uint64_t)(wv * 64 + lane) * kScanPer; // this lane's first slot
+  while (next < num_tiles) {
+    const uint64_t b = next + lo;
+    uint64_t d[kScanPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k)
+      d[k] = (b + k < num_tiles) ? desc_load(&agg[b + k]) : 0ull;
+    // published tiles at the front of this lane's share, then of the wave's
+    uint32_t cnt = kScanPer;
+#pragma unroll
+    for (int k = kScanPer - 1; k >= 0; --k)
+      if ((d[k] >> kFlagShift) == 0) cnt = (uint32_t)k;
+    const uint64_t partial = __ballot(cnt < kScanPer);
+    const uint32_t fl = partial ? (uint32_t)__builtin_ctzll(partial) : 64u;
+    const uint32_t wave_ready =
+        fl * kScanPer + (fl < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)cnt, (int)fl) : 0u);
+    if (lane == 0) s_ready[wv] = wave_ready;
+    lds_barrier();
+    uint64_t ready = 0;
+    for (uint32_t w = 0; w < kScanWaves; ++w) {
+      const uint32_t r = s_ready[w];
+      ready += r;
+      if (r < 64 * kScanPer) break;
+    }
+    if (ready > num_tiles - next) ready = num_tiles - next;
+    if (ready == 0) {
+      lds_barrier(); // s_ready is rewritten next round
+      if (++spins > kMaxSpins) {
+        if (wv == 0 && lane == 0)
+          __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    // this lane's resolved tiles: [b, b + m)
+    const uint32_t m = lo >= ready ? 0u : (uint32_t)min((uint64_t)kScanPer, ready - lo);
+    uint64_t tab = kIdentityTable, nl = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k)
+      if (k < m) {
+        tab = tbl_compose(d[k] & ((1ull << 40) - 1), tab);
+        nl += (d[k] >> 40) & ((1ull << 22) - 1);
+      }
+    // inclusive scan over the wave's lanes, older lanes applied first
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint64_t ot = __shfl_up(tab, off, 64);
+      const uint64_t on = __shfl_up(nl, off, 64);
+      if (lane >= off) {
+        tab = tbl_compose(tab, ot);
+        nl += on;
+      }
+    }
+    if (lane == 63) {
+      s_tab[wv] = tab;
+      s_nl[wv] = nl;
+    }
+    lds_barrier();
+    // carry before this wave, and after the whole round
+    Carry before = carry;
+    for (uint32_t w = 0; w < kScanWaves; ++w) {
+      if (w == wv) before = carry;
+      carry = tbl_apply(s_tab[w], s_nl[w], carry);
+    }
+    uint64_t ex_tab = __shfl_up(tab, 1, 64), ex_nl = __shfl_up(nl, 1, 64);
+    if (lane == 0) {
+      ex_tab = kIdentityTable;
+      ex_nl = 0;
+    }
+    Carry c = tbl_apply(ex_tab, ex_nl, before);
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; ++k)
+      if (k < m) {
+        c = tbl_apply(d[k] & ((1ull << 40) - 1), (d[k] >> 40) & ((1ull << 22) - 1), c);
+        desc_store(&inc[b + k], encode_inclusive(c));
+      }
+    next += ready;
+    spins = 0;
+    lds_barrier(); // s_ready, s_tab and s_nl are rewritten next round
+  }
+}
+
 // Clears the tile descriptors and the tile counter (kept out of
 // hipMemsetAsync so a captured graph replays it correctly).
 __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
@@ -291,7 +403,7 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
 // tile never reads before its file's first tile, whose carry-in is the
 // initial state.
 template <int LMAX, bool SEG>
-__global__ __launch_bounds__(kTileWords) void index_kernel(
+__global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(6))) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
     int aligned, uint64_t *__restrict__ string_out,
     uint64_t *__restrict__ leveled_out, int64_t *__restrict__ newline_out,
@@ -317,9 +429,20 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
   const uint32_t lane = t & 63u;
   const uint32_t wv = t >> 6;
 
-  // 0. tile id from the counter: ids follow dispatch order, so every tile a
-  //    look-back waits on has started
-  if (t == 0) s_tile = atomicAdd(tile_counter, 1u);
+  // 0. tile id.  Single input: block 0 is the prefix scanner and block b
+  //    indexes tile b - 1 (workgroups are dispatched in block order, so the
+  //    tile the scanner waits on has been dispatched or is next).  Multi-file:
+  //    ids from the counter follow dispatch order, so every tile a look-back
+  //    waits on has started.
+  if (!SEG) {
+    if (blockIdx.x == 0) {
+      scan_tiles(desc, desc + num_tiles, num_tiles, error_flag, lane, wv);
+      return;
+    }
+    if (t == 0) s_tile = blockIdx.x - 1;
+  } else if (t == 0) {
+    s_tile = atomicAdd(tile_counter, 1u);
+  }
   lds_barrier();
   const uint64_t tile = s_tile;
   const uint64_t w = tile * kTileWords + t;
@@ -466,7 +589,31 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
     excl.str = 0;
     excl.depth = -1;
     excl.nl = 0;
-    if (tile != first_tile) {
+    if (!SEG && lane == 0) // the aggregate, for the scanner
+      desc_store(&desc[tile], tbl | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
+    if (!SEG && tile != first_tile) {
+      // wait for the predecessor's inclusive state (one load per poll; stop
+      // when a failure is flagged)
+      const uint64_t *inc = desc + num_tiles;
+      uint32_t spins = 0;
+      uint64_t d = 0;
+      for (;;) {
+        const uint64_t v = desc_load(&inc[tile - 1]);
+        d = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+        if ((d >> kFlagShift) == kFlagInclusive) break;
+        if (++spins > kMaxSpins ||
+            ((spins & 1023u) == 0 &&
+             __hip_atomic_load(error_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+          if (lane == 0)
+            __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          d = kFlagInclusive << kFlagShift;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      excl = decode_inclusive(d);
+    } else if (SEG && tile != first_tile) {
       if (lane == 0)
         desc_store(&desc[tile], tbl | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
       // Look back 64 predecessors at a time: lane i holds tile-1-i.  Compose
@@ -533,7 +680,7 @@ __global__ __launch_bounds__(kTileWords) void index_kernel(
     }
     if (lane == 0) {
       const Carry incl = tbl_apply(tbl, NL, excl);
-      desc_store(&desc[tile], encode_inclusive(incl));
+      if (SEG) desc_store(&desc[tile], encode_inclusive(incl));
       if (tile == last_tile) *lines_out = (int64_t)(1 + incl.nl);
       s_carry = excl;
     }
