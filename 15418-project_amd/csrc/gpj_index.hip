@@ -197,18 +197,21 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 }
 
 // ---- tile descriptors -------------------------------------------------------
-// Aggregate (flag 1): a transfer table over (esc_in, str_in) in {0,1}^2.
-//   entry e = esc_out | str_out << 1 | depth_delta(8 bits) << 2; 10 bits each
-//   bits 0..39: entries at (esc_in*2+str_in)*10; bits 40..61: newline count.
+// Aggregate (flag 1): a transfer table over the four input states
+//   s = esc_in | str_in << 1, packed: bits 0..7 the output state of input s
+//   at bits 2s (esc_out | str_out << 1), bits 8..39 its depth delta (mod
+//   256) in byte s; bits 40..61: newline count.
 // Inclusive (flag 2): esc bit 0, str bit 1, depth bits 2..9 (int8),
 //   newline count bits 10..61.
 constexpr uint64_t kFlagShift = 62;
 constexpr uint64_t kFlagAggregate = 1;
 constexpr uint64_t kFlagInclusive = 2;
 constexpr uint32_t kMaxSpins = 1u << 24;
-// (esc, str) -> (esc, str), depth delta 0, for all four inputs
-constexpr uint64_t kIdentityTable =
-    (0ull << 0) | (2ull << 10) | (1ull << 20) | (3ull << 30);
+// Tables in registers are unpacked: bits 0..31 the output state of input s
+// in byte s, bits 32..63 the depth delta of input s in byte s, so that
+// composing two is two byte permutes and a bytewise add.
+// Identity: s -> s, depth delta 0.
+constexpr uint64_t kIdentityTable = 0x03020100ull;
 // The state before an input's first byte as an inclusive descriptor: no
 // escape, outside strings, depth -1, no newlines.
 constexpr uint64_t kInitialInclusive = (0xFFull << 2) | (kFlagInclusive << kFlagShift);
@@ -219,30 +222,30 @@ struct Carry {
   uint64_t nl;
 };
 
-__device__ __forceinline__ uint32_t tbl_entry(uint64_t tbl, uint32_t esc,
-                                              uint32_t str) {
-  return (uint32_t)(tbl >> ((esc * 2 + str) * 10)) & 0x3FFu;
+// Unpacks the 40-bit table field of an aggregate descriptor.
+__device__ __forceinline__ uint64_t tbl_unpack(uint64_t d) {
+  const uint32_t a2 = (uint32_t)d & 0xFFu;
+  const uint32_t a = (a2 | (a2 << 6) | (a2 << 12) | (a2 << 18)) & 0x03030303u;
+  return (uint64_t)a | (((d >> 8) & 0xFFFFFFFFull) << 32);
 }
-// (f o g)(s) = f(g(s)): apply g first.
+// (f o g)(s) = f(g(s)): apply g first.  v_perm_b32 picks, for each byte s,
+// byte g(s) of f's states and of f's deltas; the deltas add bytewise mod 256.
 __device__ __forceinline__ uint64_t tbl_compose(uint64_t f, uint64_t g) {
-  uint64_t r = 0;
-#pragma unroll
-  for (uint32_t s = 0; s < 4; ++s) {
-    const uint32_t ge = tbl_entry(g, s >> 1, s & 1);
-    const uint32_t fe = tbl_entry(f, ge & 1, (ge >> 1) & 1);
-    const uint32_t dd = ((ge >> 2) + (fe >> 2)) & 0xFFu;
-    const uint32_t e = (fe & 3u) | (dd << 2);
-    r |= (uint64_t)e << (s * 10);
-  }
-  return r;
+  const uint32_t ga = (uint32_t)g, gb = (uint32_t)(g >> 32);
+  const uint32_t a = __builtin_amdgcn_perm(0u, (uint32_t)f, ga);
+  const uint32_t x = __builtin_amdgcn_perm(0u, (uint32_t)(f >> 32), ga);
+  const uint32_t b = ((gb & 0x7F7F7F7Fu) + (x & 0x7F7F7F7Fu)) ^ ((gb ^ x) & 0x80808080u);
+  return (uint64_t)a | ((uint64_t)b << 32);
 }
 __device__ __forceinline__ Carry tbl_apply(uint64_t tbl, uint64_t nl_add,
                                            Carry c) {
-  const uint32_t e = tbl_entry(tbl, c.esc, c.str);
+  const uint32_t sh = 8u * (c.esc | (c.str << 1));
+  const uint32_t st = ((uint32_t)tbl >> sh) & 3u;
+  const uint32_t dd = ((uint32_t)(tbl >> 32) >> sh) & 0xFFu;
   Carry o;
-  o.esc = e & 1u;
-  o.str = (e >> 1) & 1u;
-  o.depth = (int32_t)(int8_t)(uint8_t)((uint32_t)c.depth + (e >> 2));
+  o.esc = st & 1u;
+  o.str = st >> 1;
+  o.depth = (int32_t)(int8_t)(uint8_t)((uint32_t)c.depth + dd);
   o.nl = c.nl + nl_add;
   return o;
 }
@@ -342,7 +345,7 @@ __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tile
 #pragma unroll
     for (uint32_t k = 0; k < kScanPer; ++k)
       if (k < m) {
-        tab = tbl_compose(d[k] & ((1ull << 40) - 1), tab);
+        tab = tbl_compose(tbl_unpack(d[k]), tab);
         nl += (d[k] >> 40) & ((1ull << 22) - 1);
       }
     // inclusive scan over the wave's lanes, older lanes applied first
@@ -375,7 +378,7 @@ __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tile
 #pragma unroll
     for (uint32_t k = 0; k < kScanPer; ++k)
       if (k < m) {
-        c = tbl_apply(d[k] & ((1ull << 40) - 1), (d[k] >> 40) & ((1ull << 22) - 1), c);
+        c = tbl_apply(tbl_unpack(d[k]), (d[k] >> 40) & ((1ull << 22) - 1), c);
         desc_store(&inc[b + k], encode_inclusive(c));
       }
     next += ready;
@@ -580,8 +583,9 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(6)))
         const uint32_t esc_out = (tail & 2u) ? X : (tail & 1u);
         const uint32_t str_out = Y ^ tile_par ^ (X & g);
         const uint32_t dd = eff ? ((TOTb - D0b) & 0xFFu) : D0b;
-        const uint64_t e = esc_out | (str_out << 1) | (dd << 2);
-        tbl |= e << ((X * 2 + Y) * 10);
+        const uint32_t in = X | (Y << 1); // input state
+        tbl |= (uint64_t)(esc_out | (str_out << 1)) << (2 * in);
+        tbl |= (uint64_t)dd << (8 + 8 * in);
       }
     }
     Carry excl;
@@ -657,7 +661,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(6)))
         // reduction steps as the window needs (none when the nearest
         // predecessor is already inclusive)
         if (first_incl > 0) {
-          uint64_t t = (lane < first_incl) ? (d & ((1ull << 40) - 1)) : kIdentityTable;
+          uint64_t t = (lane < first_incl) ? tbl_unpack(d) : kIdentityTable;
           uint64_t nlv = (lane < first_incl) ? ((d >> 40) & ((1ull << 22) - 1)) : 0ull;
           for (uint32_t off = 1; off < first_incl; off <<= 1) {
             const uint64_t ot = __shfl_down(t, off, 64);
@@ -679,7 +683,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(6)))
       }
     }
     if (lane == 0) {
-      const Carry incl = tbl_apply(tbl, NL, excl);
+      const Carry incl = tbl_apply(tbl_unpack(tbl), NL, excl);
       if (SEG) desc_store(&desc[tile], encode_inclusive(incl));
       if (tile == last_tile) *lines_out = (int64_t)(1 + incl.nl);
       s_carry = excl;
