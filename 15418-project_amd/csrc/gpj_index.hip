@@ -400,13 +400,17 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
   }
 }
 
-// The fused single-pass index kernel.  One workgroup per 16 KiB tile; tile
-// ids come from an atomic counter so every predecessor a tile waits on has
-// already started.  SEG: multi-file batch (see BatchSeg); the look-back of a
-// tile never reads before its file's first tile, whose carry-in is the
-// initial state.
+// The fused single-pass index kernel.  One workgroup per 16 KiB tile, plus
+// the scanner workgroup (block 0) for a single input.  SEG: multi-file batch
+// (see BatchSeg); tile ids come from an atomic counter so every predecessor a
+// tile waits on has already started, and the look-back of a tile never reads
+// before its file's first tile, whose carry-in is the initial state.
+// Single input, up to 4 levels: 8 waves per SIMD (64 VGPRs; the few values
+// that spill are reloaded once per tile).  The carry takes ~9 us per tile to
+// arrive, so the tiles in flight, not the work per tile, set the throughput:
+// 8 waves run the 8 GiB input in 4.75 ms against 5.14 ms at 6.
 template <int LMAX, bool SEG>
-__global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(6))) void index_kernel(
+__global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SEG && LMAX <= 4) ? 8 : 6))) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
     int aligned, uint64_t *__restrict__ string_out,
     uint64_t *__restrict__ leveled_out, int64_t *__restrict__ newline_out,
