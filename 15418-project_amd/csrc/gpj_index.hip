@@ -405,10 +405,10 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
 // (see BatchSeg); tile ids come from an atomic counter so every predecessor a
 // tile waits on has already started, and the look-back of a tile never reads
 // before its file's first tile, whose carry-in is the initial state.
-// Single input, up to 4 levels: 8 waves per SIMD (64 VGPRs; the few values
-// that spill are reloaded once per tile).  The carry takes ~9 us per tile to
-// arrive, so the tiles in flight, not the work per tile, set the throughput:
-// 8 waves run the 8 GiB input in 4.75 ms against 5.14 ms at 6.
+// Single input, up to 4 levels: 8 waves per SIMD (64 VGPRs; only the
+// byte-wise load path of a partial word spills).  The carry takes ~9 us per
+// tile to arrive, so the tiles in flight, not the work per tile, set the
+// throughput: 8 waves run the 8 GiB input in 4.49 ms against 5.14 ms at 6.
 template <int LMAX, bool SEG>
 __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SEG && LMAX <= 4) ? 8 : 6))) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
@@ -451,9 +451,11 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
     s_tile = atomicAdd(tile_counter, 1u);
   }
   lds_barrier();
-  const uint64_t tile = s_tile;
-  const uint64_t w = tile * kTileWords + t;
-  const uint64_t pos = w * 64;
+  // tile and word base are wave-uniform (scalar registers); per-lane offsets
+  // are formed from them where needed rather than kept alive in VGPRs
+  const uint64_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_tile);
+  const uint64_t wbase = tile * kTileWords;
+  const uint64_t pos = (wbase + t) * 64;
   // the file this tile belongs to (the whole input when !SEG)
   uint64_t first_tile = 0, last_tile = num_tiles - 1, file_base = 0, file_end = n;
   int64_t *nl_out = newline_out;
@@ -744,7 +746,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
   //    slot of its row; row LMAX is a sink for depths outside [0, LMAX).
   //    Depth needs no 8-bit wrap inside the word: from an entry depth in
   //    [-128, 127], 64 changes cannot wrap into [0, LMAX).
-  const bool live = w < words;
+  const bool live = wbase + t < words;
   if (live) {
 #pragma unroll
     for (int k = 0; k < LMAX; ++k) {
@@ -773,11 +775,11 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
       const uint32_t row = min((uint32_t)depth, (uint32_t)LMAX);
       atomicOr(slot + row * (2 * kTileWords), Eh & ~done);
     }
-    string_out[w] = S;
+    (string_out + wbase)[t] = S;
 #pragma unroll
     for (int k = 0; k < LMAX; ++k)
       if (k < levels)
-        leveled_out[(uint64_t)k * words + w] =
+        (leveled_out + (uint64_t)k * words + wbase)[t] =
             (uint64_t)s_rows[k][0][t] | ((uint64_t)s_rows[k][1][t] << 32);
 
     // 9. newline positions
@@ -787,7 +789,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
     while (nlm) {
       const int b = __builtin_ctzll(nlm);
       nlm &= nlm - 1;
-      if (o < nl_cap) nl_out[o] = (int64_t)(pos - file_base + (uint64_t)b);
+      if (o < nl_cap) nl_out[o] = (int64_t)(wbase * 64 - file_base + t * 64 + (uint64_t)b);
       ++o;
     }
   }
