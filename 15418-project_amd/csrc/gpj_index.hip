@@ -280,23 +280,45 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// The scanner publishes the inclusive state of the last tile of each group
+// of kIncGroup tiles only (its coherent stores all leave one XCD); a tile
+// composes the aggregates of its group's earlier tiles itself.
+constexpr uint64_t kIncGroup = 8;
 // Tiles a scan round resolves per lane: 4 waves x 64 lanes x 4 = 1,024.
 constexpr uint32_t kScanPer = 4;
 constexpr uint32_t kScanWaves = kTileWords / 64;
 
+// Tables (unpacked: states in a, deltas in b) and newline counts of the
+// lane CTRL moves in, identity where no lane moves in (DPP row / wave moves).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void dpp_tbl(uint32_t a, uint32_t b, uint32_t nl, uint32_t &oa,
+                                        uint32_t &ob, uint32_t &onl) {
+  oa = (uint32_t)__builtin_amdgcn_update_dpp((int)kIdentityTable, (int)a, CTRL, ROWMASK, 0xF, false);
+  ob = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xF, false);
+  onl = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)nl, CTRL, ROWMASK, 0xF, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void scan_step(uint64_t &tab, uint32_t &nl) {
+  uint32_t oa, ob, onl;
+  dpp_tbl<CTRL, ROWMASK>((uint32_t)tab, (uint32_t)(tab >> 32), nl, oa, ob, onl);
+  tab = tbl_compose(tab, (uint64_t)oa | ((uint64_t)ob << 32));
+  nl += onl;
+}
+
 // The prefix scanner of a single-input build (one workgroup): resolves tiles
 // in order.  Each round reads the aggregates of the next 1,024 tiles, takes
 // the longest run of them that is published, composes each lane's share, then
-// the lanes' composites with a wave scan and the waves' composites through
-// LDS, applies them to the running carry and writes every resolved tile's
-// inclusive state.  A tile waits only for its predecessor's inclusive state,
-// so the carry chain costs one descriptor round trip per round instead of a
-// look-back per tile.  Every wave takes the same branches (counts come
+// the lanes' composites with a DPP wave scan and the waves' composites through
+// LDS, applies them to the running carry and writes the inclusive state of
+// each resolved group's last tile.  A tile waits only for its group's
+// carry-in, so the carry chain costs one descriptor round trip per round
+// instead of a look-back per tile.  Every wave takes the same branches (counts come
 // through LDS), so the barriers always match.
 __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tiles,
                            uint32_t *error_flag, uint32_t lane, uint32_t wv) {
   __shared__ uint32_t s_ready[kScanWaves];
-  __shared__ uint64_t s_tab[kScanWaves], s_nl[kScanWaves];
+  __shared__ uint64_t s_tab[kScanWaves];
+  __shared__ uint32_t s_nl[kScanWaves];
   Carry carry;
   carry.esc = 0;
   carry.str = 0;
@@ -341,23 +363,21 @@ __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tile
     }
     // this lane's resolved tiles: [b, b + m)
     const uint32_t m = lo >= ready ? 0u : (uint32_t)min((uint64_t)kScanPer, ready - lo);
-    uint64_t tab = kIdentityTable, nl = 0;
+    uint64_t tab = kIdentityTable;
+    uint32_t nl = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScanPer; ++k)
       if (k < m) {
         tab = tbl_compose(tbl_unpack(d[k]), tab);
-        nl += (d[k] >> 40) & ((1ull << 22) - 1);
+        nl += (uint32_t)(d[k] >> 40) & ((1u << 22) - 1);
       }
     // inclusive scan over the wave's lanes, older lanes applied first
-#pragma unroll
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-      const uint64_t ot = __shfl_up(tab, off, 64);
-      const uint64_t on = __shfl_up(nl, off, 64);
-      if (lane >= off) {
-        tab = tbl_compose(tab, ot);
-        nl += on;
-      }
-    }
+    scan_step<0x111, 0xF>(tab, nl); // row_shr:1
+    scan_step<0x112, 0xF>(tab, nl); // row_shr:2
+    scan_step<0x114, 0xF>(tab, nl); // row_shr:4
+    scan_step<0x118, 0xF>(tab, nl); // row_shr:8
+    scan_step<0x142, 0xA>(tab, nl); // row_bcast:15
+    scan_step<0x143, 0xC>(tab, nl); // row_bcast:31
     if (lane == 63) {
       s_tab[wv] = tab;
       s_nl[wv] = nl;
@@ -369,17 +389,15 @@ __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tile
       if (w == wv) before = carry;
       carry = tbl_apply(s_tab[w], s_nl[w], carry);
     }
-    uint64_t ex_tab = __shfl_up(tab, 1, 64), ex_nl = __shfl_up(nl, 1, 64);
-    if (lane == 0) {
-      ex_tab = kIdentityTable;
-      ex_nl = 0;
-    }
-    Carry c = tbl_apply(ex_tab, ex_nl, before);
+    uint32_t ea, eb, enl; // this lane's exclusive prefix (wave_shr:1)
+    dpp_tbl<0x138, 0xF>((uint32_t)tab, (uint32_t)(tab >> 32), nl, ea, eb, enl);
+    Carry c = tbl_apply((uint64_t)ea | ((uint64_t)eb << 32), enl, before);
 #pragma unroll
     for (uint32_t k = 0; k < kScanPer; ++k)
       if (k < m) {
         c = tbl_apply(tbl_unpack(d[k]), (d[k] >> 40) & ((1ull << 22) - 1), c);
-        desc_store(&inc[b + k], encode_inclusive(c));
+        if (((b + k) & (kIncGroup - 1)) == kIncGroup - 1 || b + k + 1 == num_tiles)
+          desc_store(&inc[b + k], encode_inclusive(c));
       }
     next += ready;
     spins = 0;
@@ -602,16 +620,32 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
     if (!SEG && lane == 0) // the aggregate, for the scanner
       desc_store(&desc[tile], tbl | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
     if (!SEG && tile != first_tile) {
-      // wait for the predecessor's inclusive state (one load per poll; stop
-      // when a failure is flagged)
+      // wait for the inclusive state before this tile's group (the initial
+      // state for group 0) and for the aggregates of the group's earlier
+      // tiles, all in one load per poll (lanes 0 .. need-1 and 63); stop when
+      // a failure is flagged
       const uint64_t *inc = desc + num_tiles;
+      const uint64_t g0 = tile & ~(kIncGroup - 1);
+      const uint32_t need = (uint32_t)(tile - g0);
       uint32_t spins = 0;
       uint64_t d = 0;
+      uint64_t acc = kIdentityTable, acc_nl = 0;
       for (;;) {
-        const uint64_t v = desc_load(&inc[tile - 1]);
-        d = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32) |
-            (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-        if ((d >> kFlagShift) == kFlagInclusive) break;
+        uint64_t v = 0;
+        if (lane < need) v = desc_load(&desc[g0 + lane]);
+        else if (lane == 63) v = g0 > 0 ? desc_load(&inc[g0 - 1]) : kInitialInclusive;
+        const uint64_t missing = __ballot(lane < need && (v >> kFlagShift) == 0);
+        d = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63) << 32) |
+            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+        if (missing == 0 && (d >> kFlagShift) == kFlagInclusive) {
+          for (uint32_t i = 0; i < need; ++i) {
+            const uint64_t a = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)i) << 32) |
+                               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)i);
+            acc = tbl_compose(tbl_unpack(a), acc);
+            acc_nl += (a >> 40) & ((1ull << 22) - 1);
+          }
+          break;
+        }
         if (++spins > kMaxSpins ||
             ((spins & 1023u) == 0 &&
              __hip_atomic_load(error_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
@@ -622,7 +656,8 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      excl = decode_inclusive(d);
+      excl = tbl_apply(acc, acc_nl, decode_inclusive(d));
+
     } else if (SEG && tile != first_tile) {
       if (lane == 0)
         desc_store(&desc[tile], tbl | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
