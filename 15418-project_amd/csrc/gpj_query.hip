@@ -90,14 +90,13 @@ typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 // The input as aligned 16-byte granules.  Granule g holds file offsets
 // [16g - head, 16g - head + 16), where head is the input address modulo 16.
-// The first and last granules may hold bytes outside [0, n); they are
-// assembled once with those bytes cleared, every granule between is loaded
-// whole.
+// The first and last granules may hold bytes outside [0, n); they are loaded
+// like the others (an aligned granule never crosses a page, so its bytes are
+// readable wherever one of them is) and those bytes are cleared.
 struct Text {
   const u64x2 *__restrict__ blocks;
   int64_t head;
   int64_t last; // index of the last granule that holds a byte of the input
-  u64x2 first_val, last_val;
 };
 
 __device__ __forceinline__ u64x2 clear_past(u64x2 v, int64_t valid) {
@@ -110,37 +109,12 @@ __device__ __forceinline__ u64x2 clear_past(u64x2 v, int64_t valid) {
   return v;
 }
 
-__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
-
 __device__ __forceinline__ Text make_text(const uint8_t *__restrict__ file, int64_t n) {
   Text t;
   const int64_t head = (int64_t)(reinterpret_cast<uintptr_t>(file) & 15);
   t.head = head;
   t.blocks = reinterpret_cast<const u64x2 *>(file - head);
   t.last = n > 0 ? (n - 1 + head) >> 4 : -1;
-  t.first_val = u64x2{0, 0};
-  t.last_val = u64x2{0, 0};
-  for (int64_t k = 0; k < 16; ++k) {
-    const int64_t p0 = k - head, p1 = 16 * t.last + k - head;
-    const uint64_t sh = 8 * (k & 7);
-    if (p0 >= 0 && p0 < n) {
-      if (k < 8) t.first_val.x |= (uint64_t)file[p0] << sh;
-      else t.first_val.y |= (uint64_t)file[p0] << sh;
-    }
-    if (p1 >= 0 && p1 < n) {
-      if (k < 8) t.last_val.x |= (uint64_t)file[p1] << sh;
-      else t.last_val.y |= (uint64_t)file[p1] << sh;
-    }
-  }
-  // the same for every lane: keep in scalar registers
-  t.first_val.x = uniform64(t.first_val.x);
-  t.first_val.y = uniform64(t.first_val.y);
-  t.last_val.x = uniform64(t.last_val.x);
-  t.last_val.y = uniform64(t.last_val.y);
   return t;
 }
 
@@ -181,9 +155,17 @@ struct Ctx {
 // Granule g of the input (granules outside the input read as 0).
 __device__ __forceinline__ u64x2 text_granule(const Ctx &c, int64_t g) {
   if (g > 0 && g < c.t.last) return c.t.blocks[g]; // interior granule
-  if (g == 0) return c.t.first_val;
-  if (g == c.t.last) return c.t.last_val;
-  return u64x2{0, 0};
+  if (g < 0 || g > c.t.last) return u64x2{0, 0};
+  // the first or the last granule: clear the bytes before offset 0 and
+  // from offset n on
+  const int64_t base = 16 * g - c.t.head; // file offset of the granule's byte 0
+  u64x2 v = clear_past(c.t.blocks[g], c.n - base);
+  const int64_t lo = -base; // bytes before offset 0
+  if (lo > 0) {
+    v.x &= lo >= 8 ? 0ull : (~0ull << (8 * lo));
+    v.y &= lo >= 16 ? 0ull : (lo <= 8 ? ~0ull : (~0ull << (8 * (lo - 8))));
+  }
+  return v;
 }
 
 // Fills the lane's granule pair with granules g and g + 1 (one round trip).
@@ -503,26 +485,41 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       const int64_t lend = s.level_end.get(s.lvl);
       const uint64_t key0 = ir.u64(key_pos);
       const uint64_t mask0 = arg >= 8 ? ~0ull : ((1ull << (8 * arg)) - 1);
+      // A key of at most 5 bytes that does not end in a backslash: the 8
+      // bytes ending at a compact key's ':' hold its opening quote, the key,
+      // its closing quote and the ':', so one text read decides the key
+      // (wave-uniform pattern and mask over bytes 5 - arg .. 7).
+      const bool short_key =
+          arg <= 5 && (arg == 0 || ((key0 >> (8 * (arg - 1))) & 0xFFu) != '\\');
+      const uint32_t jss = short_key ? (uint32_t)(5 - arg) : 0u;
+      const uint64_t pattern = ((uint64_t)'"' << (8 * jss)) | ((key0 & mask0) << (8 * (jss + 1))) |
+                               (0x22ull << 48) | (0x3Aull << 56);
+      const uint64_t pmask = ~0ull << (8 * jss);
       int64_t idx = s.idx;
       for (;;) {
         idx = next_structural(c, lend, idx + 1, s.lvl);
-        if (byte_at(c, idx) != ':') break;
+        const uint64_t w = text8(c, idx - 7); // bytes idx - 7 .. idx
+        if ((w >> 56) != ':') break;
         // idx is outside strings, so an unescaped quote right before it closes
         // a string whose last in-string byte (body or opening quote) is
-        // idx - 2: the string mask's last set bit in [line_start + 1, idx - 1]
-        int64_t string_end;
-        if (idx - 2 >= line_start + 1 && byte_at(c, idx - 1) == '"' && byte_at(c, idx - 2) != '\\')
-          string_end = idx - 2;
-        else
-          string_end = last_string_bit(c, line_start + 1, idx - 1);
-        const int64_t ss = string_end - arg;
-        bool match = ss >= line_start && byte_at(c, ss) == '"';
-        if (match && arg > 0) {
-          match = ((text8(c, ss + 1) ^ key0) & mask0) == 0;
-          for (int32_t k = 8; match && k < arg; k += 8) {
-            const int32_t rest = arg - k;
-            const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
-            match = ((text8(c, ss + 1 + k) ^ ir.u64(key_pos + (uint32_t)k)) & m) == 0;
+        // idx - 2 (compact): the string mask's last set bit in
+        // [line_start + 1, idx - 1]
+        const bool compact =
+            idx - 2 >= line_start + 1 && ((w >> 48) & 0xFFu) == '"' && ((w >> 40) & 0xFFu) != '\\';
+        bool match;
+        if (compact && short_key) {
+          match = idx - 2 - arg >= line_start && ((w ^ pattern) & pmask) == 0;
+        } else {
+          const int64_t string_end = compact ? idx - 2 : last_string_bit(c, line_start + 1, idx - 1);
+          const int64_t ss = string_end - arg;
+          match = ss >= line_start && byte_at(c, ss) == '"';
+          if (match && arg > 0) {
+            match = ((text8(c, ss + 1) ^ key0) & mask0) == 0;
+            for (int32_t k = 8; match && k < arg; k += 8) {
+              const int32_t rest = arg - k;
+              const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
+              match = ((text8(c, ss + 1 + k) ^ ir.u64(key_pos + (uint32_t)k)) & m) == 0;
+            }
           }
         }
         if (match) break;
@@ -735,9 +732,6 @@ __global__ __launch_bounds__(64) void query_files_kernel(
     text.blocks = reinterpret_cast<const u64x2 *>(file); // files are tile aligned
     text.head = 0;
     text.last = (n - 1) >> 4;
-    text.first_val = text.blocks[0];
-    text.last_val = clear_past(text.blocks[text.last], n - 16 * text.last);
-    if (text.last == 0) text.first_val = text.last_val;
     const uint64_t word_off = bf.offset >> 6;
     Ctx c{text, n, string + word_off, leveled + word_off, level_stride, num_levels, 0,
           kNoTag, u64x2{0, 0}, u64x2{0, 0}, kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}};
