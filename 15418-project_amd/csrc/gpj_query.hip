@@ -273,16 +273,31 @@ __device__ __forceinline__ int64_t last_row_bit(Ctx &c, int64_t row, bool odd, i
   }
 }
 
-// First structural bit of `level` in [idx, level_end], else level_end
+// A leveled row as next_structural reads it: the row's first word index and
+// its parity (which row cache it uses); valid = false for levels outside the
+// built rows, which read as empty.
+struct RowRef {
+  int64_t row;
+  bool odd;
+  bool valid;
+};
+__device__ __forceinline__ RowRef row_ref(const Ctx &c, int level) {
+  RowRef r;
+  r.valid = level >= 0 && level + c.base < c.num_levels;
+  r.row = c.level_size * (int64_t)(level + c.base);
+  r.odd = ((level + c.base) & 1) != 0;
+  return r;
+}
+
+// First structural bit of row r in [idx, level_end], else level_end
 // (find_next_structural_char, query.cu:79-115).
-__device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, int64_t idx, int level) {
+__device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, int64_t idx,
+                                                   const RowRef &r) {
   if (idx >= level_end) return level_end;
-  if (level < 0 || level + c.base >= c.num_levels) return level_end;
+  if (!r.valid) return level_end;
   if (idx < 0) idx = 0;
-  const int64_t row = c.level_size * (int64_t)(level + c.base);
-  const bool odd = ((level + c.base) & 1) != 0;
   int64_t wi = idx >> 6, wb = wi << 6;
-  uint64_t word = row_word(c, row + wi, odd) & (~0ull << (idx - wb));
+  uint64_t word = row_word(c, r.row + wi, r.odd) & (~0ull << (idx - wb));
   for (;;) {
     uint64_t cand = word;
     const int64_t last = level_end - wb;
@@ -291,8 +306,11 @@ __device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, in
     ++wi;
     wb += 64;
     if (wb > level_end || wi >= c.level_size) return level_end;
-    word = row_word(c, row + wi, odd);
+    word = row_word(c, r.row + wi, r.odd);
   }
+}
+__device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, int64_t idx, int level) {
+  return next_structural(c, level_end, idx, row_ref(c, level));
 }
 
 // find_previous_structural_char, query.cu:117-128
@@ -495,9 +513,10 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       const uint64_t pattern = ((uint64_t)'"' << (8 * jss)) | ((key0 & mask0) << (8 * (jss + 1))) |
                                (0x22ull << 48) | (0x3Aull << 56);
       const uint64_t pmask = ~0ull << (8 * jss);
+      const RowRef rr = row_ref(c, s.lvl);
       int64_t idx = s.idx;
       for (;;) {
-        idx = next_structural(c, lend, idx + 1, s.lvl);
+        idx = next_structural(c, lend, idx + 1, rr);
         const uint64_t w = text8(c, idx - 7); // bytes idx - 7 .. idx
         if ((w >> 56) != ':') break;
         // idx is outside strings, so an unescaped quote right before it closes
@@ -523,7 +542,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
           }
         }
         if (match) break;
-        idx = next_structural(c, lend, idx + 1, s.lvl);
+        idx = next_structural(c, lend, idx + 1, rr);
         if (byte_at(c, idx) == '}') { fail = true; break; }
       }
       s.idx = idx + 1;
