@@ -75,6 +75,25 @@ def test_backslash_runs_across_tiles(gpu, orc):
     check_index(gpu, orc, data, 4)
 
 
+def soup_np(seed: int, n: int) -> bytes:
+    # bytes dense in the structural characters, with runs of backslashes
+    rng = np.random.default_rng(seed)
+    palette = np.frombuffer(b'\\"{}[]:,\nabcd x', dtype=np.uint8)
+    out = palette[rng.integers(0, palette.size, n)]
+    for at in rng.integers(0, n - 200, n // 20000):
+        out[at:at + int(rng.integers(1, 140))] = ord("\\")
+    return out.tobytes()
+
+
+def test_many_scan_rounds(gpu, orc):
+    # more 16 KiB tiles than one round of the carry scan resolves (1,024),
+    # with escape, string and depth carries across every tile, every group
+    # of 8 tiles and every round; one string spans more than a round
+    long_string = b'"' + b"x" * (17 << 20) + b'"'
+    data = soup_np(7, 20 << 20) + long_string + soup_np(8, (20 << 20) + 12345)
+    check_index(gpu, orc, data, 4)
+
+
 def test_strings_spanning_tiles(gpu, orc):
     long_string = b'"' + b"{[:,]}" * 20000 + b'"'
     data = b'{"a":' + long_string + b',"b":[' + long_string + b',{"c":2}]}\n' * 3
