@@ -41,8 +41,8 @@ uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 uint64_t tiles_for(uint64_t n) { return div_up(div_up(n, 64), gpj::kTileWords); }
 
 // Scratch layout of the index build: tile aggregates (multi-file: tile
-// descriptors), tile inclusive states (one uint64 per tile each), then the
-// tile counter and the failure flag.
+// descriptors), tile inclusive states (one uint64 per tile each), then a
+// spare word and the failure flag.
 uint64_t scratch_bytes(uint64_t n) {
   return tiles_for(n) * 16 + 256;
 }

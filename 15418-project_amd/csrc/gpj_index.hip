@@ -13,11 +13,12 @@
 // tile publishes, before it knows its carry-in, a 4-entry transfer table
 // indexed by (escape-in, string-in).  In a single-input build one extra
 // workgroup scans those tables in tile order, 1,024 tiles per round, and
-// writes every tile's inclusive state; a tile waits for its predecessor's.
-// (One atomic tile counter, or a look-back per tile, each cost about 11 ns
-// per tile in series on MI355X: the chain, not the work, bounded the
-// kernel.)  A multi-file batch resolves its prefixes with a decoupled
-// look-back over predecessors' descriptors, restarted at each file.
+// writes the inclusive state of every 8th tile; a tile waits for the one
+// before its group of 8.  (One atomic tile counter, or a look-back per tile,
+// each cost about 11 ns per tile in series on MI355X: the chain, not the
+// work, bounded the kernel.)  A multi-file batch resolves its prefixes with a
+// decoupled look-back over predecessors' descriptors, restarted at each file
+// (a median file has 16 tiles).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -405,24 +406,24 @@ __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tile
   }
 }
 
-// Clears the tile descriptors and the tile counter (kept out of
-// hipMemsetAsync so a captured graph replays it correctly).
+// Clears the tile descriptors, a spare word and the failure flag (kept out
+// of hipMemsetAsync so a captured graph replays it correctly).
 __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
                                    uint32_t *counter) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t k = i; k < count; k += stride) desc[k] = 0;
   if (i == 0) {
-    counter[0] = 0; // tile counter
+    counter[0] = 0; // spare
     counter[1] = 0; // look-back failure flag
   }
 }
 
 // The fused single-pass index kernel.  One workgroup per 16 KiB tile, plus
 // the scanner workgroup (block 0) for a single input.  SEG: multi-file batch
-// (see BatchSeg); tile ids come from an atomic counter so every predecessor a
-// tile waits on has already started, and the look-back of a tile never reads
-// before its file's first tile, whose carry-in is the initial state.
+// (see BatchSeg); a tile resolves its carry with a decoupled look-back that
+// never reads before its file's first tile, whose carry-in is the initial
+// state.
 // Single input, up to 4 levels: 8 waves per SIMD (64 VGPRs; only the
 // byte-wise load path of a partial word spills).  The carry takes ~9 us per
 // tile to arrive, so the tiles in flight, not the work per tile, set the
@@ -454,11 +455,11 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
   const uint32_t lane = t & 63u;
   const uint32_t wv = t >> 6;
 
-  // 0. tile id.  Single input: block 0 is the prefix scanner and block b
-  //    indexes tile b - 1 (workgroups are dispatched in block order, so the
-  //    tile the scanner waits on has been dispatched or is next).  Multi-file:
-  //    ids from the counter follow dispatch order, so every tile a look-back
-  //    waits on has started.
+  // 0. tile id from the block: workgroups are dispatched in block order, so
+  //    the tile a wait depends on has been dispatched or is next in line.
+  //    Single input: block 0 is the prefix scanner, block b indexes tile
+  //    b - 1.  Multi-file: block b indexes tile b.  (An atomic tile counter
+  //    costs ~11 ns per tile in series on MI355X.)
   if (!SEG) {
     if (blockIdx.x == 0) {
       scan_tiles(desc, desc + num_tiles, num_tiles, error_flag, lane, wv);
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
     }
     if (t == 0) s_tile = blockIdx.x - 1;
   } else if (t == 0) {
-    s_tile = atomicAdd(tile_counter, 1u);
+    s_tile = blockIdx.x;
   }
   lds_barrier();
   // tile and word base are wave-uniform (scalar registers); per-lane offsets
