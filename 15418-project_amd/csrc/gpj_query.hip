@@ -394,6 +394,19 @@ template <int MAXLV> struct LaneState {
         fr_pack{w.fr_pack, lane}, fr_sep{w.fr_sep, lane} {}
 };
 
+// MOVE_DOWN: one level deeper; a level entered for the first time ends at
+// the next structural character of the level above.  False: the path fails.
+template <int MAXLV>
+__device__ __forceinline__ bool move_down(Ctx &c, LaneState<MAXLV> &s) {
+  if (s.lvl + 1 >= MAXLV) return false;
+  ++s.lvl;
+  if (s.level_end.get(s.lvl) == -1) {
+    s.level_end.set(s.lvl, next_structural(c, s.level_end.get(s.lvl - 1), s.idx + 1, s.lvl - 1));
+    while (byte_at(c, s.idx) == ' ') ++s.idx;
+  }
+  return true;
+}
+
 // A finished path (END or any failure): resume the innermost element
 // iteration at its next element, or end the record.  Returns false when the
 // record is done.
@@ -465,6 +478,13 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       key_pos = qn;
       if (op == kMoveToKey || op == kExpressionStringEquals) qn += (uint32_t)arg;
     }
+    // a key step that the bytecode follows with MOVE_DOWN runs both in one
+    // dispatch (each lane still executes them in order)
+    bool then_down = false;
+    if (op == kMoveToKey && qn < ir.len && ir.byte(qn) == kMoveDown) {
+      then_down = true;
+      ++qn;
+    }
     if (!(live && s.qp == qpu)) continue;
 
     s.qp = (int32_t)qn;
@@ -490,13 +510,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       --s.lvl;
       break;
     case kMoveDown:
-      if (s.lvl + 1 >= MAXLV) { fail = true; break; }
-      ++s.lvl;
-      if (s.level_end.get(s.lvl) == -1) {
-        s.level_end.set(s.lvl,
-                        next_structural(c, s.level_end.get(s.lvl - 1), s.idx + 1, s.lvl - 1));
-        while (byte_at(c, s.idx) == ' ') ++s.idx;
-      }
+      fail = !move_down(c, s);
       break;
     case kMoveToKey: {
       if (arg < 0 || byte_at(c, s.idx) != '{') { fail = true; break; }
@@ -546,6 +560,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
         if (byte_at(c, idx) == '}') { fail = true; break; }
       }
       s.idx = idx + 1;
+      if (!fail && then_down) fail = !move_down(c, s);
       break;
     }
     case kMoveToIndex: {
