@@ -478,13 +478,13 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       key_pos = qn;
       if (op == kMoveToKey || op == kExpressionStringEquals) qn += (uint32_t)arg;
     }
-    // a key step that the bytecode follows with MOVE_DOWN runs both in one
-    // dispatch (each lane still executes them in order)
-    bool then_down = false;
-    if (op == kMoveToKey && qn < ir.len && ir.byte(qn) == kMoveDown) {
-      then_down = true;
-      ++qn;
-    }
+    // a key step that the bytecode follows with MOVE_DOWN, and a store that
+    // it follows with END, run both in one dispatch (each lane still
+    // executes them in order)
+    const uint32_t next_op = qn < ir.len ? ir.byte(qn) : (uint32_t)kOffEnd;
+    const bool then_down = op == kMoveToKey && next_op == kMoveDown;
+    const bool then_end = op == kStoreResult && next_op == kEnd;
+    if (then_down || then_end) ++qn;
     if (!(live && s.qp == qpu)) continue;
 
     s.qp = (int32_t)qn;
@@ -501,6 +501,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
         res[s.count * 2 + 1] = end_str;
       }
       ++s.count;
+      if (then_end) fail = true; // END: the path is finished
       break;
     }
     case kMoveUp:
