@@ -394,6 +394,18 @@ template <int MAXLV> struct LaneState {
         fr_pack{w.fr_pack, lane}, fr_sep{w.fr_sep, lane} {}
 };
 
+// RESET_POSITION: back to the innermost mark, clearing the levels above it up
+// to the first one already clear.  False: no mark (the path fails).
+template <int MAXLV>
+__device__ __forceinline__ bool reset_position(LaneState<MAXLV> &s) {
+  if (s.mi < 0) return false;
+  s.idx = s.marked_pos.get(s.mi);
+  s.lvl = s.marked_lvl.get(s.mi);
+  --s.mi;
+  s.level_end.fill_above_until(s.lvl, -1);
+  return true;
+}
+
 // MOVE_DOWN: one level deeper; a level entered for the first time ends at
 // the next structural character of the level above.  False: the path fails.
 template <int MAXLV>
@@ -478,13 +490,16 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       key_pos = qn;
       if (op == kMoveToKey || op == kExpressionStringEquals) qn += (uint32_t)arg;
     }
-    // a key step that the bytecode follows with MOVE_DOWN, and a store that
-    // it follows with END, run both in one dispatch (each lane still
-    // executes them in order)
+    // a key, index or element step that the bytecode follows with MOVE_DOWN,
+    // a store followed by END and a string test followed by RESET_POSITION
+    // run both in one dispatch (each lane still executes them in order)
     const uint32_t next_op = qn < ir.len ? ir.byte(qn) : (uint32_t)kOffEnd;
-    const bool then_down = op == kMoveToKey && next_op == kMoveDown;
+    const bool then_down = (op == kMoveToKey || op == kMoveToIndex ||
+                            op == kMoveToIndexReverse || op == kMoveToEach) &&
+                           next_op == kMoveDown;
     const bool then_end = op == kStoreResult && next_op == kEnd;
-    if (then_down || then_end) ++qn;
+    const bool then_reset = op == kExpressionStringEquals && next_op == kResetPosition;
+    if (then_down || then_end || then_reset) ++qn;
     if (!(live && s.qp == qpu)) continue;
 
     s.qp = (int32_t)qn;
@@ -588,6 +603,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       s.curr_index.set(s.lvl, ci);
       if (!fail && byte_at(c, idx) == ']') fail = true;
       s.idx = idx + 1;
+      if (!fail && then_down) fail = !move_down(c, s);
       break;
     }
     case kMoveToIndexReverse: {
@@ -614,6 +630,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       }
       s.curr_index.set(s.lvl, ci);
       s.idx = idx + 1;
+      if (!fail && then_down) fail = !move_down(c, s);
       break;
     }
     case kMarkPosition:
@@ -622,15 +639,9 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       s.marked_pos.set(s.mi, s.idx);
       s.marked_lvl.set(s.mi, (uint8_t)s.lvl);
       break;
-    case kResetPosition: {
-      if (s.mi < 0) { fail = true; break; }
-      s.idx = s.marked_pos.get(s.mi);
-      s.lvl = s.marked_lvl.get(s.mi);
-      --s.mi;
-      // clear the levels above, up to the first one already clear
-      s.level_end.fill_above_until(s.lvl, -1);
+    case kResetPosition:
+      fail = !reset_position(s);
       break;
-    }
     case kExpressionStringEquals: {
       if (arg < 0) { fail = true; break; }
       const int64_t lend = s.level_end.get(s.lvl);
@@ -643,6 +654,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
         const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
         if (((text8(c, s.idx + k) ^ ir.u64(key_pos + (uint32_t)k)) & m) != 0) { fail = true; break; }
       }
+      if (!fail && then_reset) fail = !reset_position(s);
       break;
     }
     case kMoveToEach: {
@@ -654,11 +666,14 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       int64_t p = s.idx + 1;
       while (p < first && byte_at(c, p) == ' ') ++p;
       if (p == first && byte_at(c, first) == ']') { fail = true; break; } // empty array
-      s.fr_pack.set(s.nfr, (uint32_t)s.qp | ((uint32_t)s.lvl << 16) |
+      // the frame resumes at the instruction after MOVE_TO_EACH (a fused
+      // MOVE_DOWN runs again for every element)
+      s.fr_pack.set(s.nfr, (uint32_t)(then_down ? s.qp - 1 : s.qp) | ((uint32_t)s.lvl << 16) |
                                ((uint32_t)(s.mi + 1) << 21));
       s.fr_sep.set(s.nfr, s.idx);
       ++s.nfr;
       ++s.idx;
+      if (then_down) fail = !move_down(c, s);
       break;
     }
     case kJumpIfNotArray:
