@@ -544,10 +544,9 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
                                (0x22ull << 48) | (0x3Aull << 56);
       const uint64_t pmask = ~0ull << (8 * jss);
       const RowRef rr = row_ref(c, s.lvl);
-      int64_t idx = s.idx;
+      int64_t idx = next_structural(c, lend, s.idx + 1, rr);
+      uint64_t w = text8(c, idx - 7); // bytes idx - 7 .. idx
       for (;;) {
-        idx = next_structural(c, lend, idx + 1, rr);
-        const uint64_t w = text8(c, idx - 7); // bytes idx - 7 .. idx
         if ((w >> 56) != ':') break;
         // idx is outside strings, so an unescaped quote right before it closes
         // a string whose last in-string byte (body or opening quote) is
@@ -572,8 +571,21 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
           }
         }
         if (match) break;
-        idx = next_structural(c, lend, idx + 1, rr);
-        if (byte_at(c, idx) == '}') { fail = true; break; }
+        // the separator after the value ends the object if it is '}'; the
+        // next key's 8-byte window usually holds it as well
+        const int64_t sep = next_structural(c, lend, idx + 1, rr);
+        const int64_t nxt = next_structural(c, lend, sep + 1, rr);
+        const uint64_t wn = text8(c, nxt - 7);
+        const int64_t off = sep - (nxt - 7);
+        const uint32_t sep_byte =
+            (off >= 0 && off < 8) ? (uint32_t)(wn >> (8 * off)) & 0xFFu : (uint32_t)byte_at(c, sep);
+        if (sep_byte == '}') {
+          idx = sep;
+          fail = true;
+          break;
+        }
+        idx = nxt;
+        w = wn;
       }
       s.idx = idx + 1;
       if (!fail && then_down) fail = !move_down(c, s);
