@@ -87,6 +87,7 @@ struct IrReader {
 };
 
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+typedef int64_t i64x2a8 __attribute__((ext_vector_type(2), aligned(8)));
 
 // The input as aligned 16-byte granules.  Granule g holds file offsets
 // [16g - head, 16g - head + 16), where head is the input address modulo 16.
@@ -511,10 +512,8 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       int64_t end_str = lend;
       while (byte_at(c, s.idx) == ' ' && s.idx < lend) ++s.idx;
       while (end_str > s.idx && byte_at(c, end_str - 1) == ' ') --end_str;
-      if (s.count < cap) {
-        res[s.count * 2] = s.idx;
-        res[s.count * 2 + 1] = end_str;
-      }
+      if (s.count < cap) // one 16-byte store (the span array is 8-byte aligned)
+        *reinterpret_cast<i64x2a8 *>(res + s.count * 2) = i64x2a8{s.idx, end_str};
       ++s.count;
       if (then_end) fail = true; // END: the path is finished
       break;
