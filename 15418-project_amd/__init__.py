@@ -151,6 +151,7 @@ def lib():
         L.gpj_index_build_into.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp, vp]
         L.gpj_debug_escape_quote.argtypes = [vp, u64, vp, vp, vp]
         L.gpj_debug_copy.argtypes = [vp, vp, u64, vp]
+        L.gpj_debug_set_narrow_span.argtypes = [i64]
         L.gpj_query_run.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp, vp]
         L.gpj_query_run_ext.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp,
                                         vp, vp]
@@ -187,7 +188,7 @@ def lib():
                      "gpj_batch_tile_map", "gpj_batch_index", "gpj_batch_query",
                      "gpj_batch_per_file", "gpj_graph_begin", "gpj_graph_end",
                      "gpj_graph_launch", "gpj_graph_destroy", "gpj_stream_create",
-                     "gpj_stream_destroy"):
+                     "gpj_stream_destroy", "gpj_debug_set_narrow_span"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -509,6 +510,24 @@ def run_query_records(dev: DeviceBuffer | int, n: int, index: BuiltIndex, bounds
             spans[np.arange(cap)[None, :] >= counts[:, None]] = -1
         out.append((spans, counts))
     return out
+
+
+class narrow_span:
+    """Context manager: the query kernels hand records reaching `nbytes` from
+    their start (or backward searches reaching that far below it) to their
+    64-bit kernel (default 2^31 - 2^20).  Results never depend on it; tests
+    lower it to drive the 64-bit kernel on small inputs."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = nbytes
+
+    def __enter__(self):
+        check(lib().gpj_debug_set_narrow_span(self.nbytes), "narrow span")
+        return self
+
+    def __exit__(self, *exc):
+        check(lib().gpj_debug_set_narrow_span(0), "narrow span")
+        return False
 
 
 def debug_escape_quote(dev: DeviceBuffer | int, n: int, stream=None):

@@ -5,6 +5,7 @@
 // gpj_status and leave a thread-local message for gpj_last_error().
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -396,6 +397,10 @@ gpj_status gpj_debug_escape_quote(const uint8_t *d_in, uint64_t n, uint64_t *d_e
 
 namespace {
 
+// Record extent the narrow query kernels take (gpj::PosMode); tests lower it
+// through gpj_debug_set_narrow_span to drive the wide kernel on small inputs.
+std::atomic<int64_t> g_narrow_span{gpj::kNarrowSpan};
+
 // Deepest interpreter level of a query batch (0 = unknown counts as 15).
 int deepest_level(const gpj_query_desc *queries, int32_t num_queries) {
   int deepest = 0;
@@ -442,21 +447,39 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
   // one wave per block (the kernel keeps each wave's interpreter arrays in
   // its block's LDS)
   const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_records, 64), 1u << 22);
-#define GPJ_QLAUNCH(LV)                                                                   \
-  hipLaunchKernelGGL(gpj::query_batch_kernel<LV>, dim3((uint32_t)blocks), dim3(64), 0,       \
+  // 32-bit positions; records the narrow kernel marks (only possible when
+  // the input is longer than the span) are rerun by the wide kernel
+  const int64_t span = g_narrow_span.load();
+  const bool fixup = (int64_t)n > span;
+#define GPJ_QLAUNCH(LV, NW, FIX)                                                          \
+  hipLaunchKernelGGL((gpj::query_batch_kernel<LV, NW>), dim3((uint32_t)blocks), dim3(64), 0, \
                      as_stream(stream), d_in, (int64_t)n, d_starts, d_num_starts,            \
                      max_records, index->d_string, index->d_leveled, index->level_size,      \
-                     index->num_levels, mode, batch)
-  if (deepest + 1 <= 4) GPJ_QLAUNCH(4);
-  else if (deepest + 1 <= 6) GPJ_QLAUNCH(6);
-  else if (deepest + 1 <= 8) GPJ_QLAUNCH(8);
-  else GPJ_QLAUNCH(16);
+                     index->num_levels, mode, gpj::PosMode{span, FIX}, batch)
+#define GPJ_QPAIR(LV)                                                                     \
+  do {                                                                                    \
+    GPJ_QLAUNCH(LV, true, 0);                                                             \
+    if (fixup) GPJ_QLAUNCH(LV, false, 1);                                                 \
+  } while (0)
+  if (deepest + 1 <= 4) GPJ_QPAIR(4);
+  else if (deepest + 1 <= 6) GPJ_QPAIR(6);
+  else if (deepest + 1 <= 8) GPJ_QPAIR(8);
+  else GPJ_QPAIR(16);
+#undef GPJ_QPAIR
 #undef GPJ_QLAUNCH
   GPJ_HIP(hipGetLastError(), "query_batch_kernel launch");
   return GPJ_OK;
 }
 
 } // namespace
+
+gpj_status gpj_debug_set_narrow_span(int64_t bytes) {
+  if (bytes == 0) bytes = gpj::kNarrowSpan;
+  if (bytes < 4096 || bytes > gpj::kNarrowSpan)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "narrow span: 0 or 4096 .. 2^31 - 2^20 bytes");
+  g_narrow_span.store(bytes);
+  return GPJ_OK;
+}
 
 gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                                const gpj_query_desc *queries, int32_t num_queries,
@@ -644,14 +667,23 @@ gpj_status gpj_batch_query(const uint8_t *d_in, uint64_t total_bytes,
   const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_records, 64), 1u << 22);
   const int64_t stride = (int64_t)div_up(total_bytes, 64);
   const int deepest = deepest_level(queries, num_queries);
-#define GPJ_FLAUNCH(LV)                                                                  \
-  hipLaunchKernelGGL(gpj::query_files_kernel<LV>, dim3((uint32_t)blocks), dim3(64), 0,    \
-                     as_stream(stream), d_in, d_newline, d_string, d_leveled, stride,    \
-                     levels, fm, batch)
-  if (deepest + 1 <= 4) GPJ_FLAUNCH(4);
-  else if (deepest + 1 <= 6) GPJ_FLAUNCH(6);
-  else if (deepest + 1 <= 8) GPJ_FLAUNCH(8);
-  else GPJ_FLAUNCH(16);
+  // every file is at most total_bytes long (see run_batch for the two kernels)
+  const int64_t span = g_narrow_span.load();
+  const bool fixup = (int64_t)total_bytes > span;
+#define GPJ_FLAUNCH(LV, NW, FIX)                                                          \
+  hipLaunchKernelGGL((gpj::query_files_kernel<LV, NW>), dim3((uint32_t)blocks), dim3(64), 0, \
+                     as_stream(stream), d_in, d_newline, d_string, d_leveled, stride,     \
+                     levels, fm, gpj::PosMode{span, FIX}, batch)
+#define GPJ_FPAIR(LV)                                                                     \
+  do {                                                                                    \
+    GPJ_FLAUNCH(LV, true, 0);                                                             \
+    if (fixup) GPJ_FLAUNCH(LV, false, 1);                                                 \
+  } while (0)
+  if (deepest + 1 <= 4) GPJ_FPAIR(4);
+  else if (deepest + 1 <= 6) GPJ_FPAIR(6);
+  else if (deepest + 1 <= 8) GPJ_FPAIR(8);
+  else GPJ_FPAIR(16);
+#undef GPJ_FPAIR
 #undef GPJ_FLAUNCH
   GPJ_HIP(hipGetLastError(), "query_files_kernel launch");
   return GPJ_OK;

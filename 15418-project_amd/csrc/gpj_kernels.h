@@ -77,12 +77,28 @@ struct RecordMode {
   int32_t drop_last;
 };
 
-template <int MAXLV>
+// How the query kernels hold positions.  The narrow kernels (NARROW = true)
+// keep 32-bit offsets from each record's 64-byte aligned start; a record
+// that reaches `span` bytes from that start, or a backward search that would
+// reach further than `span` bytes below it, is left to the wide kernel: the
+// narrow kernel stores kWideMark in the first entry of the batch's first
+// query (its count in variable mode, the start of its first span in fixed
+// mode) and the wide kernel (64-bit file offsets) launched with fixup = 1
+// reruns exactly the records so marked.  Neither can happen when the input
+// (or every file of a batch) is at most `span` bytes long.
+constexpr int64_t kNarrowSpan = (1ll << 31) - (1ll << 20);
+constexpr int64_t kWideMark = -2;
+struct PosMode {
+  int64_t span;
+  int32_t fixup;
+};
+
+template <int MAXLV, bool NARROW>
 __global__ void query_batch_kernel(const uint8_t *file, int64_t n, const int64_t *newline,
                                    const int64_t *num_lines, int64_t max_lines,
                                    const uint64_t *string, const uint64_t *leveled,
                                    int64_t level_size, int32_t num_levels, RecordMode mode,
-                                   QueryBatch batch);
+                                   PosMode pm, QueryBatch batch);
 
 // Multi-file mode of the query kernel (see BatchFile).
 struct FilesMode {
@@ -94,11 +110,12 @@ struct FilesMode {
   int64_t *counts[kBatchSlots];
 };
 
-template <int MAXLV>
+// Marks (PosMode) go to the record's count in result slot 0.
+template <int MAXLV, bool NARROW>
 __global__ void query_files_kernel(const uint8_t *batch_in, const int64_t *newline,
                                    const uint64_t *string, const uint64_t *leveled,
                                    int64_t level_stride, int32_t num_levels, FilesMode fm,
-                                   QueryBatch batch);
+                                   PosMode pm, QueryBatch batch);
 
 // Record bounds of a top-level array: positions of the set bits of one
 // bitmap row (count per 256-word tile, scan, write).

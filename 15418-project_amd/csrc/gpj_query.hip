@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "gpj_kernels.h"
 
 namespace gpj {
@@ -89,16 +91,14 @@ struct IrReader {
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 typedef int64_t i64x2a8 __attribute__((ext_vector_type(2), aligned(8)));
 
-// The input as aligned 16-byte granules.  Granule g holds file offsets
-// [16g - head, 16g - head + 16), where head is the input address modulo 16.
-// The first and last granules may hold bytes outside [0, n); they are loaded
-// like the others (an aligned granule never crosses a page, so its bytes are
-// readable wherever one of them is) and those bytes are cleared.
-struct Text {
-  const u64x2 *__restrict__ blocks;
-  int64_t head;
-  int64_t last; // index of the last granule that holds a byte of the input
-};
+// Positions.  The interpreter holds byte positions as P: int32_t offsets from
+// the record's 64-byte aligned start `base` in the narrow kernels, int64_t
+// file offsets (base 0) in the wide ones.  The caches are tagged with the low
+// 32 bits of file-absolute granule and word indices either way.
+constexpr int32_t kUnset = INT32_MIN;            // level end not known yet
+constexpr int32_t kNone = INT32_MIN + (1 << 19); // a backward search found nothing
+// (narrow positions stay within span + 4 KiB of 0, span <= kNarrowSpan, so
+// neither sentinel is ever a position, nor kNone minus a key length)
 
 __device__ __forceinline__ u64x2 clear_past(u64x2 v, int64_t valid) {
   // keep the first `valid` bytes (0..16) of a granule
@@ -110,58 +110,118 @@ __device__ __forceinline__ u64x2 clear_past(u64x2 v, int64_t valid) {
   return v;
 }
 
-__device__ __forceinline__ Text make_text(const uint8_t *__restrict__ file, int64_t n) {
-  Text t;
-  const int64_t head = (int64_t)(reinterpret_cast<uintptr_t>(file) & 15);
-  t.head = head;
-  t.blocks = reinterpret_cast<const u64x2 *>(file - head);
-  t.last = n > 0 ? (n - 1 + head) >> 4 : -1;
-  return t;
-}
-
-// Cache tags keep the low 32 bits of granule and word indices.  Word indices
-// are >= -1 and below 2^32 - 2 (the host keeps the leveled rows below that),
-// so the initial tag 2^32 - 2 never names a real word; granule indices of real
-// bytes stay below 2^32 - 2^27 (inputs below 2^35 bytes), and a negative
-// position that aliases a tag (the text pair's initial tags name granules -2
-// and -1) only ever meets zeros.
+// Cache tags keep the low 32 bits of file-absolute granule and word indices.
+// Word indices are >= -1 and below 2^32 - 2 (the host keeps the leveled rows
+// below that), so the initial tag 2^32 - 2 never names a real word; granule
+// indices of real bytes stay below 2^32 - 2^27 (inputs below 2^35 bytes), and
+// a negative position that aliases a tag (the text pair's initial tags name
+// granules -2 and -1) only ever meets zeros.
 constexpr uint32_t kNoTag = 0xFFFFFFFEu;
 
-// Per-lane view of the record's text and bitmaps, with caches for the last
+// Per-lane view of one record's text and bitmaps, with caches for the last
 // two consecutive text granules (filled together: a forward walk usually
 // moves on into the next granule) and for the last granule read from even and
-// from odd
-// leveled rows (a walk alternates between a level and the one below it, so
-// the two never evict each other).  The caches stay valid across the queries
-// of a batch, which run over the same record.  String-mask words are read
-// directly: a key lookup needs them only when the byte before its ':' is not
-// an unescaped quote.
-struct Ctx {
-  Text t;
+// from odd leveled rows (a walk alternates between a level and the one below
+// it, so the two never evict each other).  The caches stay valid across the
+// queries of a batch, which run over the same record.  String-mask words are
+// read directly: a key lookup needs them only when the byte before its ':' is
+// not an unescaped quote.
+//
+// Text granule G (file-absolute) holds file offsets [16G - head, 16G - head +
+// 16), where head is the input address modulo 16: granules 0 < G < last hold
+// input bytes only, granules below 0 or above last none (they read as 0), and
+// the two edge granules are loaded (an aligned granule never crosses a page)
+// with the bytes outside [0, n) cleared.  Position p lies in granule base / 16
+// + (p + head) / 16 and in bitmap word base / 64 + p / 64.  Only base (held
+// as gtag = base / 16, below 2^31 for inputs below 32 GiB) and the caches
+// differ between lanes.
+template <typename P> struct Ctx {
+  const u64x2 *__restrict__ blocks; // granule 0 of the file
+  const uint64_t *__restrict__ str;
+  const uint64_t *__restrict__ lev; // leveled row k starts at lev + k * level_size
   int64_t n;
-  const uint64_t *__restrict__ string;
-  const uint64_t *__restrict__ leveled;
   int64_t level_size;
-  int num_levels;
-  int base;        // leveled row of interpreter level 0
-  uint32_t text_at; // granules text_at, text_at + 1 held in text_val, text_nxt
+  int64_t span;     // positions reach at most span below 0 (see pos0)
+  uint32_t lpar;    // address parity of leveled word 0 (words are read in 16-byte granules)
+  uint32_t last;    // last granule holding an input byte (n > 0)
+  uint32_t inner;   // last - 1, or 0 when last is 0: interior granules G - 1 < inner
+  int32_t head;
+  int32_t num_levels;
+  int32_t lbase;    // leveled row of interpreter level 0
+  uint32_t gtag;    // base / 16, base = file offset of position 0 (a multiple of 64)
+  bool wide;        // a backward search reached pos0 while clamped: the record
+                    // goes to the wide kernel
+  uint32_t text_at; // granules text_at, text_at + 1 (absolute) in text_val, text_nxt
   u64x2 text_val;
   u64x2 text_nxt;
-  uint32_t row_at0; // first word index (low 32 bits) of the granule in row_val0
+  uint32_t row_at0; // absolute word index (low 32 bits) of the granule in row_val0
   u64x2 row_val0;   // (a granule of an even row)
   uint32_t row_at1; // same for odd rows
   u64x2 row_val1;
 };
 
+template <typename P>
+__device__ __forceinline__ Ctx<P> make_ctx(const uint8_t *file, int64_t n, const uint64_t *string,
+                                           const uint64_t *leveled, int64_t level_size,
+                                           int32_t num_levels, int32_t lbase, int64_t base,
+                                           int64_t span) {
+  Ctx<P> c;
+  const int64_t head = (int64_t)(reinterpret_cast<uintptr_t>(file) & 15);
+  const int64_t last = n > 0 ? (n - 1 + head) >> 4 : 0;
+  c.blocks = reinterpret_cast<const u64x2 *>(file - head);
+  c.str = string;
+  c.lev = leveled;
+  c.n = n;
+  c.level_size = level_size;
+  c.span = span;
+  c.lpar = (uint32_t)(reinterpret_cast<uintptr_t>(leveled) >> 3) & 1u;
+  c.last = (uint32_t)last;
+  c.inner = last > 0 ? (uint32_t)(last - 1) : 0u;
+  c.head = (int32_t)head;
+  c.num_levels = num_levels;
+  c.lbase = lbase;
+  c.gtag = (uint32_t)(base >> 4);
+  c.wide = false;
+  c.text_at = kNoTag;
+  c.text_val = u64x2{0, 0};
+  c.text_nxt = u64x2{0, 0};
+  c.row_at0 = kNoTag;
+  c.row_val0 = u64x2{0, 0};
+  c.row_at1 = kNoTag;
+  c.row_val1 = u64x2{0, 0};
+  return c;
+}
+
+template <typename P> __device__ __forceinline__ int64_t base_of(const Ctx<P> &c) {
+  return (int64_t)c.gtag << 4;
+}
+// Bitmap word of relative word 0 (base / 64).
+template <typename P> __device__ __forceinline__ int32_t bword(const Ctx<P> &c) {
+  return (int32_t)(c.gtag >> 2);
+}
+// Lowest position a backward search reaches: file offset 0, or -span when
+// that lies further down (clamped; narrow kernels only).
+template <typename P> __device__ __forceinline__ bool clamped(const Ctx<P> &c) {
+  return base_of(c) > c.span;
+}
+template <typename P> __device__ __forceinline__ P pos0(const Ctx<P> &c) {
+  return (P)(clamped(c) ? -c.span : -base_of(c));
+}
+// Text granule g (relative); its absolute index gtag + g lies in (-2^31, 2^31).
+template <typename P> __device__ __forceinline__ const u64x2 *granule_ptr(const Ctx<P> &c, P g) {
+  return c.blocks + (int64_t)(int32_t)(c.gtag + (uint32_t)g);
+}
+
 // Granule g of the input (granules outside the input read as 0).
-__device__ __forceinline__ u64x2 text_granule(const Ctx &c, int64_t g) {
-  if (g > 0 && g < c.t.last) return c.t.blocks[g]; // interior granule
-  if (g < 0 || g > c.t.last) return u64x2{0, 0};
+template <typename P> __device__ __forceinline__ u64x2 text_granule(const Ctx<P> &c, P g) {
+  const uint32_t ga = c.gtag + (uint32_t)g;         // absolute granule, |G| < 2^31
+  if (ga - 1u < c.inner) return *granule_ptr(c, g); // interior granule
+  if (ga > c.last || c.n <= 0) return u64x2{0, 0}; // G < 0 or G > last
   // the first or the last granule: clear the bytes before offset 0 and
   // from offset n on
-  const int64_t base = 16 * g - c.t.head; // file offset of the granule's byte 0
-  u64x2 v = clear_past(c.t.blocks[g], c.n - base);
-  const int64_t lo = -base; // bytes before offset 0
+  const int64_t fo = 16 * (int64_t)(int32_t)ga - c.head; // file offset of byte 0
+  u64x2 v = clear_past(*granule_ptr(c, g), c.n - fo);
+  const int64_t lo = -fo; // bytes before offset 0
   if (lo > 0) {
     v.x &= lo >= 8 ? 0ull : (~0ull << (8 * lo));
     v.y &= lo >= 16 ? 0ull : (lo <= 8 ? ~0ull : (~0ull << (8 * (lo - 8))));
@@ -170,19 +230,21 @@ __device__ __forceinline__ u64x2 text_granule(const Ctx &c, int64_t g) {
 }
 
 // Fills the lane's granule pair with granules g and g + 1 (one round trip).
-__device__ __forceinline__ void text_fill(Ctx &c, int64_t g) {
-  if (g > 0 && g + 1 < c.t.last) {
-    c.text_val = c.t.blocks[g];
-    c.text_nxt = c.t.blocks[g + 1];
+template <typename P> __device__ __forceinline__ void text_fill(Ctx<P> &c, P g) {
+  const uint32_t ga = c.gtag + (uint32_t)g;
+  if (ga - 1u < c.inner && ga < c.inner) { // both interior
+    const u64x2 *q = granule_ptr(c, g);
+    c.text_val = q[0];
+    c.text_nxt = q[1];
   } else {
     c.text_val = text_granule(c, g);
-    c.text_nxt = text_granule(c, g + 1);
+    c.text_nxt = text_granule(c, (P)(g + 1));
   }
-  c.text_at = (uint32_t)g;
+  c.text_at = ga;
 }
 // Granule g from the lane's pair, filling the pair from g on a miss.
-__device__ __forceinline__ u64x2 text_get(Ctx &c, int64_t g) {
-  const uint32_t d = (uint32_t)g - c.text_at;
+template <typename P> __device__ __forceinline__ u64x2 text_get(Ctx<P> &c, P g) {
+  const uint32_t d = c.gtag + (uint32_t)g - c.text_at;
   if (d > 1u) {
     text_fill(c, g);
     return c.text_val;
@@ -190,137 +252,149 @@ __device__ __forceinline__ u64x2 text_get(Ctx &c, int64_t g) {
   return d ? c.text_nxt : c.text_val;
 }
 
-// Byte p of the input; bytes outside [0, n) read as 0 (the first and last
-// granules are stored with those bytes cleared, granules beyond them read
-// as 0).
-__device__ __forceinline__ uint8_t byte_at(Ctx &c, int64_t p) {
-  const int64_t a = p + c.t.head;
-  const u64x2 v = text_get(c, a >> 4);
+// Byte at position p; bytes outside [0, n) read as 0.
+template <typename P> __device__ __forceinline__ uint8_t byte_at(Ctx<P> &c, P p) {
+  const P a = p + c.head;
+  const u64x2 v = text_get(c, (P)(a >> 4));
   const uint64_t w = (a & 8) ? v.y : v.x;
   return (uint8_t)(w >> (8 * (a & 7)));
 }
 
-__device__ __forceinline__ uint64_t text8(Ctx &c, int64_t p) {
-  const int64_t a = p + c.t.head;
-  const int64_t g = a >> 4;
-  const int64_t o = a & 15;
+// Little-endian 8 bytes from position p.
+template <typename P> __device__ __forceinline__ uint64_t text8(Ctx<P> &c, P p) {
+  const P a = p + c.head;
+  const P g = a >> 4;
+  const int32_t o = (int32_t)(a & 15);
   const u64x2 v0 = text_get(c, g);
   if (o == 0) return v0.x;
   if (o == 8) return v0.y;
   const uint32_t sh = 8 * (uint32_t)(o & 7);
   if (o < 8) return (v0.x >> sh) | (v0.y << (64 - sh));
-  const u64x2 v1 = text_get(c, g + 1);
+  const u64x2 v1 = text_get(c, (P)(g + 1));
   return (v0.y >> sh) | (v1.x << (64 - sh));
 }
 
-// Word wi of an 8-byte-aligned bitmap, read as part of the 16-byte aligned
-// granule that holds it (the granule's other word may lie just outside the
-// bitmap, inside the same aligned 16 bytes); the lane keeps one granule per
-// bitmap.
-__device__ __forceinline__ uint64_t granule_word(const uint64_t *__restrict__ base, int64_t wi,
+// A leveled row as the searches read it: the index, counted from the start
+// of the leveled rows, of the row's word that holds position 0, which row
+// cache it uses (row parity), and valid = false for levels outside the built
+// rows, which read as empty.
+struct RowRef {
+  int64_t w0;
+  bool odd;
+  bool valid;
+};
+template <typename P> __device__ __forceinline__ RowRef row_ref(const Ctx<P> &c, int level) {
+  RowRef r;
+  const int row = level + c.lbase;
+  r.valid = level >= 0 && row < c.num_levels;
+  r.w0 = c.level_size * (int64_t)row + bword(c);
+  r.odd = (row & 1) != 0;
+  return r;
+}
+
+// Word w of row r, read as part of its aligned 16-byte granule (the
+// granule's other word may lie just outside the bitmap, inside the same
+// aligned 16 bytes); the lane keeps one granule per row parity, tagged with
+// the low 32 bits of the granule's word index.
+template <typename P>
+__device__ __forceinline__ uint64_t granule_word(const Ctx<P> &c, const RowRef &r, P w,
                                                  uint32_t &at, u64x2 &val) {
-  const int64_t odd = (int64_t)((reinterpret_cast<uintptr_t>(base + wi) >> 3) & 1);
-  const int64_t g = wi - odd;
+  const int64_t wi = r.w0 + (int64_t)w;
+  const uint32_t hi = ((uint32_t)wi ^ c.lpar) & 1u; // wi is its granule's second word
+  const int64_t g = wi - (int64_t)hi;
   if ((uint32_t)g != at) {
-    val = *reinterpret_cast<const u64x2 *>(base + g);
+    val = *reinterpret_cast<const u64x2 *>(c.lev + g);
     at = (uint32_t)g;
   }
-  return odd ? val.y : val.x;
+  return hi ? val.y : val.x;
 }
-// Word wi of the leveled rows (row-major, so wi includes the row offset);
-// `odd` is the parity of the row.
-__device__ __forceinline__ uint64_t row_word(Ctx &c, int64_t wi, bool odd) {
-  return odd ? granule_word(c.leveled, wi, c.row_at1, c.row_val1)
-             : granule_word(c.leveled, wi, c.row_at0, c.row_val0);
+template <typename P> __device__ __forceinline__ uint64_t row_word(Ctx<P> &c, const RowRef &r, P w) {
+  return r.odd ? granule_word(c, r, w, c.row_at1, c.row_val1)
+               : granule_word(c, r, w, c.row_at0, c.row_val0);
 }
-__device__ __forceinline__ uint64_t str_word(const Ctx &c, int64_t wi) { return c.string[wi]; }
+template <typename P> __device__ __forceinline__ uint64_t str_word(const Ctx<P> &c, P w) {
+  return c.str[(int64_t)(bword(c) + (int32_t)w)];
+}
 
 __device__ __forceinline__ uint64_t through(int b) {
   return b >= 63 ? ~0ull : ((1ull << (b + 1)) - 1);
 }
 
-// Last set bit of the string mask in [lo, idx], or -1.
-__device__ __forceinline__ int64_t last_string_bit(const Ctx &c, int64_t lo, int64_t idx) {
-  if (idx < lo || idx < 0) return -1;
-  int64_t wi = idx >> 6, wb = wi << 6;
+// Last set bit of the string mask in [lo, idx], or kNone.
+// (lo >= pos0 in every call, so the search never leaves the file.)
+template <typename P> __device__ __forceinline__ P last_string_bit(const Ctx<P> &c, P lo, P idx) {
+  if (idx < lo) return (P)kNone;
+  P wi = idx >> 6, wb = wi * 64;
   uint64_t word = str_word(c, wi) & through((int)(idx - wb));
   for (;;) {
     uint64_t cand = word;
-    const int64_t first = lo - wb;
+    const P first = lo - wb;
     if (first > 0) cand &= ~through((int)(first - 1));
-    if (cand) return wb + 63 - __builtin_clzll(cand);
-    if (wi == 0 || wb - 1 < lo) return -1;
+    if (cand) return wb + 63 - (P)__builtin_clzll(cand);
+    if (bword(c) + (int32_t)wi <= 0 || wb - 1 < lo) return (P)kNone;
     --wi;
     wb -= 64;
     word = str_word(c, wi);
   }
 }
 
-// Last set bit of the leveled row starting at word `row` in [lo, idx], or -1.
-__device__ __forceinline__ int64_t last_row_bit(Ctx &c, int64_t row, bool odd, int64_t lo,
-                                                int64_t idx) {
-  if (idx < lo || idx < 0) return -1;
-  int64_t wi = idx >> 6, wb = wi << 6;
-  uint64_t word = row_word(c, row + wi, odd) & through((int)(idx - wb));
+// Last set bit of row r in [lo, idx], or kNone.
+template <typename P>
+__device__ __forceinline__ P last_row_bit(Ctx<P> &c, const RowRef &r, P lo, P idx) {
+  if (idx < lo) return (P)kNone;
+  P wi = idx >> 6, wb = wi * 64;
+  uint64_t word = row_word(c, r, wi) & through((int)(idx - wb));
   for (;;) {
     uint64_t cand = word;
-    const int64_t first = lo - wb;
+    const P first = lo - wb;
     if (first > 0) cand &= ~through((int)(first - 1));
-    if (cand) return wb + 63 - __builtin_clzll(cand);
-    if (wi == 0 || wb - 1 < lo) return -1;
+    if (cand) return wb + 63 - (P)__builtin_clzll(cand);
+    if (bword(c) + (int32_t)wi <= 0 || wb - 1 < lo) return (P)kNone;
     --wi;
     wb -= 64;
-    word = row_word(c, row + wi, odd);
+    word = row_word(c, r, wi);
   }
-}
-
-// A leveled row as next_structural reads it: the row's first word index and
-// its parity (which row cache it uses); valid = false for levels outside the
-// built rows, which read as empty.
-struct RowRef {
-  int64_t row;
-  bool odd;
-  bool valid;
-};
-__device__ __forceinline__ RowRef row_ref(const Ctx &c, int level) {
-  RowRef r;
-  r.valid = level >= 0 && level + c.base < c.num_levels;
-  r.row = c.level_size * (int64_t)(level + c.base);
-  r.odd = ((level + c.base) & 1) != 0;
-  return r;
 }
 
 // First structural bit of row r in [idx, level_end], else level_end
-// (find_next_structural_char, query.cu:79-115).
-__device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, int64_t idx,
-                                                   const RowRef &r) {
+// (find_next_structural_char, query.cu:79-115).  Every call passes a
+// position at or above file offset 0 (one past a position >= -1), so the
+// reference's clamp of idx to 0 never applies.
+template <typename P>
+__device__ __forceinline__ P next_structural(Ctx<P> &c, P level_end, P idx, const RowRef &r) {
   if (idx >= level_end) return level_end;
   if (!r.valid) return level_end;
-  if (idx < 0) idx = 0;
-  int64_t wi = idx >> 6, wb = wi << 6;
-  uint64_t word = row_word(c, r.row + wi, r.odd) & (~0ull << (idx - wb));
+  P wi = idx >> 6, wb = wi * 64;
+  uint64_t word = row_word(c, r, wi) & (~0ull << (idx - wb));
   for (;;) {
     uint64_t cand = word;
-    const int64_t last = level_end - wb;
+    const P last = level_end - wb;
     if (last < 63) cand &= through((int)last);
-    if (cand) return wb + __builtin_ctzll(cand);
+    if (cand) return wb + (P)__builtin_ctzll(cand);
     ++wi;
     wb += 64;
-    if (wb > level_end || wi >= c.level_size) return level_end;
-    word = row_word(c, r.row + wi, r.odd);
+    if (wb > level_end || (int64_t)(bword(c) + (int32_t)wi) >= c.level_size) return level_end;
+    word = row_word(c, r, wi);
   }
 }
-__device__ __forceinline__ int64_t next_structural(Ctx &c, int64_t level_end, int64_t idx, int level) {
+template <typename P>
+__device__ __forceinline__ P next_structural(Ctx<P> &c, P level_end, P idx, int level) {
   return next_structural(c, level_end, idx, row_ref(c, level));
 }
 
-// find_previous_structural_char, query.cu:117-128
-__device__ __forceinline__ int64_t prev_structural(Ctx &c, int64_t level_start, int64_t idx, int level) {
-  if (idx <= level_start) return level_start;
-  if (level < 0 || level + c.base >= c.num_levels) return level_start;
-  const int64_t row = c.level_size * (int64_t)(level + c.base);
-  const int64_t s = last_row_bit(c, row, ((level + c.base) & 1) != 0, level_start, idx);
-  return s == -1 ? level_start : s;
+// Last structural bit of the level's row in [file offset 0, idx], else file
+// offset 0 (find_previous_structural_char, query.cu:117-128, always called
+// with level start 0).  When file offset 0 lies below pos0 and the answer
+// is pos0, the record goes to the wide kernel.
+template <typename P> __device__ __forceinline__ P prev_structural(Ctx<P> &c, P idx, int level) {
+  const P lo = pos0(c);
+  P out = lo;
+  if (idx > lo && level >= 0 && level + c.lbase < c.num_levels) {
+    const P s = last_row_bit(c, row_ref(c, level), lo, idx);
+    if (s != (P)kNone) out = s;
+  }
+  if (clamped(c) && out == lo) c.wide = true;
+  return out;
 }
 
 // Per-lane array of N entries kept in LDS as [N][64] (one column per lane of
@@ -354,11 +428,11 @@ template <typename T, int N> struct LdsArray {
 };
 
 // LDS home of the per-lane arrays of one wave.
-template <int MAXLV> struct WaveArrays {
+template <int MAXLV, typename P> struct WaveArrays {
   static constexpr int kF = MAXLV < kMaxFrames ? MAXLV : kMaxFrames;
-  int64_t level_end[MAXLV][64];
-  int64_t marked_pos[MAXLV][64];
-  int64_t fr_sep[kF][64];
+  P level_end[MAXLV][64];
+  P marked_pos[MAXLV][64];
+  P fr_sep[kF][64];
   int32_t curr_index[MAXLV][64];
   uint32_t fr_pack[kF][64]; // qp | lvl << 16 | (mi + 1) << 21
   uint8_t marked_lvl[MAXLV][64];
@@ -377,19 +451,19 @@ __device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
 // registers, the arrays in the wave's LDS block.
 // MAXLV bounds the level (the host derives it from the bytecode); levels,
 // marks and element iterations each use at most MAXLV entries.
-template <int MAXLV> struct LaneState {
-  static constexpr int kF = WaveArrays<MAXLV>::kF;
-  int64_t idx;
+template <int MAXLV, typename P> struct LaneState {
+  static constexpr int kF = WaveArrays<MAXLV, P>::kF;
+  P idx;
   int32_t lvl, qp, mi, nfr;
-  int64_t count; // results stored (fixed) or found (variable)
-  LdsArray<int64_t, MAXLV> level_end;
+  P count; // results stored (fixed) or found (variable); at most one per byte
+  LdsArray<P, MAXLV> level_end;
   LdsArray<int32_t, MAXLV> curr_index;
-  LdsArray<int64_t, MAXLV> marked_pos;
+  LdsArray<P, MAXLV> marked_pos;
   LdsArray<uint8_t, MAXLV> marked_lvl;
   LdsArray<uint32_t, kF> fr_pack; // bytecode position (after MOVE_TO_EACH, so
                                   // <= 4097), level and mark depth + 1
-  LdsArray<int64_t, kF> fr_sep;
-  __device__ __forceinline__ LaneState(WaveArrays<MAXLV> &w, int lane)
+  LdsArray<P, kF> fr_sep;
+  __device__ __forceinline__ LaneState(WaveArrays<MAXLV, P> &w, int lane)
       : level_end{w.level_end, lane}, curr_index{w.curr_index, lane},
         marked_pos{w.marked_pos, lane}, marked_lvl{w.marked_lvl, lane},
         fr_pack{w.fr_pack, lane}, fr_sep{w.fr_sep, lane} {}
@@ -397,24 +471,25 @@ template <int MAXLV> struct LaneState {
 
 // RESET_POSITION: back to the innermost mark, clearing the levels above it up
 // to the first one already clear.  False: no mark (the path fails).
-template <int MAXLV>
-__device__ __forceinline__ bool reset_position(LaneState<MAXLV> &s) {
+template <int MAXLV, typename P>
+__device__ __forceinline__ bool reset_position(LaneState<MAXLV, P> &s) {
   if (s.mi < 0) return false;
   s.idx = s.marked_pos.get(s.mi);
   s.lvl = s.marked_lvl.get(s.mi);
   --s.mi;
-  s.level_end.fill_above_until(s.lvl, -1);
+  s.level_end.fill_above_until(s.lvl, (P)kUnset);
   return true;
 }
 
 // MOVE_DOWN: one level deeper; a level entered for the first time ends at
 // the next structural character of the level above.  False: the path fails.
-template <int MAXLV>
-__device__ __forceinline__ bool move_down(Ctx &c, LaneState<MAXLV> &s) {
+template <int MAXLV, typename P>
+__device__ __forceinline__ bool move_down(Ctx<P> &c, LaneState<MAXLV, P> &s) {
   if (s.lvl + 1 >= MAXLV) return false;
   ++s.lvl;
-  if (s.level_end.get(s.lvl) == -1) {
-    s.level_end.set(s.lvl, next_structural(c, s.level_end.get(s.lvl - 1), s.idx + 1, s.lvl - 1));
+  if (s.level_end.get(s.lvl) == (P)kUnset) {
+    s.level_end.set(s.lvl, next_structural(c, s.level_end.get(s.lvl - 1), (P)(s.idx + 1),
+                                           s.lvl - 1));
     while (byte_at(c, s.idx) == ' ') ++s.idx;
   }
   return true;
@@ -423,19 +498,19 @@ __device__ __forceinline__ bool move_down(Ctx &c, LaneState<MAXLV> &s) {
 // A finished path (END or any failure): resume the innermost element
 // iteration at its next element, or end the record.  Returns false when the
 // record is done.
-template <int MAXLV>
-__device__ __forceinline__ bool resume(Ctx &c, LaneState<MAXLV> &s) {
+template <int MAXLV, typename P>
+__device__ __forceinline__ bool resume(Ctx<P> &c, LaneState<MAXLV, P> &s) {
   for (;;) {
     if (s.nfr == 0) return false;
     const int f = s.nfr - 1;
     const uint32_t pack = s.fr_pack.get(f);
     const int flvl = (int)((pack >> 16) & 31u);
-    const int64_t lend = s.level_end.get(flvl);
-    const int64_t sep = next_structural(c, lend, s.fr_sep.get(f) + 1, flvl);
+    const P lend = s.level_end.get(flvl);
+    const P sep = next_structural(c, lend, (P)(s.fr_sep.get(f) + 1), flvl);
     if (byte_at(c, sep) == ',' && sep < lend) {
       s.fr_sep.set(f, sep);
       s.lvl = flvl;
-      s.level_end.fill_above(s.lvl, -1);
+      s.level_end.fill_above(s.lvl, (P)kUnset);
       s.mi = (int32_t)(pack >> 21) - 1;
       s.idx = sep + 1;
       s.qp = (int32_t)(pack & 0xFFFFu);
@@ -449,21 +524,21 @@ __device__ __forceinline__ bool resume(Ctx &c, LaneState<MAXLV> &s) {
 // record (trimmed to line_start < line_end).  Fixed mode (reference): STORE i
 // writes slot i; any failure ends the record.  Variable mode (extension):
 // STORE appends (first `cap` stored, all counted) and a finished path
-// resumes the innermost MOVE_TO_EACH at its next element.  Returns the
-// lane's result count.
-template <int MAXLV>
-__device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool live,
-                                             int64_t line_start, int64_t line_end,
-                                             const IrReader ir, int32_t cap, bool variable,
-                                             int64_t *res) {
-  LaneState<MAXLV> s(w, (int)(threadIdx.x & 63));
+// resumes the innermost MOVE_TO_EACH at its next element.  A lane whose
+// record needs the wide kernel (c.wide) stops.  Returns the lane's result
+// count.  Stored spans are file offsets (base + position).
+template <int MAXLV, typename P>
+__device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w, bool live,
+                                             P line_start, P line_end, const IrReader ir,
+                                             int32_t cap, bool variable, int64_t *res) {
+  LaneState<MAXLV, P> s(w, (int)(threadIdx.x & 63));
   s.idx = line_start;
   s.lvl = 0;
   s.qp = 0;
   s.mi = -1;
   s.nfr = 0;
   s.count = 0;
-  s.level_end.fill(-1);
+  s.level_end.fill((P)kUnset);
   s.level_end.set(0, line_end);
   s.curr_index.fill(-1);
 
@@ -508,12 +583,13 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
     switch (op) {
     case kStoreResult: {
       if (!variable && s.count >= cap) { fail = true; break; }
-      const int64_t lend = s.level_end.get(s.lvl);
-      int64_t end_str = lend;
+      const P lend = s.level_end.get(s.lvl);
+      P end_str = lend;
       while (byte_at(c, s.idx) == ' ' && s.idx < lend) ++s.idx;
-      while (end_str > s.idx && byte_at(c, end_str - 1) == ' ') --end_str;
+      while (end_str > s.idx && byte_at(c, (P)(end_str - 1)) == ' ') --end_str;
       if (s.count < cap) // one 16-byte store (the span array is 8-byte aligned)
-        *reinterpret_cast<i64x2a8 *>(res + s.count * 2) = i64x2a8{s.idx, end_str};
+        *reinterpret_cast<i64x2a8 *>(res + (int64_t)s.count * 2) =
+            i64x2a8{base_of(c) + (int64_t)s.idx, base_of(c) + (int64_t)end_str};
       ++s.count;
       if (then_end) fail = true; // END: the path is finished
       break;
@@ -521,7 +597,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
     case kMoveUp:
       if (s.lvl <= 0) { fail = true; break; }
       s.idx = s.level_end.get(s.lvl);
-      s.level_end.set(s.lvl, -1);
+      s.level_end.set(s.lvl, (P)kUnset);
       --s.lvl;
       break;
     case kMoveDown:
@@ -529,7 +605,7 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       break;
     case kMoveToKey: {
       if (arg < 0 || byte_at(c, s.idx) != '{') { fail = true; break; }
-      const int64_t lend = s.level_end.get(s.lvl);
+      const P lend = s.level_end.get(s.lvl);
       const uint64_t key0 = ir.u64(key_pos);
       const uint64_t mask0 = arg >= 8 ? ~0ull : ((1ull << (8 * arg)) - 1);
       // A key of at most 5 bytes that does not end in a backslash: the 8
@@ -543,8 +619,8 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
                                (0x22ull << 48) | (0x3Aull << 56);
       const uint64_t pmask = ~0ull << (8 * jss);
       const RowRef rr = row_ref(c, s.lvl);
-      int64_t idx = next_structural(c, lend, s.idx + 1, rr);
-      uint64_t w = text8(c, idx - 7); // bytes idx - 7 .. idx
+      P idx = next_structural(c, lend, (P)(s.idx + 1), rr);
+      uint64_t w = text8(c, (P)(idx - 7)); // bytes idx - 7 .. idx
       for (;;) {
         if ((w >> 56) != ':') break;
         // idx is outside strings, so an unescaped quote right before it closes
@@ -557,25 +633,26 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
         if (compact && short_key) {
           match = idx - 2 - arg >= line_start && ((w ^ pattern) & pmask) == 0;
         } else {
-          const int64_t string_end = compact ? idx - 2 : last_string_bit(c, line_start + 1, idx - 1);
-          const int64_t ss = string_end - arg;
+          const P string_end =
+              compact ? (P)(idx - 2) : last_string_bit(c, (P)(line_start + 1), (P)(idx - 1));
+          const P ss = string_end - arg;
           match = ss >= line_start && byte_at(c, ss) == '"';
           if (match && arg > 0) {
-            match = ((text8(c, ss + 1) ^ key0) & mask0) == 0;
+            match = ((text8(c, (P)(ss + 1)) ^ key0) & mask0) == 0;
             for (int32_t k = 8; match && k < arg; k += 8) {
               const int32_t rest = arg - k;
               const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
-              match = ((text8(c, ss + 1 + k) ^ ir.u64(key_pos + (uint32_t)k)) & m) == 0;
+              match = ((text8(c, (P)(ss + 1 + k)) ^ ir.u64(key_pos + (uint32_t)k)) & m) == 0;
             }
           }
         }
         if (match) break;
         // the separator after the value ends the object if it is '}'; the
         // next key's 8-byte window usually holds it as well
-        const int64_t sep = next_structural(c, lend, idx + 1, rr);
-        const int64_t nxt = next_structural(c, lend, sep + 1, rr);
-        const uint64_t wn = text8(c, nxt - 7);
-        const int64_t off = sep - (nxt - 7);
+        const P sep = next_structural(c, lend, (P)(idx + 1), rr);
+        const P nxt = next_structural(c, lend, (P)(sep + 1), rr);
+        const uint64_t wn = text8(c, (P)(nxt - 7));
+        const P off = sep - (nxt - 7);
         const uint32_t sep_byte =
             (off >= 0 && off < 8) ? (uint32_t)(wn >> (8 * off)) & 0xFFu : (uint32_t)byte_at(c, sep);
         if (sep_byte == '}') {
@@ -594,23 +671,25 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       const int32_t target = arg;
       const uint8_t ch = byte_at(c, s.idx);
       if (ch != '[' && ch != ',' && ch != ']') { fail = true; break; }
-      const int64_t lend = s.level_end.get(s.lvl);
+      const P lend = s.level_end.get(s.lvl);
       int32_t ci = (ch == '[') ? 0 : s.curr_index.get(s.lvl) + 1;
-      int64_t idx = s.idx;
+      P idx = s.idx;
       for (;;) {
         if (ci < target) {
-          idx = next_structural(c, lend, idx + 1, s.lvl);
+          idx = next_structural(c, lend, (P)(idx + 1), s.lvl);
           if (byte_at(c, idx) == ',') { ++ci; continue; }
           fail = true;
           break;
         }
         if (ci > target) {
-          idx = prev_structural(c, 0, idx - 1, s.lvl);
+          idx = prev_structural(c, (P)(idx - 1), s.lvl);
+          if (c.wide) break;
           --ci;
           continue;
         }
         break;
       }
+      if (c.wide) break;
       s.curr_index.set(s.lvl, ci);
       if (!fail && byte_at(c, idx) == ']') fail = true;
       s.idx = idx + 1;
@@ -619,26 +698,30 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
     }
     case kMoveToIndexReverse: {
       const int32_t target = arg;
-      const int64_t lend = s.level_end.get(s.lvl);
-      int64_t idx = lend - 1;
-      while (idx >= 0 && byte_at(c, idx) == ' ') --idx;
+      const P lend = s.level_end.get(s.lvl);
+      P idx = lend - 1;
+      const P lo = pos0(c);
+      while (idx >= lo && byte_at(c, idx) == ' ') --idx;
+      if (idx < lo && clamped(c)) { c.wide = true; break; }
       const uint8_t ch = byte_at(c, idx);
       if (ch != ']' && ch != ',' && ch != '[') { fail = true; break; }
       int32_t ci = (ch == ']') ? 0 : s.curr_index.get(s.lvl) + 1;
       for (;;) {
         if (ci < target) {
-          idx = prev_structural(c, 0, idx - 1, s.lvl);
+          idx = prev_structural(c, (P)(idx - 1), s.lvl);
+          if (c.wide) break;
           if (byte_at(c, idx) == ',' || ci + 1 == target) { ++ci; continue; }
           fail = true;
           break;
         }
         if (ci > target) {
-          idx = next_structural(c, lend, idx + 1, s.lvl);
+          idx = next_structural(c, lend, (P)(idx + 1), s.lvl);
           --ci;
           continue;
         }
         break;
       }
+      if (c.wide) break;
       s.curr_index.set(s.lvl, ci);
       s.idx = idx + 1;
       if (!fail && then_down) fail = !move_down(c, s);
@@ -655,26 +738,29 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       break;
     case kExpressionStringEquals: {
       if (arg < 0) { fail = true; break; }
-      const int64_t lend = s.level_end.get(s.lvl);
+      const P lend = s.level_end.get(s.lvl);
       while (byte_at(c, s.idx) == ' ' && s.idx < lend) ++s.idx;
-      int64_t string_end = lend - 1;
+      P string_end = lend - 1;
       while (s.idx < string_end && byte_at(c, string_end) == ' ') --string_end;
-      if (string_end - s.idx + 1 != arg) { fail = true; break; }
+      if (string_end - s.idx + 1 != (P)arg) { fail = true; break; }
       for (int32_t k = 0; k < arg; k += 8) {
         const int32_t rest = arg - k;
         const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
-        if (((text8(c, s.idx + k) ^ ir.u64(key_pos + (uint32_t)k)) & m) != 0) { fail = true; break; }
+        if (((text8(c, (P)(s.idx + k)) ^ ir.u64(key_pos + (uint32_t)k)) & m) != 0) {
+          fail = true;
+          break;
+        }
       }
       if (!fail && then_reset) fail = !reset_position(s);
       break;
     }
     case kMoveToEach: {
-      if (!variable || byte_at(c, s.idx) != '[' || s.nfr >= LaneState<MAXLV>::kF) {
+      if (!variable || byte_at(c, s.idx) != '[' || s.nfr >= LaneState<MAXLV, P>::kF) {
         fail = true;
         break;
       }
-      const int64_t first = next_structural(c, s.level_end.get(s.lvl), s.idx + 1, s.lvl);
-      int64_t p = s.idx + 1;
+      const P first = next_structural(c, s.level_end.get(s.lvl), (P)(s.idx + 1), s.lvl);
+      P p = s.idx + 1;
       while (p < first && byte_at(c, p) == ' ') ++p;
       if (p == first && byte_at(c, first) == ']') { fail = true; break; } // empty array
       // the frame resumes at the instruction after MOVE_TO_EACH (a fused
@@ -695,9 +781,24 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
       fail = true;
       break;
     }
-    if (fail) live = resume<MAXLV>(c, s);
+    if (c.wide) live = false; // the wide kernel redoes this record
+    else if (fail) live = resume<MAXLV>(c, s);
   }
-  return s.count;
+  return (int64_t)s.count;
+}
+
+// The record's bounds trimmed to its first '{' and last '}' (query.cu:160-172);
+// false when that leaves nothing.
+template <typename P> __device__ __forceinline__ bool trim_record(Ctx<P> &c, P &start, P &end) {
+  while (end > start && byte_at(c, end) != '}') --end;
+  while (start < end && byte_at(c, start) != '{') ++start;
+  return start != end;
+}
+
+// Where a batch's marks live: the count of the first query in variable mode,
+// the start of its first span in fixed mode.
+__device__ __forceinline__ int64_t *mark_slot(const QuerySlot &q0, int64_t line) {
+  return q0.variable ? q0.counts + line : q0.spans + line * 2 * (int64_t)q0.cap;
 }
 
 } // namespace
@@ -711,39 +812,43 @@ __device__ __forceinline__ int64_t run_query(Ctx &c, WaveArrays<MAXLV> &w, bool 
 // one runs to the end of the input) or, in top-level-array mode, the spans
 // between consecutive leveled-row-0 characters, with interpreter level k on
 // leveled row k + 1 (mode.level_base = 1, mode.drop_last = 1: the last bound
-// opens no record).
-template <int MAXLV>
+// opens no record).  Positions as PosMode describes.
+template <int MAXLV, bool NARROW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ? 4 : 1))) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
     const int64_t *__restrict__ newline, const int64_t *__restrict__ num_lines,
     int64_t max_lines, const uint64_t *__restrict__ string,
     const uint64_t *__restrict__ leveled, int64_t level_size, int32_t num_levels,
-    RecordMode mode, QueryBatch batch) {
+    RecordMode mode, PosMode pm, QueryBatch batch) {
+  typedef typename std::conditional<NARROW, int32_t, int64_t>::type P;
   const int64_t all = *num_lines;
   const int64_t avail = all - mode.drop_last;
   const int64_t lines = avail < max_lines ? avail : max_lines;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t wave_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
-  const Text text = make_text(file, n);
-  __shared__ WaveArrays<MAXLV> arrays;
-  for (int64_t base = wave_base; base < lines; base += stride) {
-    const int64_t line = base + (threadIdx.x & 63);
-    const bool valid = line < lines;
-    Ctx c{text, n, string, leveled, level_size, num_levels, mode.level_base, kNoTag,
-          u64x2{0, 0}, u64x2{0, 0}, kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}};
-    int64_t line_start = 0, line_end = 0;
-    bool record = false;
+  const int64_t span = NARROW ? pm.span : INT64_MAX;
+  __shared__ WaveArrays<MAXLV, P> arrays;
+  for (int64_t wb = wave_base; wb < lines; wb += stride) {
+    const int64_t line = wb + (threadIdx.x & 63);
+    bool valid = line < lines;
+    if (!NARROW && pm.fixup && valid) valid = *mark_slot(batch.q[0], line) == kWideMark;
+    int64_t ls = 0, le = 0;
+    bool record = false, mark = false;
     if (valid) {
-      line_start = newline[line];
-      line_end = (line + 1 < all) ? newline[line + 1] : n - 1;
-      record = line_start >= 0 && line_start < n && line_end >= 0;
-      if (record) {
-        if (line_end >= n) line_end = n - 1;
-        while (line_end > line_start && byte_at(c, line_end) != '}') --line_end;
-        while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
-        record = line_start != line_end;
-      }
+      ls = newline[line];
+      le = (line + 1 < all) ? newline[line + 1] : n - 1;
+      record = ls >= 0 && ls < n && le >= 0;
+      if (record && le >= n) le = n - 1;
     }
+    const int64_t base = (NARROW && record) ? (ls & ~63ll) : 0;
+    if (NARROW && record && le - base >= span) {
+      record = false;
+      mark = true;
+    }
+    Ctx<P> c = make_ctx<P>(file, n, string, leveled, level_size, num_levels, mode.level_base,
+                           base, span);
+    P line_start = (P)(ls - base), line_end = (P)(le - base);
+    if (record) record = trim_record(c, line_start, line_end);
     for (int q = 0; q < batch.count; ++q) {
       const QuerySlot &qs = batch.q[q];
       int64_t *res = qs.spans + line * 2 * (int64_t)qs.cap;
@@ -751,10 +856,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
         for (int i = 0; i < 2 * qs.cap; ++i) res[i] = -1;
       const IrReader ir = IrReader::make(qs.ir, qs.ir_len);
       const int64_t found =
-          run_query<MAXLV>(c, arrays, record, line_start, line_end, ir, qs.cap,
-                           qs.variable != 0, res);
+          run_query<MAXLV, P>(c, arrays, record && !c.wide, line_start, line_end, ir, qs.cap,
+                              qs.variable != 0, res);
       if (valid && qs.counts) qs.counts[line] = found;
     }
+    if (NARROW && (mark || c.wide)) *mark_slot(batch.q[0], line) = kWideMark;
   }
 }
 
@@ -763,19 +869,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
 // 1]; each file runs the batch queries its query_mask selects, its k-th one
 // writing result slot k ([total lines][cap][2] spans, [total lines] counts;
 // a slot no query of the file uses gets count 0).  Per file, the results are
-// those of a single-file launch over that file alone.
-template <int MAXLV>
+// those of a single-file launch over that file alone.  Positions are
+// file-relative, as PosMode describes; marks go to the count of slot 0.
+template <int MAXLV, bool NARROW>
 __global__ __launch_bounds__(64) void query_files_kernel(
     const uint8_t *__restrict__ batch_in, const int64_t *__restrict__ newline,
     const uint64_t *__restrict__ string, const uint64_t *__restrict__ leveled,
-    int64_t level_stride, int32_t num_levels, FilesMode fm, QueryBatch batch) {
+    int64_t level_stride, int32_t num_levels, FilesMode fm, PosMode pm, QueryBatch batch) {
+  typedef typename std::conditional<NARROW, int32_t, int64_t>::type P;
   const int64_t total = fm.line_prefix[fm.num_files];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t wave_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
-  __shared__ WaveArrays<MAXLV> arrays;
-  for (int64_t base = wave_base; base < total; base += stride) {
-    const int64_t g = base + (threadIdx.x & 63);
-    const bool valid = g < total;
+  const int64_t span = NARROW ? pm.span : INT64_MAX;
+  __shared__ WaveArrays<MAXLV, P> arrays;
+  for (int64_t wb = wave_base; wb < total; wb += stride) {
+    const int64_t g = wb + (threadIdx.x & 63);
+    bool valid = g < total;
+    if (!NARROW && pm.fixup && valid) valid = fm.counts[0][g] == kWideMark;
     // the file holding record g: last f with line_prefix[f] <= g
     int32_t lo = 0, hi = fm.num_files - 1;
     while (lo < hi) {
@@ -788,28 +898,25 @@ __global__ __launch_bounds__(64) void query_files_kernel(
     const int64_t line = g - fm.line_prefix[lo];
     int64_t all = fm.line_prefix[lo + 1] - fm.line_prefix[lo];
     if (all > (int64_t)bf.nl_capacity) all = (int64_t)bf.nl_capacity; // stay in the slice
-    const uint8_t *file = batch_in + bf.offset;
-    Text text;
-    text.blocks = reinterpret_cast<const u64x2 *>(file); // files are tile aligned
-    text.head = 0;
-    text.last = (n - 1) >> 4;
-    const uint64_t word_off = bf.offset >> 6;
-    Ctx c{text, n, string + word_off, leveled + word_off, level_stride, num_levels, 0,
-          kNoTag, u64x2{0, 0}, u64x2{0, 0}, kNoTag, u64x2{0, 0}, kNoTag, u64x2{0, 0}};
     const int64_t *starts = newline + bf.nl_offset;
-    int64_t line_start = 0, line_end = 0;
-    bool record = false;
+    int64_t ls = 0, le = 0;
+    bool record = false, mark = false;
     if (valid && line < all) {
-      line_start = starts[line];
-      line_end = (line + 1 < all) ? starts[line + 1] : n - 1;
-      record = line_start >= 0 && line_start < n && line_end >= 0;
-      if (record) {
-        if (line_end >= n) line_end = n - 1;
-        while (line_end > line_start && byte_at(c, line_end) != '}') --line_end;
-        while (line_start < line_end && byte_at(c, line_start) != '{') ++line_start;
-        record = line_start != line_end;
-      }
+      ls = starts[line];
+      le = (line + 1 < all) ? starts[line + 1] : n - 1;
+      record = ls >= 0 && ls < n && le >= 0;
+      if (record && le >= n) le = n - 1;
     }
+    const int64_t base = (NARROW && record) ? (ls & ~63ll) : 0;
+    if (NARROW && record && le - base >= span) {
+      record = false;
+      mark = true;
+    }
+    const uint64_t word_off = bf.offset >> 6;
+    Ctx<P> c = make_ctx<P>(batch_in + bf.offset, n, string + word_off, leveled + word_off,
+                           level_stride, num_levels, 0, base, span);
+    P line_start = (P)(ls - base), line_end = (P)(le - base);
+    if (record) record = trim_record(c, line_start, line_end);
     if (valid)
       for (int k = 0; k < kBatchSlots; ++k) fm.counts[k][g] = 0;
     for (int q = 0; q < batch.count; ++q) {
@@ -821,22 +928,34 @@ __global__ __launch_bounds__(64) void query_files_kernel(
       if (mine && !qs.variable)
         for (int i = 0; i < 2 * fm.cap; ++i) res[i] = -1;
       const IrReader ir = IrReader::make(qs.ir, qs.ir_len);
-      const int64_t found = run_query<MAXLV>(c, arrays, record && mine, line_start, line_end,
-                                             ir, qs.variable ? fm.cap : qs.cap,
-                                             qs.variable != 0, res);
+      const int64_t found = run_query<MAXLV, P>(c, arrays, record && mine && !c.wide,
+                                                line_start, line_end, ir,
+                                                qs.variable ? fm.cap : qs.cap,
+                                                qs.variable != 0, res);
       if (mine) (slot == 0 ? fm.counts[0] : fm.counts[1])[g] = found;
     }
+    if (NARROW && (mark || c.wide)) fm.counts[0][g] = kWideMark;
   }
 }
 
-template __global__ void query_files_kernel<4>(const uint8_t *, const int64_t *, const uint64_t *, const uint64_t *, int64_t, int32_t, FilesMode, QueryBatch);
-template __global__ void query_files_kernel<6>(const uint8_t *, const int64_t *, const uint64_t *, const uint64_t *, int64_t, int32_t, FilesMode, QueryBatch);
-template __global__ void query_files_kernel<8>(const uint8_t *, const int64_t *, const uint64_t *, const uint64_t *, int64_t, int32_t, FilesMode, QueryBatch);
-template __global__ void query_files_kernel<16>(const uint8_t *, const int64_t *, const uint64_t *, const uint64_t *, int64_t, int32_t, FilesMode, QueryBatch);
-
-template __global__ void query_batch_kernel<4>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
-template __global__ void query_batch_kernel<6>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
-template __global__ void query_batch_kernel<8>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
-template __global__ void query_batch_kernel<16>(const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *, const uint64_t *, int64_t, int32_t, RecordMode, QueryBatch);
+#define GPJ_QUERY_KERNELS(LV, NW)                                                              \
+  template __global__ void query_files_kernel<LV, NW>(const uint8_t *, const int64_t *,        \
+                                                      const uint64_t *, const uint64_t *,       \
+                                                      int64_t, int32_t, FilesMode, PosMode,     \
+                                                      QueryBatch);                              \
+  template __global__ void query_batch_kernel<LV, NW>(const uint8_t *, int64_t, const int64_t *, \
+                                                      const int64_t *, int64_t,                 \
+                                                      const uint64_t *, const uint64_t *,       \
+                                                      int64_t, int32_t, RecordMode, PosMode,    \
+                                                      QueryBatch);
+GPJ_QUERY_KERNELS(4, true)
+GPJ_QUERY_KERNELS(6, true)
+GPJ_QUERY_KERNELS(8, true)
+GPJ_QUERY_KERNELS(16, true)
+GPJ_QUERY_KERNELS(4, false)
+GPJ_QUERY_KERNELS(6, false)
+GPJ_QUERY_KERNELS(8, false)
+GPJ_QUERY_KERNELS(16, false)
+#undef GPJ_QUERY_KERNELS
 
 } // namespace gpj

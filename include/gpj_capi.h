@@ -136,6 +136,14 @@ gpj_status gpj_debug_escape_quote(const uint8_t *d_in, uint64_t n,
  * profiler FETCH_SIZE / WRITE_SIZE readings into bytes.  16-byte aligned. */
 gpj_status gpj_debug_copy(void *d_dst, const void *d_src, uint64_t bytes, void *stream);
 
+/* Debug (no counterpart in the reference): the query kernels hold positions
+ * as 32-bit offsets from each record's start and hand records that reach
+ * `bytes` from it (or backward searches that reach `bytes` below it) to a
+ * 64-bit kernel.  The default, 2^31 - 2^20, is restored with 0; tests set a
+ * small value (>= 4096) to drive the 64-bit kernel on small inputs.  Results
+ * never depend on it.  Process-wide; set it while no query runs. */
+gpj_status gpj_debug_set_narrow_span(int64_t bytes);
+
 /* ---- query: replaces the query kernel launch of QueryExecutor::execute_batch
  *      (src/gpjson/query/kernels/orig/query_executor_host.cu:179-237,
  *      src/gpjson/query/kernels/orig/query.cu:144-454) */
