@@ -831,7 +831,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
   for (int64_t wb = wave_base; wb < lines; wb += stride) {
     const int64_t line = wb + (threadIdx.x & 63);
     bool valid = line < lines;
-    if (!NARROW && pm.fixup && valid) valid = *mark_slot(batch.q[0], line) == kWideMark;
+    if (!NARROW && pm.fixup) {
+      if (valid) valid = *mark_slot(batch.q[0], line) == kWideMark;
+      if (!__ballot(valid)) continue; // (wave-uniform: lanes share wb)
+    }
     int64_t ls = 0, le = 0;
     bool record = false, mark = false;
     if (valid) {
@@ -885,7 +888,10 @@ __global__ __launch_bounds__(64) void query_files_kernel(
   for (int64_t wb = wave_base; wb < total; wb += stride) {
     const int64_t g = wb + (threadIdx.x & 63);
     bool valid = g < total;
-    if (!NARROW && pm.fixup && valid) valid = fm.counts[0][g] == kWideMark;
+    if (!NARROW && pm.fixup) {
+      if (valid) valid = fm.counts[0][g] == kWideMark;
+      if (!__ballot(valid)) continue; // (wave-uniform: lanes share wb)
+    }
     // the file holding record g: last f with line_prefix[f] <= g
     int32_t lo = 0, hi = fm.num_files - 1;
     while (lo < hi) {
