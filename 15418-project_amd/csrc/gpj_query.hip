@@ -434,7 +434,7 @@ template <int MAXLV, typename P> struct WaveArrays {
   P marked_pos[MAXLV][64];
   P fr_sep[kF][64];
   int32_t curr_index[MAXLV][64];
-  uint32_t fr_pack[kF][64]; // qp | lvl << 16 | (mi + 1) << 21
+  uint32_t fr_pack[kF][64]; // qp | lvl << 16 | (mi + 1) << 21 | down << 31
   uint8_t marked_lvl[MAXLV][64];
 };
 
@@ -460,8 +460,10 @@ template <int MAXLV, typename P> struct LaneState {
   LdsArray<int32_t, MAXLV> curr_index;
   LdsArray<P, MAXLV> marked_pos;
   LdsArray<uint8_t, MAXLV> marked_lvl;
-  LdsArray<uint32_t, kF> fr_pack; // bytecode position (after MOVE_TO_EACH, so
-                                  // <= 4097), level and mark depth + 1
+  LdsArray<uint32_t, kF> fr_pack; // bytecode position (after MOVE_TO_EACH and
+                                  // its MOVE_DOWN, so <= 4097), level, mark
+                                  // depth + 1, and whether a MOVE_DOWN follows
+                                  // MOVE_TO_EACH
   LdsArray<P, kF> fr_sep;
   __device__ __forceinline__ LaneState(WaveArrays<MAXLV, P> &w, int lane)
       : level_end{w.level_end, lane}, curr_index{w.curr_index, lane},
@@ -497,7 +499,10 @@ __device__ __forceinline__ bool move_down(Ctx<P> &c, LaneState<MAXLV, P> &s) {
 
 // A finished path (END or any failure): resume the innermost element
 // iteration at its next element, or end the record.  Returns false when the
-// record is done.
+// record is done.  When the bytecode has a MOVE_DOWN right after MOVE_TO_EACH
+// it runs here for the new element (its failure fails the element's path,
+// which resumes the next element), so the element's next step is the
+// instruction after it.
 template <int MAXLV, typename P>
 __device__ __forceinline__ bool resume(Ctx<P> &c, LaneState<MAXLV, P> &s) {
   for (;;) {
@@ -511,9 +516,10 @@ __device__ __forceinline__ bool resume(Ctx<P> &c, LaneState<MAXLV, P> &s) {
       s.fr_sep.set(f, sep);
       s.lvl = flvl;
       s.level_end.fill_above(s.lvl, (P)kUnset);
-      s.mi = (int32_t)(pack >> 21) - 1;
+      s.mi = (int32_t)((pack >> 21) & 31u) - 1;
       s.idx = sep + 1;
       s.qp = (int32_t)(pack & 0xFFFFu);
+      if ((pack >> 31) && !move_down(c, s)) continue;
       return true;
     }
     --s.nfr;
@@ -763,10 +769,10 @@ __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w,
       P p = s.idx + 1;
       while (p < first && byte_at(c, p) == ' ') ++p;
       if (p == first && byte_at(c, first) == ']') { fail = true; break; } // empty array
-      // the frame resumes at the instruction after MOVE_TO_EACH (a fused
-      // MOVE_DOWN runs again for every element)
-      s.fr_pack.set(s.nfr, (uint32_t)(then_down ? s.qp - 1 : s.qp) | ((uint32_t)s.lvl << 16) |
-                               ((uint32_t)(s.mi + 1) << 21));
+      // the frame resumes after MOVE_TO_EACH and its fused MOVE_DOWN (which
+      // resume() runs again for every element)
+      s.fr_pack.set(s.nfr, (uint32_t)s.qp | ((uint32_t)s.lvl << 16) |
+                               ((uint32_t)(s.mi + 1) << 21) | (then_down ? 1u << 31 : 0u));
       s.fr_sep.set(s.nfr, s.idx);
       ++s.nfr;
       ++s.idx;
