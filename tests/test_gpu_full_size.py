@@ -1,0 +1,164 @@
+"""GPU parity at the full configured sizes, through properties that hold at
+any size:
+
+* 8 GiB NDJSON of ~1 KiB records, four queries over one 3-level index: the
+  newline index is checked exhaustively (count, order, every entry a '\\n');
+  the string mask and leveled rows of random record-aligned windows equal
+  the oracle's index of the window alone (at a record start the carried
+  state is the initial one); each query's spans on 2,000 random records
+  equal the oracle's on those records alone; and each query's total equals
+  a count of byte patterns the generator writes only for matching values
+  (inside a string every '"' is escaped, so a pattern that starts with an
+  unescaped '"' after '{' or ',' cannot occur there).
+* The 1 GiB top-level array of 2 M objects, $[*].items[*].sku: element
+  records on 2,000 random elements equal the oracle's line mode on those
+  elements' bytes, and the total equals the number of '"sku":' keys.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+SAMPLE = 2000
+THROUGHPUT_QUERIES = ["$.user.lang", "$.e[0].f", "$.h[*].t", '$.h[?(@.k == "v")].id']
+
+
+def count_pattern(data: np.ndarray, pat: bytes, chunk: int = 1 << 28) -> int:
+    """Occurrences of `pat` in `data`, by chunks that overlap by len(pat) - 1."""
+    total, n, k = 0, data.size, len(pat)
+    for lo in range(0, n, chunk):
+        hi = min(n, lo + chunk + k - 1)
+        total += data[lo:hi].tobytes().count(pat)
+    return total
+
+
+def count_byte(data: np.ndarray, b: int, chunk: int = 1 << 28) -> int:
+    return sum(int(np.count_nonzero(data[lo:lo + chunk] == b))
+               for lo in range(0, data.size, chunk))
+
+
+def bits(words: np.ndarray) -> np.ndarray:
+    return np.unpackbits(np.ascontiguousarray(words).view(np.uint8), bitorder="little")
+
+
+def masked(spans: np.ndarray, counts) -> np.ndarray:
+    out = spans.copy()
+    if counts is not None:
+        out[np.arange(out.shape[1])[None, :] >= counts[:, None]] = -1
+    return out
+
+
+def relative(spans: np.ndarray, starts: np.ndarray) -> np.ndarray:
+    """Spans made relative to each record's first byte (-1 stays -1)."""
+    out = spans.copy()
+    hit = out >= 0
+    out[hit] = (out - starts[:, None, None])[hit]
+    return out
+
+
+@pytest.fixture(scope="module")
+def ndjson_8gib(gpu):
+    data = gpu.generate(gpu.SCHEMA_NDJSON_1K, 8 << 30, 0x15418002)
+    print(f"[full] ndjson {data.size} bytes", flush=True)
+    compiled = [gpu.compile_query(q, extensions=True) for q in THROUGHPUT_QUERIES]
+    levels = max(c[1] for c in compiled)
+    dev = gpu.DeviceBuffer.from_host(data)
+    idx = gpu.build_index(dev, data.size, levels)
+    results = gpu.run_query_batch(dev, data.size, idx, compiled, capacity=8)
+    yield data, compiled, levels, idx, results
+    idx.free()
+    dev.free()
+
+
+def test_ndjson_8gib_newline_index(ndjson_8gib, gpu):
+    data, _c, _l, idx, _r = ndjson_8gib
+    nl = idx.newline()
+    assert nl[0] == 0
+    assert nl.size == 1 + count_byte(data, 10)
+    assert np.all(np.diff(nl[1:]) > 0)
+    assert np.all(data[nl[1:]] == 10)
+
+
+def test_ndjson_8gib_index_windows(ndjson_8gib, gpu, orc):
+    data, _c, levels, idx, _r = ndjson_8gib
+    nl = idx.newline()
+    rng = np.random.default_rng(7)
+    for i in rng.integers(0, nl.size - 40, 12):
+        s = 0 if i == 0 else int(nl[i]) + 1
+        e = int(nl[i + 32]) + 1
+        w0, w1 = s // 64, (e - 1) // 64
+        print(f"[full] window {s}..{e}", flush=True)
+        os_, ol, onl = orc.build_index(data[s:e].tobytes(), levels)
+        np.testing.assert_array_equal(onl[1:] + s, nl[i + 1:i + 33])
+        lo = s - 64 * w0
+        got = bits(gpu.d2h(idx.raw.d_string + 8 * w0, np.uint64, w1 - w0 + 1))
+        np.testing.assert_array_equal(got[lo:lo + e - s], bits(os_)[:e - s])
+        for k in range(levels):
+            ptr = idx.raw.d_leveled + 8 * (k * idx.level_size + w0)
+            got = bits(gpu.d2h(ptr, np.uint64, w1 - w0 + 1))
+            np.testing.assert_array_equal(got[lo:lo + e - s], bits(ol[k])[:e - s])
+
+
+def test_ndjson_8gib_query_sample(ndjson_8gib, gpu, orc):
+    data, compiled, levels, idx, results = ndjson_8gib
+    nl = idx.newline()
+    rng = np.random.default_rng(11)
+    recs = np.sort(rng.choice(nl.size - 1, SAMPLE, replace=False))
+    starts = np.where(recs == 0, 0, nl[recs] + 1)
+    ends = nl[recs + 1]
+    lines = [data[a:b].tobytes() for a, b in zip(starts, ends)]
+    mini = b"\n".join(lines) + b"\n"
+    mstarts = np.cumsum([0] + [len(x) + 1 for x in lines[:-1]])
+    oix = orc.build_index(mini, levels)
+    for (ir, _d, slots), (spans, counts), q in zip(compiled, results, THROUGHPUT_QUERIES):
+        print(f"[full] sample {q}", flush=True)
+        got = relative(masked(spans[recs], None if counts is None else counts[recs]), starts)
+        if counts is None:
+            want = orc.query(mini, oix, ir, slots)[:SAMPLE]
+        else:
+            ws, wc = orc.query_ext(mini, oix, ir, spans.shape[1])
+            np.testing.assert_array_equal(counts[recs], wc[:SAMPLE])
+            want = masked(ws[:SAMPLE], wc[:SAMPLE])
+        np.testing.assert_array_equal(got, relative(want, mstarts))
+
+
+def test_ndjson_8gib_query_totals(ndjson_8gib):
+    data, _c, _l, idx, results = ndjson_8gib
+    records = idx.num_lines - 1
+    (lang, _), (f0, _), (t_sp, t_ct), (id_sp, id_ct) = results
+    # every record has user.lang and a first e element with "f"
+    assert int((lang[:, 0, 0] >= 0).sum()) == records
+    assert int((f0[:, 0, 0] >= 0).sum()) == records
+    # "t" keys appear only in h elements, k == "v" only as '"k":"v","id":'
+    assert int(t_ct.sum()) == count_pattern(data, b'{"t":')
+    assert int(id_ct.sum()) == count_pattern(data, b',"k":"v","id":')
+    assert int(id_ct.sum()) > 0
+
+
+def test_array_1gib_items_sku(gpu, orc):
+    data = gpu.generate(gpu.SCHEMA_BIG_ARRAY, 2_000_000, 0x15418001)
+    print(f"[full] array {data.size} bytes", flush=True)
+    query = "$[*].items[*].sku"
+    compiled = gpu.compile_element_query(query)
+    levels = compiled[1] + 1
+    dev = gpu.DeviceBuffer.from_host(data)
+    idx = gpu.build_index(dev, data.size, levels)
+    bounds, m = gpu.array_bounds(idx)
+    assert m == 2_000_001
+    (spans, counts), = gpu.run_query_records(dev, data.size, idx, bounds, m, [compiled], 24)
+    b = bounds.to_host(np.int64, m)
+    assert int(counts.sum()) == count_pattern(data, b'"sku":')
+    rng = np.random.default_rng(13)
+    recs = np.sort(rng.choice(m - 1, SAMPLE, replace=False))
+    starts = b[recs] + 1
+    lines = [data[a:z].tobytes() for a, z in zip(starts, b[recs + 1])]
+    mini = b"\n".join(lines) + b"\n"
+    mstarts = np.cumsum([0] + [len(x) + 1 for x in lines[:-1]])
+    ir, depth, _slots = gpu.compile_query("$" + query[4:], extensions=True)
+    assert ir == compiled[0]
+    oix = orc.build_index(mini, depth)
+    ws, wc = orc.query_ext(mini, oix, ir, spans.shape[1])
+    np.testing.assert_array_equal(counts[recs], wc[:SAMPLE])
+    np.testing.assert_array_equal(relative(spans[recs], starts),
+                                  relative(masked(ws[:SAMPLE], wc[:SAMPLE]), mstarts))
+    idx.free()
+    dev.free()
