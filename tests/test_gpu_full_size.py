@@ -13,9 +13,16 @@ any size:
 * The 1 GiB top-level array of 2 M objects, $[*].items[*].sku: element
   records on 2,000 random elements equal the oracle's line mode on those
   elements' bytes, and the total equals the number of '"sku":' keys.
+* The 64 MiB plain input, compared whole with the oracle.
+* The 10,000-file batch (7.9 GB): line prefix, and every structure and
+  result of 31 files (30 random ones and the largest) equal the oracle run on
+  each file alone; plain per-file launches equal the batched launch for all
+  files.
 """
 import numpy as np
 import pytest
+
+from batchdata import BATCH_QUERIES, plan
 
 pytestmark = pytest.mark.gpu
 SAMPLE = 2000
@@ -162,3 +169,55 @@ def test_array_1gib_items_sku(gpu, orc):
                                   relative(masked(ws[:SAMPLE], wc[:SAMPLE]), mstarts))
     idx.free()
     dev.free()
+
+
+def test_ndjson_64mib_ab_exact(gpu, orc):
+    # the plain 64 MiB configuration is small enough to compare whole
+    data = gpu.generate(gpu.SCHEMA_NDJSON_2_4K, 64 << 20, 0x15418000)
+    ir, depth, slots = gpu.compile_query("$.a.b")
+    dev = gpu.DeviceBuffer.from_host(data)
+    idx = gpu.build_index(dev, data.size, depth)
+    got = gpu.run_query(dev, data.size, idx, ir, slots)
+    s, lv, nl = orc.build_index(data, depth, 8)
+    np.testing.assert_array_equal(idx.newline(), nl)
+    np.testing.assert_array_equal(idx.string(), s)
+    np.testing.assert_array_equal(idx.leveled(), lv)
+    want = orc.query(data, (s, lv, nl), ir, slots, 8)
+    np.testing.assert_array_equal(got, want)
+    assert int((got[:, 0, 0] >= 0).sum()) > 0
+
+
+def test_batch_10k_files_sample(gpu, orc):
+    from test_gpu_batch_files import CAP, oracle_file
+    schemas, targets, masks = plan(10000, 256 << 10, 1.5, 1 << 10, 256 << 20, 0x15418003)
+    buf, off, sz = gpu.generate_files(schemas, targets, 0x15418003)
+    print(f"[full] batch {int(sz.sum())} bytes", flush=True)
+    compiled = [gpu.compile_query(q, extensions=True) for q in BATCH_QUERIES]
+    b = gpu.FileBatch(buf, off, sz, masks, compiled, capacity=CAP, stream=None)
+    b.launch()
+    b.buf = buf
+    np.testing.assert_array_equal(b.device_line_prefix(), b.line_prefix)
+    spans, counts = b.fetch()
+    rng = np.random.default_rng(17)
+    files = set(rng.choice(b.num_files, 30, replace=False).tolist())
+    files.add(int(np.argmax(b.sizes)))
+    for f in sorted(files):
+        print(f"[full] file {f} bytes={int(b.sizes[f])}", flush=True)
+        data, (os_, ol, onl), per_slot = oracle_file(orc, gpu, b, f)
+        w0, w = int(b.offsets[f]) // 64, (int(b.sizes[f]) + 63) // 64
+        np.testing.assert_array_equal(gpu.d2h(b.d_string.ptr + 8 * w0, np.uint64, w), os_)
+        for k in range(b.levels):
+            np.testing.assert_array_equal(
+                gpu.d2h(b.d_leveled.ptr + 8 * (k * b.words + w0), np.uint64, w), ol[k])
+        a, z = int(b.line_prefix[f]), int(b.line_prefix[f + 1])
+        np.testing.assert_array_equal(gpu.d2h(b.d_newline.ptr + 8 * a, np.int64, z - a), onl)
+        for k, (sp, ct) in enumerate(per_slot):
+            np.testing.assert_array_equal(counts[k][a:z], ct)
+            np.testing.assert_array_equal(spans[k][a:z], sp)
+    # plain per-file launches give the same results for every file
+    b.per_file_setup()
+    b.launch_per_file()
+    ps, pc = b.fetch(per_file=True)
+    for k in range(2):
+        np.testing.assert_array_equal(pc[k], counts[k])
+        np.testing.assert_array_equal(ps[k], spans[k])
