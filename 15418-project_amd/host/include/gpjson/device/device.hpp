@@ -52,6 +52,61 @@ private:
   size_t size_bytes_{0};
 };
 
+/* Host bytes headed for the device: page-locked when the runtime provides
+ * them (copies then run at full link speed and overlap kernels), pageable
+ * otherwise (no device, or the pinned allocation fails).  Uninitialized;
+ * move-only. */
+class HostBuffer {
+public:
+  HostBuffer() = default;
+  explicit HostBuffer(size_t bytes);
+  ~HostBuffer();
+
+  HostBuffer(HostBuffer &&other) noexcept;
+  HostBuffer &operator=(HostBuffer &&other) noexcept;
+  HostBuffer(const HostBuffer &) = delete;
+  HostBuffer &operator=(const HostBuffer &) = delete;
+
+  std::byte *data() { return ptr_; }
+  const std::byte *data() const { return ptr_; }
+  size_t size_bytes() const { return size_bytes_; }
+  bool pinned() const { return pinned_; }
+
+private:
+  void release();
+  std::byte *ptr_{nullptr};
+  size_t size_bytes_{0};
+  bool pinned_{false};
+};
+
+/* A HIP stream of its own (non-blocking, default priority), created on first
+ * use; move-only. */
+class Stream {
+public:
+  Stream() = default;
+  ~Stream();
+  Stream(const Stream &) = delete;
+  Stream &operator=(const Stream &) = delete;
+  void *get();
+
+private:
+  void *stream_{nullptr};
+};
+
+/* A host-to-device copy in flight on a caller's stream: waits for it (and
+ * releases its event) when completed or destroyed. */
+class PendingCopy {
+public:
+  PendingCopy(void *d_dst, const void *h_src, size_t bytes, void *stream);
+  ~PendingCopy();
+  PendingCopy(const PendingCopy &) = delete;
+  PendingCopy &operator=(const PendingCopy &) = delete;
+  void wait();
+
+private:
+  void *event_{nullptr};
+};
+
 /* Throws DeviceError (or the matching GpJSONError subclass) for a non-zero
  * gpj_status, with `context` and the library's message. */
 void check(int32_t status, const char *context);

@@ -5,6 +5,7 @@
 #include "gpjson/device/device.hpp"
 
 #include <cstddef>
+#include <memory>
 #include <string>
 
 namespace gpjson::file {
@@ -27,8 +28,13 @@ public:
   const void *host_bytes() const { return host_data_; }
   const void *device_bytes() const { return device_data_.data(); }
   bool device_loaded() const;
-  /* Copy the partition to the device; idempotent. */
+  /* Copy the partition to the device; idempotent.  Completes a copy started
+   * by prefetch_to_device. */
   void load_to_device();
+  /* Start the copy on `stream` and return (no counterpart in the reference:
+   * the engine overlaps the next partition's copy with the current one's
+   * index and queries).  device_bytes() is usable after load_to_device(). */
+  void prefetch_to_device(void *stream);
 
 private:
   size_t partition_id_{0};
@@ -36,6 +42,7 @@ private:
   size_t global_end_offset_{0};
   const void *host_data_{nullptr};
   device::DeviceArray device_data_;
+  std::unique_ptr<device::PendingCopy> pending_; // destroyed (waited for) first
 };
 
 struct FileMetadata {

@@ -54,8 +54,8 @@ public:
 private:
   std::string file_path_;
   FileMetadata metadata_;
-  std::vector<std::byte> bytes_;
-  std::vector<FilePartition> partitions_;
+  device::HostBuffer bytes_;              // the file, page-locked when possible
+  std::vector<FilePartition> partitions_; // destroyed first: waits for their copies
 };
 
 /* Maps the file read-only. */

@@ -51,6 +51,86 @@ DeviceArray &DeviceArray::operator=(DeviceArray &&other) noexcept {
   return *this;
 }
 
+HostBuffer::HostBuffer(size_t bytes) : size_bytes_(bytes) {
+  if (bytes == 0) return;
+  void *p = nullptr;
+  if (gpj_host_alloc(&p, bytes) == GPJ_OK && p) {
+    ptr_ = static_cast<std::byte *>(p);
+    pinned_ = true;
+  } else {
+    ptr_ = new std::byte[bytes];
+  }
+}
+
+void HostBuffer::release() {
+  if (!ptr_) return;
+  if (pinned_) gpj_host_free(ptr_);
+  else delete[] ptr_;
+  ptr_ = nullptr;
+  size_bytes_ = 0;
+  pinned_ = false;
+}
+
+HostBuffer::~HostBuffer() { release(); }
+
+HostBuffer::HostBuffer(HostBuffer &&other) noexcept
+    : ptr_(other.ptr_), size_bytes_(other.size_bytes_), pinned_(other.pinned_) {
+  other.ptr_ = nullptr;
+  other.size_bytes_ = 0;
+  other.pinned_ = false;
+}
+
+HostBuffer &HostBuffer::operator=(HostBuffer &&other) noexcept {
+  if (this == &other) return *this;
+  release();
+  ptr_ = other.ptr_;
+  size_bytes_ = other.size_bytes_;
+  pinned_ = other.pinned_;
+  other.ptr_ = nullptr;
+  other.size_bytes_ = 0;
+  other.pinned_ = false;
+  return *this;
+}
+
+Stream::~Stream() {
+  if (stream_) gpj_stream_destroy(stream_);
+}
+
+void *Stream::get() {
+  if (!stream_) check(gpj_stream_create(&stream_), "stream");
+  return stream_;
+}
+
+PendingCopy::PendingCopy(void *d_dst, const void *h_src, size_t bytes, void *stream) {
+  check(gpj_event_create(&event_), "copy event");
+  const int32_t st = gpj_h2d(d_dst, h_src, bytes, stream);
+  if (st == GPJ_OK) {
+    const int32_t rec = gpj_event_record(event_, stream);
+    if (rec == GPJ_OK) return;
+    gpj_stream_sync(stream);
+    gpj_event_destroy(event_);
+    event_ = nullptr;
+    check(rec, "copy event");
+  }
+  gpj_event_destroy(event_);
+  event_ = nullptr;
+  check(st, "copy host to device");
+}
+
+void PendingCopy::wait() {
+  if (!event_) return;
+  const int32_t st = gpj_event_sync(event_);
+  gpj_event_destroy(event_);
+  event_ = nullptr;
+  check(st, "copy host to device");
+}
+
+PendingCopy::~PendingCopy() {
+  if (!event_) return;
+  gpj_event_sync(event_);
+  gpj_event_destroy(event_);
+}
+
 DeviceArray DeviceArray::adopt(void *ptr, size_t bytes) {
   DeviceArray a;
   a.ptr_ = ptr;

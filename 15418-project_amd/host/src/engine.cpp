@@ -1,6 +1,8 @@
 // Engine::query (reference: src/gpjson/engine.cpp:17-175): compile the
 // queries, partition the file at newlines, and per partition copy it to the
-// device, build the indices, run the queries, materialize and append.
+// device, build the indices, run the queries, materialize and append.  The
+// next partition's copy runs on a stream of its own while the current one is
+// indexed and queried (the reference copies each partition synchronously).
 #include "gpjson/engine.hpp"
 
 #include "gpjson/error/common.hpp"
@@ -54,6 +56,7 @@ query::MaterializedBatchResult Engine::query(const std::string &file_path,
   for (const auto &q : queries_src) compiled.add(compiler.compile(q, options));
   profiler.end(comp);
 
+  device::Stream copy_stream; // outlives the reader's partitions and their copies
   auto reader = make_reader(file_path, options.file_reader_options);
   const auto part_seg = profiler.begin("file_reader.create_partitions");
   reader->create_partitions(options.index_builder_options.file_partition_size);
@@ -64,11 +67,14 @@ query::MaterializedBatchResult Engine::query(const std::string &file_path,
   merged.reserve(compiled.size());
   for (const auto &q : compiled.queries()) merged.emplace_back(q.query_text());
 
-  for (auto &partition : reader->get_partitions()) {
+  auto &partitions = reader->get_partitions();
+  for (size_t i = 0; i < partitions.size(); ++i) {
+    auto &partition = partitions[i];
     const size_t pid = partition.partition_id();
     const auto pseg = profiler.begin_nestedf("partition %zu total", pid);
     const auto lseg = profiler.begin_nestedf("partition %zu load_to_device", pid);
     partition.load_to_device();
+    if (i + 1 < partitions.size()) partitions[i + 1].prefetch_to_device(copy_stream.get());
     profiler.end(lseg);
     const auto bseg = profiler.begin_nestedf("partition %zu build_index", pid);
     index::BuiltIndices built = builder->build(partition, compiled.get_max_depth(),

@@ -21,10 +21,22 @@ FilePartition::FilePartition(size_t partition_id, size_t global_start_offset,
       global_end_offset_(global_end_offset), host_data_(data) {}
 
 bool FilePartition::device_loaded() const {
-  return size_bytes() == 0 || device_data_.data() != nullptr;
+  return size_bytes() == 0 || (device_data_.data() != nullptr && !pending_);
+}
+
+void FilePartition::prefetch_to_device(void *stream) {
+  if (size_bytes() == 0 || device_data_.data() != nullptr) return;
+  device::DeviceArray d(size_bytes());
+  pending_ = std::make_unique<device::PendingCopy>(d.data(), host_data_, size_bytes(), stream);
+  device_data_ = std::move(d);
 }
 
 void FilePartition::load_to_device() {
+  if (pending_) {
+    pending_->wait();
+    pending_.reset();
+    return;
+  }
   if (device_loaded()) return;
   device::DeviceArray d(size_bytes());
   d.copy_from_host(host_data_, size_bytes());
@@ -84,7 +96,8 @@ CopiedFileReader::CopiedFileReader(std::string file_path) : file_path_(std::move
 
 void CopiedFileReader::create_partitions(size_t partition_size_bytes) {
   metadata_.file_size_bytes = file_size_or_throw(file_path_);
-  bytes_.assign(metadata_.file_size_bytes, std::byte{0});
+  partitions_.clear();
+  bytes_ = device::HostBuffer(metadata_.file_size_bytes); // pinned when possible
   std::ifstream in(file_path_, std::ios::binary);
   if (!in) throw error::file::FileOpenError("Failed to open file '" + file_path_ + "' for reading.");
   if (metadata_.file_size_bytes > 0) {
@@ -93,7 +106,7 @@ void CopiedFileReader::create_partitions(size_t partition_size_bytes) {
     if (!in || static_cast<size_t>(in.gcount()) != metadata_.file_size_bytes)
       throw error::file::FileReadError("Failed to read all bytes from file '" + file_path_ + "'.");
   }
-  partitions_ = partition_bytes(file_path_, bytes_.empty() ? nullptr : bytes_.data(),
+  partitions_ = partition_bytes(file_path_, bytes_.size_bytes() == 0 ? nullptr : bytes_.data(),
                                 metadata_.file_size_bytes, partition_size_bytes);
   metadata_.num_partitions = partitions_.size();
 }
@@ -102,7 +115,10 @@ MmapFileReader::MmapFileReader(std::string file_path) : file_path_(std::move(fil
   metadata_.file_path = file_path_;
 }
 
-MmapFileReader::~MmapFileReader() { unmap(); }
+MmapFileReader::~MmapFileReader() {
+  partitions_.clear(); // waits for copies still reading the mapping
+  unmap();
+}
 
 void MmapFileReader::map() {
   if (fd_ != -1 || mapped_size_ != 0) return;
