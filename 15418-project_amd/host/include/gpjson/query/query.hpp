@@ -65,6 +65,7 @@ public:
   void add_offset(QueryOffset offset) { offsets_.push_back(std::move(offset)); }
 
 private:
+  friend class BatchQueryResult; // materialize() && moves the texts out
   std::vector<QueryOffset> offsets_;
 };
 
@@ -74,6 +75,7 @@ public:
   const std::vector<LineQueryResult> &lines() const { return lines_; }
 
 private:
+  friend class BatchQueryResult;
   std::vector<LineQueryResult> lines_;
 };
 
@@ -86,8 +88,10 @@ public:
   void add_line_result(size_t query_index, LineQueryResult line_result);
   void merge(const BatchQueryResult &other);
   void set_query_text(size_t query_index, std::string query_text);
-  /* Drops unmatched slots and lines without any value (query.cpp:87-109). */
-  MaterializedBatchResult materialize() const;
+  /* Drops unmatched slots and lines without any value (query.cpp:87-109).
+   * The rvalue form moves the value texts instead of copying them. */
+  MaterializedBatchResult materialize() const &;
+  MaterializedBatchResult materialize() &&;
   size_t num_queries() const { return queries_.size(); }
   const std::vector<QueryResult> &queries() const { return queries_; }
 

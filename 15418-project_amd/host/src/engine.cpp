@@ -84,7 +84,7 @@ query::MaterializedBatchResult Engine::query(const std::string &file_path,
     query::BatchQueryResult batch = executor.execute_batch(compiled, partition, built);
     profiler.end(qseg);
     const auto mseg = profiler.begin_nestedf("partition %zu materialize_results", pid);
-    query::MaterializedBatchResult part = batch.materialize();
+    query::MaterializedBatchResult part = std::move(batch).materialize();
     profiler.end(mseg);
     if (part.queries().size() != merged.size())
       throw error::common::ImplementationError(

@@ -28,7 +28,24 @@ void BatchQueryResult::set_query_text(size_t query_index, std::string query_text
   if (query_index < query_texts_.size()) query_texts_[query_index] = std::move(query_text);
 }
 
-MaterializedBatchResult BatchQueryResult::materialize() const {
+MaterializedBatchResult BatchQueryResult::materialize() && {
+  MaterializedBatchResult batch;
+  for (size_t q = 0; q < queries_.size(); ++q) {
+    MaterializedQueryResult result(std::move(query_texts_[q]));
+    for (auto &line : queries_[q].lines_) {
+      MaterializedLineResult m;
+      for (auto &off : line.offsets_)
+        if (off.json_text.has_value()) m.add_value(MaterializedValue(std::move(*off.json_text)));
+      if (!m.values().empty()) result.add_line_result(std::move(m));
+    }
+    batch.add_query_result(std::move(result));
+  }
+  queries_.clear();
+  query_texts_.clear();
+  return batch;
+}
+
+MaterializedBatchResult BatchQueryResult::materialize() const & {
   MaterializedBatchResult batch;
   for (size_t q = 0; q < queries_.size(); ++q) {
     MaterializedQueryResult result(query_texts_[q]);

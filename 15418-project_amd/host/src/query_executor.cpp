@@ -7,7 +7,10 @@
 #include "gpjson/query/error.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <exception>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace gpjson::query {
@@ -100,26 +103,64 @@ BatchQueryResult QueryExecutor::execute_batch(const BatchCompiledQuery &compiled
       const size_t count = static_cast<size_t>(num_lines) * slots * 2;
       std::vector<int64_t> host_spans(count, -1);
       spans[q - first].copy_to_host(host_spans.data(), count * sizeof(int64_t));
-      for (int64_t line = 0; line < num_lines; ++line) {
-        LineQueryResult lr;
-        for (int s = 0; s < slots; ++s) {
-          const int64_t a = host_spans[(static_cast<size_t>(line) * slots + s) * 2];
-          const int64_t b = host_spans[(static_cast<size_t>(line) * slots + s) * 2 + 1];
-          if (a < 0 || b < 0) {
-            lr.add_offset(QueryOffset{});
-            continue;
+      // one host string per match (query_executor_host.cu:104-119), built by
+      // up to 16 threads over contiguous line ranges, appended in order
+      const size_t lines_n = static_cast<size_t>(num_lines);
+      std::vector<LineQueryResult> lines(lines_n);
+      std::atomic<bool> bad_range{false};
+      auto build = [&](size_t lo, size_t hi) {
+        for (size_t line = lo; line < hi && !bad_range.load(std::memory_order_relaxed); ++line) {
+          LineQueryResult &lr = lines[line];
+          for (int s = 0; s < slots; ++s) {
+            const int64_t a = host_spans[(line * slots + s) * 2];
+            const int64_t b = host_spans[(line * slots + s) * 2 + 1];
+            if (a < 0 || b < 0) {
+              lr.add_offset(QueryOffset{});
+              continue;
+            }
+            if (a > b || static_cast<size_t>(b) > partition.size_bytes()) {
+              bad_range = true;
+              return;
+            }
+            QueryOffset off;
+            off.start = base + static_cast<size_t>(a);
+            off.end = base + static_cast<size_t>(b);
+            off.json_text = std::string(host + a, static_cast<size_t>(b - a));
+            lr.add_offset(std::move(off));
           }
-          if (a > b || static_cast<size_t>(b) > partition.size_bytes())
-            throw error::query::QueryExecutionError(
-                "Query kernel produced an invalid result range");
-          QueryOffset off;
-          off.start = base + static_cast<size_t>(a);
-          off.end = base + static_cast<size_t>(b);
-          off.json_text = std::string(host + a, static_cast<size_t>(b - a));
-          lr.add_offset(std::move(off));
         }
-        batch.add_line_result(q, std::move(lr));
+      };
+      const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+      const size_t workers = std::min({hw, size_t{16}, (lines_n + 65535) / 65536});
+      if (workers <= 1) {
+        build(0, lines_n);
+      } else {
+        const size_t per = (lines_n + workers - 1) / workers;
+        std::vector<std::exception_ptr> errors(workers);
+        std::vector<std::thread> pool;
+        try {
+          for (size_t t = 0; t < workers; ++t)
+            pool.emplace_back([&, t] {
+              try {
+                build(t * per, std::min(lines_n, (t + 1) * per));
+              } catch (...) {
+                errors[t] = std::current_exception();
+                bad_range = true;
+              }
+            });
+        } catch (...) { // a thread could not start: stop and join the others
+          bad_range = true;
+          for (auto &th : pool) th.join();
+          throw;
+        }
+        for (auto &th : pool) th.join();
+        for (auto &e : errors)
+          if (e) std::rethrow_exception(e);
       }
+      if (bad_range)
+        throw error::query::QueryExecutionError("Query kernel produced an invalid result range");
+      for (size_t line = 0; line < lines_n; ++line)
+        batch.add_line_result(q, std::move(lines[line]));
     }
   }
   return batch;
