@@ -8,10 +8,13 @@
 #include <cstring>
 #include <fcntl.h>
 #include <filesystem>
-#include <fstream>
+#include <algorithm>
+#include <atomic>
 #include <sstream>
 #include <sys/mman.h>
+#include <thread>
 #include <unistd.h>
+#include <vector>
 
 namespace gpjson::file {
 
@@ -88,6 +91,41 @@ size_t file_size_or_throw(const std::string &path) {
     throw error::file::FileOpenError("Failed to get size for file '" + path + "': " + ec.message());
   return static_cast<size_t>(sz);
 }
+
+// Reads bytes [0, size) of the open file into dst: 64 MiB pieces on up to 16
+// threads (a cached file is then read at memory speed, not one core's copy
+// speed).  False if any piece comes up short.
+bool read_all(int fd, std::byte *dst, size_t size) {
+  constexpr size_t kPiece = size_t{64} << 20;
+  const size_t pieces = (size + kPiece - 1) / kPiece;
+  const size_t workers =
+      std::min<size_t>({std::max(1u, std::thread::hardware_concurrency()), 16, pieces});
+  std::atomic<size_t> next{0};
+  std::atomic<bool> ok{true};
+  auto work = [&] {
+    for (size_t k; ok.load(std::memory_order_relaxed) && (k = next++) < pieces;) {
+      size_t off = k * kPiece;
+      const size_t end = std::min(size, off + kPiece);
+      while (off < end) {
+        const ssize_t got = pread(fd, dst + off, end - off, static_cast<off_t>(off));
+        if (got < 0 && errno == EINTR) continue;
+        if (got <= 0) {
+          ok = false;
+          return;
+        }
+        off += static_cast<size_t>(got);
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  try {
+    for (size_t t = 1; t < workers; ++t) pool.emplace_back(work);
+  } catch (...) { // fewer threads: the ones running and this one finish the pieces
+  }
+  work();
+  for (auto &th : pool) th.join();
+  return ok.load();
+}
 } // namespace
 
 CopiedFileReader::CopiedFileReader(std::string file_path) : file_path_(std::move(file_path)) {
@@ -98,14 +136,13 @@ void CopiedFileReader::create_partitions(size_t partition_size_bytes) {
   metadata_.file_size_bytes = file_size_or_throw(file_path_);
   partitions_.clear();
   bytes_ = device::HostBuffer(metadata_.file_size_bytes); // pinned when possible
-  std::ifstream in(file_path_, std::ios::binary);
-  if (!in) throw error::file::FileOpenError("Failed to open file '" + file_path_ + "' for reading.");
-  if (metadata_.file_size_bytes > 0) {
-    in.read(reinterpret_cast<char *>(bytes_.data()),
-            static_cast<std::streamsize>(metadata_.file_size_bytes));
-    if (!in || static_cast<size_t>(in.gcount()) != metadata_.file_size_bytes)
-      throw error::file::FileReadError("Failed to read all bytes from file '" + file_path_ + "'.");
-  }
+  const int fd = open(file_path_.c_str(), O_RDONLY);
+  if (fd == -1) throw error::file::FileOpenError("Failed to open file '" + file_path_ + "' for reading.");
+  const bool ok = metadata_.file_size_bytes == 0 ||
+                  read_all(fd, bytes_.data(), metadata_.file_size_bytes);
+  close(fd);
+  if (!ok)
+    throw error::file::FileReadError("Failed to read all bytes from file '" + file_path_ + "'.");
   partitions_ = partition_bytes(file_path_, bytes_.size_bytes() == 0 ? nullptr : bytes_.data(),
                                 metadata_.file_size_bytes, partition_size_bytes);
   metadata_.num_partitions = partitions_.size();
