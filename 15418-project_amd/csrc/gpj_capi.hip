@@ -445,8 +445,13 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "Query execution requires GPU-resident newline, string, and bitmap indices");
   // one wave per block (the kernel keeps each wave's interpreter arrays in
-  // its block's LDS)
-  const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_records, 64), 1u << 22);
+  // its block's LDS).  Few records: fewer per wave (down to 8), so that more
+  // waves share the SIMDs and a wave's step waits for fewer records.
+  int32_t per_wave = 64;
+  while (per_wave > 8 && (uint64_t)max_records / (uint64_t)per_wave < 4096) per_wave /= 2;
+  mode.per_wave = per_wave;
+  const uint64_t blocks =
+      std::min<uint64_t>(div_up((uint64_t)max_records, (uint64_t)per_wave), 1u << 22);
   // 32-bit positions; records the narrow kernel marks (only possible when
   // the input is longer than the span) are rerun by the wide kernel
   const int64_t span = g_narrow_span.load();
@@ -486,7 +491,7 @@ gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index 
                                int64_t max_lines, void *stream) {
   if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null index");
   return run_batch(d_in, n, index, index->d_newline, index->d_num_lines,
-                   gpj::RecordMode{0, 0}, queries, num_queries, max_lines, stream);
+                   gpj::RecordMode{0, 0, 64}, queries, num_queries, max_lines, stream);
 }
 
 uint64_t gpj_row_bounds_scratch_bytes(int64_t level_size) {
@@ -526,7 +531,7 @@ gpj_status gpj_query_run_records(const uint8_t *d_in, uint64_t n, const gpj_inde
   if (index->num_levels < 2)
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "array records need an index with at least 2 leveled rows");
-  return run_batch(d_in, n, index, d_bounds, d_num_bounds, gpj::RecordMode{1, 1}, queries,
+  return run_batch(d_in, n, index, d_bounds, d_num_bounds, gpj::RecordMode{1, 1, 64}, queries,
                    num_queries, max_records, stream);
 }
 

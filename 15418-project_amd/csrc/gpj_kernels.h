@@ -71,10 +71,12 @@ struct QueryBatch {
 
 // How records are cut: lines of a newline index (level_base 0, drop_last 0)
 // or top-level array elements between row-0 bounds (level_base 1,
-// drop_last 1).
+// drop_last 1); and how many consecutive records one wave takes at a time
+// (per_wave, 1..64; lanes past it idle).
 struct RecordMode {
   int32_t level_base;
   int32_t drop_last;
+  int32_t per_wave;
 };
 
 // How the query kernels hold positions.  The narrow kernels (NARROW = true)

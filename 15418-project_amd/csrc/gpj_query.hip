@@ -818,7 +818,8 @@ __device__ __forceinline__ int64_t *mark_slot(const QuerySlot &q0, int64_t line)
 // one runs to the end of the input) or, in top-level-array mode, the spans
 // between consecutive leveled-row-0 characters, with interpreter level k on
 // leveled row k + 1 (mode.level_base = 1, mode.drop_last = 1: the last bound
-// opens no record).  Positions as PosMode describes.
+// opens no record).  Each wave takes mode.per_wave consecutive records at a
+// time.  Positions as PosMode describes.
 template <int MAXLV, bool NARROW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ? 4 : 1))) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
@@ -830,13 +831,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
   const int64_t all = *num_lines;
   const int64_t avail = all - mode.drop_last;
   const int64_t lines = avail < max_lines ? avail : max_lines;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t wave_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  // one wave per block, mode.per_wave records per wave
+  const int64_t stride = (int64_t)gridDim.x * mode.per_wave;
+  const int64_t wave_base = (int64_t)blockIdx.x * mode.per_wave;
+  const int32_t lane = (int32_t)(threadIdx.x & 63);
   const int64_t span = NARROW ? pm.span : INT64_MAX;
   __shared__ WaveArrays<MAXLV, P> arrays;
   for (int64_t wb = wave_base; wb < lines; wb += stride) {
-    const int64_t line = wb + (threadIdx.x & 63);
-    bool valid = line < lines;
+    const int64_t line = wb + lane;
+    bool valid = lane < mode.per_wave && line < lines;
     if (!NARROW && pm.fixup) {
       if (valid) valid = *mark_slot(batch.q[0], line) == kWideMark;
       if (!__ballot(valid)) continue; // (wave-uniform: lanes share wb)
