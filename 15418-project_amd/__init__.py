@@ -171,6 +171,9 @@ def lib():
         L.gpj_batch_query.argtypes = [vp, u64, vp, i32, i32, vp, vp, vp, vp, i64,
                                       ctypes.POINTER(QueryDescT), i32,
                                       ctypes.POINTER(BatchResultsT), vp]
+        L.gpj_batch_query_shaped.argtypes = [vp, u64, vp, i32, i32, vp, vp, vp, vp, i64, i32,
+                                             ctypes.POINTER(QueryDescT), i32,
+                                             ctypes.POINTER(BatchResultsT), vp]
         L.gpj_batch_per_file.argtypes = [vp, ctypes.POINTER(BatchFileT), i32, i32, vp, vp, vp,
                                          vp, vp, ctypes.POINTER(QueryDescT), i32,
                                          ctypes.POINTER(BatchResultsT), vp, vp]
@@ -186,6 +189,7 @@ def lib():
                      "gpj_query_run_ext", "gpj_query_run_batch", "gpj_query_compile",
                      "gpj_debug_copy", "gpj_row_bounds", "gpj_query_run_records",
                      "gpj_batch_tile_map", "gpj_batch_index", "gpj_batch_query",
+                     "gpj_batch_query_shaped",
                      "gpj_batch_per_file", "gpj_graph_begin", "gpj_graph_end",
                      "gpj_graph_launch", "gpj_graph_destroy", "gpj_stream_create",
                      "gpj_stream_destroy", "gpj_debug_set_narrow_span"):
@@ -660,6 +664,7 @@ class FileBatch:
                                        int(uses_extension_opcodes(ir)), int(depth), None, None)
         self.results = self._alloc_results()
         self.grid_records = self.total_lines
+        self.per_wave = 64  # records per one-wave workgroup of the query launch
         self.graph = None
 
     def _alloc_results(self):
@@ -679,11 +684,12 @@ class FileBatch:
                                 self.d_tile_file.ptr, self.levels, self.d_string.ptr,
                                 self.d_leveled.ptr, self.d_newline.ptr, self.d_prefix.ptr,
                                 self.d_scratch.ptr, self.stream), "batch index")
-        check(L.gpj_batch_query(self.d_in.ptr, self.n, self.d_files.ptr, self.num_files,
-                                self.levels, self.d_string.ptr, self.d_leveled.ptr,
-                                self.d_newline.ptr, self.d_prefix.ptr, self.grid_records,
-                                self.descs, len(self.compiled), ctypes.byref(self.results[0]),
-                                self.stream), "batch query")
+        check(L.gpj_batch_query_shaped(self.d_in.ptr, self.n, self.d_files.ptr, self.num_files,
+                                       self.levels, self.d_string.ptr, self.d_leveled.ptr,
+                                       self.d_newline.ptr, self.d_prefix.ptr, self.grid_records,
+                                       self.per_wave, self.descs, len(self.compiled),
+                                       ctypes.byref(self.results[0]), self.stream),
+              "batch query")
 
     def capture(self):
         """Capture launch() in a HIP graph (needs a non-default stream)."""
@@ -746,29 +752,38 @@ class FileBatch:
 
 
 def autotune_batch(batch: FileBatch, time_fn, cache_path: str | None = None):
-    """Pick the query launch size for a file batch by measurement.
-    Candidates: one grid pass over all records, or 4..32 waves per CU (256
-    CUs) looping over the records.  `time_fn(batch)` returns the time of one
-    batched run in ms.  The choice is cached by (record-count bucket, levels,
-    query count) in cache_path (JSON) when given."""
+    """Pick the query launch of a file batch by measurement: its size (one
+    grid pass over all records, or 4..32 waves per CU on 256 CUs looping over
+    the records) and its workgroup shape (64, 32 or 16 records per one-wave
+    workgroup).  The index launch has no free parameter (one workgroup per
+    16 KiB tile, levels = the queries' deepest level, the fewest that answer
+    them).  `time_fn(batch)` returns the time of one batched run in ms.  The
+    choice is cached by (record-count bucket, levels, query count) in
+    cache_path (JSON) when given."""
     import json as _json
-    key = f"lines~2^{max(batch.total_lines, 1).bit_length()}/levels{batch.levels}/q{len(batch.compiled)}"
+    key = (f"lines~2^{max(batch.total_lines, 1).bit_length()}/levels{batch.levels}"
+           f"/q{len(batch.compiled)}/shape")
     cache = {}
     if cache_path and os.path.exists(cache_path):
         with open(cache_path) as fh:
             cache = _json.load(fh)
     if key in cache:
         batch.grid_records = int(cache[key]["grid_records"])
+        batch.per_wave = int(cache[key]["per_wave"])
         return cache[key]
-    cands = [batch.total_lines] + [64 * 256 * w for w in (4, 8, 16, 32)]
     trials = []
-    for g in cands:
-        batch.grid_records = max(64, min(g, batch.total_lines))
-        trials.append((time_fn(batch), batch.grid_records))
+    for per_wave in (64, 32, 16):
+        waves = [None] + [256 * w for w in (4, 8, 16, 32)]  # None: one pass over all records
+        for w in waves:
+            g = batch.total_lines if w is None else w * per_wave
+            batch.grid_records = max(per_wave, min(g, batch.total_lines))
+            batch.per_wave = per_wave
+            trials.append((time_fn(batch), batch.grid_records, per_wave))
     best = min(trials)
-    batch.grid_records = best[1]
-    choice = {"grid_records": best[1], "ms": round(best[0], 4),
-              "trials": [{"grid_records": g, "ms": round(t, 4)} for t, g in trials]}
+    batch.grid_records, batch.per_wave = best[1], best[2]
+    choice = {"grid_records": best[1], "per_wave": best[2], "ms": round(best[0], 4),
+              "trials": [{"grid_records": g, "per_wave": p, "ms": round(t, 4)}
+                         for t, g, p in trials]}
     if cache_path:
         cache[key] = choice
         with open(cache_path, "w") as fh:

@@ -637,6 +637,21 @@ gpj_status gpj_batch_query(const uint8_t *d_in, uint64_t total_bytes,
                            int64_t max_records, const gpj_query_desc *queries,
                            int32_t num_queries, const gpj_batch_results *results,
                            void *stream) {
+  return gpj_batch_query_shaped(d_in, total_bytes, d_files, num_files, levels, d_string,
+                                d_leveled, d_newline, d_line_prefix, max_records, 64, queries,
+                                num_queries, results, stream);
+}
+
+gpj_status gpj_batch_query_shaped(const uint8_t *d_in, uint64_t total_bytes,
+                                  const gpj_batch_file *d_files, int32_t num_files,
+                                  int32_t levels, const uint64_t *d_string,
+                                  const uint64_t *d_leveled, const int64_t *d_newline,
+                                  const int64_t *d_line_prefix, int64_t max_records,
+                                  int32_t records_per_wave, const gpj_query_desc *queries,
+                                  int32_t num_queries, const gpj_batch_results *results,
+                                  void *stream) {
+  if (records_per_wave < 1 || records_per_wave > 64)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_query: records per wave 1..64");
   if (num_files <= 0 || !d_in || !d_files || !d_string || !d_leveled || !d_newline ||
       !d_line_prefix || !results)
     return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_query: empty batch or null pointer");
@@ -668,8 +683,9 @@ gpj_status gpj_batch_query(const uint8_t *d_in, uint64_t total_bytes,
   const gpj::FilesMode fm{reinterpret_cast<const gpj::BatchFile *>(d_files), num_files,
                           results->capacity, d_line_prefix,
                           {results->d_spans[0], results->d_spans[1]},
-                          {results->d_counts[0], results->d_counts[1]}};
-  const uint64_t blocks = std::min<uint64_t>(div_up((uint64_t)max_records, 64), 1u << 22);
+                          {results->d_counts[0], results->d_counts[1]}, records_per_wave};
+  const uint64_t blocks =
+      std::min<uint64_t>(div_up((uint64_t)max_records, (uint64_t)records_per_wave), 1u << 22);
   const int64_t stride = (int64_t)div_up(total_bytes, 64);
   const int deepest = deepest_level(queries, num_queries);
   // every file is at most total_bytes long (see run_batch for the two kernels)

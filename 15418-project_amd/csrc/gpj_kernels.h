@@ -110,6 +110,7 @@ struct FilesMode {
   const int64_t *line_prefix; // [num_files + 1] exclusive prefix of lines
   int64_t *spans[kBatchSlots];
   int64_t *counts[kBatchSlots];
+  int32_t per_wave;           // consecutive records per wave (1..64)
 };
 
 // Marks (PosMode) go to the record's count in result slot 0.

@@ -890,13 +890,15 @@ __global__ __launch_bounds__(64) void query_files_kernel(
     int64_t level_stride, int32_t num_levels, FilesMode fm, PosMode pm, QueryBatch batch) {
   typedef typename std::conditional<NARROW, int32_t, int64_t>::type P;
   const int64_t total = fm.line_prefix[fm.num_files];
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t wave_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  // one wave per block, fm.per_wave records per wave
+  const int64_t stride = (int64_t)gridDim.x * fm.per_wave;
+  const int64_t wave_base = (int64_t)blockIdx.x * fm.per_wave;
+  const int32_t lane = (int32_t)(threadIdx.x & 63);
   const int64_t span = NARROW ? pm.span : INT64_MAX;
   __shared__ WaveArrays<MAXLV, P> arrays;
   for (int64_t wb = wave_base; wb < total; wb += stride) {
-    const int64_t g = wb + (threadIdx.x & 63);
-    bool valid = g < total;
+    const int64_t g = wb + lane;
+    bool valid = lane < fm.per_wave && g < total;
     if (!NARROW && pm.fixup) {
       if (valid) valid = fm.counts[0][g] == kWideMark;
       if (!__ballot(valid)) continue; // (wave-uniform: lanes share wb)

@@ -400,10 +400,11 @@ def run_file_batch(args, gpj, hip, num_files: int, seed: int):
                                 b.levels, b.d_string.ptr, b.d_leveled.ptr, b.d_newline.ptr,
                                 b.d_prefix.ptr, b.d_scratch.ptr, stream), "batch index")
     hip.record(e[1], stream)
-    gpj.check(L.gpj_batch_query(b.d_in.ptr, b.n, b.d_files.ptr, b.num_files, b.levels,
-                                b.d_string.ptr, b.d_leveled.ptr, b.d_newline.ptr, b.d_prefix.ptr,
-                                b.grid_records, b.descs, len(compiled),
-                                ctypes.byref(b.results[0]), stream), "batch query")
+    gpj.check(L.gpj_batch_query_shaped(b.d_in.ptr, b.n, b.d_files.ptr, b.num_files, b.levels,
+                                       b.d_string.ptr, b.d_leveled.ptr, b.d_newline.ptr,
+                                       b.d_prefix.ptr, b.grid_records, b.per_wave, b.descs,
+                                       len(compiled), ctypes.byref(b.results[0]), stream),
+              "batch query")
     hip.record(e[2], stream)
     hip.sync_event(e[2])
     index_ms, query_ms = hip.elapsed_ms(e[0], e[1]), hip.elapsed_ms(e[1], e[2])

@@ -283,6 +283,18 @@ gpj_status gpj_batch_query(const uint8_t *d_in, uint64_t total_bytes,
                            int64_t max_records, const gpj_query_desc *queries,
                            int32_t num_queries, const gpj_batch_results *results,
                            void *stream);
+/* gpj_batch_query with the launch's workgroup shape chosen by the caller (no
+ * counterpart in the reference; the batch autotuner measures it): each
+ * one-wave workgroup takes records_per_wave (1..64) consecutive records at a
+ * time.  gpj_batch_query takes 64.  Results never depend on it. */
+gpj_status gpj_batch_query_shaped(const uint8_t *d_in, uint64_t total_bytes,
+                                  const gpj_batch_file *d_files, int32_t num_files,
+                                  int32_t levels, const uint64_t *d_string,
+                                  const uint64_t *d_leveled, const int64_t *d_newline,
+                                  const int64_t *d_line_prefix, int64_t max_records,
+                                  int32_t records_per_wave, const gpj_query_desc *queries,
+                                  int32_t num_queries, const gpj_batch_results *results,
+                                  void *stream);
 /* The same work as plain per-file launches (one index launch and one query
  * launch per file, like the reference's per-partition loop), writing the
  * same result layout; files: host array; line_prefix: host array of the

@@ -96,3 +96,21 @@ def test_graph_replays_equal_eager(gpu, batch):
     finally:
         batch.release_graph()
         batch.stream = old
+
+
+def test_launch_shape_does_not_change_results(batch):
+    # records per wave and the grid size the autotuner picks from
+    want = batch.fetch()
+    old = (batch.grid_records, batch.per_wave)
+    try:
+        for per_wave, grid in ((16, 256 * 4 * 16), (32, batch.total_lines), (64, 256 * 8 * 64)):
+            print(f"[batch] per_wave={per_wave} grid={grid}", flush=True)
+            batch.per_wave, batch.grid_records = per_wave, min(grid, batch.total_lines)
+            batch.launch()
+            got = batch.fetch()
+            for k in range(2):
+                np.testing.assert_array_equal(got[1][k], want[1][k])
+                np.testing.assert_array_equal(got[0][k], want[0][k])
+    finally:
+        batch.grid_records, batch.per_wave = old
+        batch.launch()
