@@ -561,9 +561,12 @@ __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w,
           __builtin_amdgcn_readfirstlane(wave_max_i32(live ? s.qp : INT32_MIN));
       if (__ballot(live && s.qp == m)) qpu = m;
     }
-    // decode the instruction at qpu (wave-uniform)
+    // decode the instruction at qpu (wave-uniform); a MARK_POSITION runs in
+    // the same dispatch as the instruction after it
     uint32_t qn = (uint32_t)qpu;
-    const uint32_t op = qn >= ir.len ? (uint32_t)kOffEnd : ir.byte(qn++);
+    uint32_t op = qn >= ir.len ? (uint32_t)kOffEnd : ir.byte(qn++);
+    const bool pre_mark = op == kMarkPosition;
+    if (pre_mark) op = qn >= ir.len ? (uint32_t)kOffEnd : ir.byte(qn++);
     int32_t arg = 0;
     uint32_t key_pos = 0;
     if (op == kMoveToKey || op == kMoveToIndex || op == kMoveToIndexReverse ||
@@ -586,7 +589,16 @@ __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w,
 
     s.qp = (int32_t)qn;
     bool fail = false;
-    switch (op) {
+    if (pre_mark) {
+      if (s.mi + 1 >= MAXLV) {
+        fail = true;
+      } else {
+        ++s.mi;
+        s.marked_pos.set(s.mi, s.idx);
+        s.marked_lvl.set(s.mi, (uint8_t)s.lvl);
+      }
+    }
+    if (!fail) switch (op) {
     case kStoreResult: {
       if (!variable && s.count >= cap) { fail = true; break; }
       const P lend = s.level_end.get(s.lvl);
