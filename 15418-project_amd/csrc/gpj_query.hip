@@ -11,15 +11,18 @@
 // negative key length or jump offset (which would move the bytecode position
 // backwards and never end).
 //
-// Execution model.  Each lane interprets one record, but the bytecode is
-// decoded once per wave: every step picks the lowest bytecode position held
-// by a live lane, reads that instruction through the scalar unit, and runs it
-// on exactly the lanes that sit there.  Lanes of one query normally move in
+// Execution model.  Each lane interprets one record (a wave takes up to 64
+// consecutive records), but the bytecode is decoded once per wave: every step
+// picks the bytecode position all live lanes share, or else the furthest
+// ahead, reads that instruction through the scalar unit, and runs it on
+// exactly the lanes that sit there.  Lanes of one query normally move in
 // lockstep, so an opcode costs one scalar decode instead of a divergent
-// switch per lane; lanes that drift apart (failed filters, element
-// iterations of different lengths) reconverge at the lowest position.  Each
-// lane still executes its own instruction sequence in order, so results are
-// those of a per-record sequential interpreter.  The bytecode is read from
+// switch per lane; lanes that run ahead (past a filter) finish their step and
+// loop back to where the others wait.  Some pairs always run in one dispatch
+// (a step and the MOVE_DOWN after it, a store and END, a string test and
+// RESET_POSITION, MARK_POSITION and what follows it).  Each lane still
+// executes its own instruction sequence in order, so results are those of a
+// per-record sequential interpreter.  The bytecode is read from
 // its device buffer directly (no constant-memory copy), so concurrent
 // launches with different queries are independent.
 #include <hip/hip_runtime.h>
