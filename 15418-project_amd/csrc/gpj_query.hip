@@ -529,6 +529,23 @@ __device__ __forceinline__ bool resume(Ctx<P> &c, LaneState<MAXLV, P> &s) {
   }
 }
 
+// STORE_RESULT: the current value (spaces trimmed) goes to the next result
+// entry.  False when fixed mode has no slot left (the path fails).
+template <int MAXLV, typename P>
+__device__ __forceinline__ bool store_result(Ctx<P> &c, LaneState<MAXLV, P> &s, int32_t cap,
+                                             bool variable, int64_t *res) {
+  if (!variable && s.count >= cap) return false;
+  const P lend = s.level_end.get(s.lvl);
+  P end_str = lend;
+  while (byte_at(c, s.idx) == ' ' && s.idx < lend) ++s.idx;
+  while (end_str > s.idx && byte_at(c, (P)(end_str - 1)) == ' ') --end_str;
+  if (s.count < cap) // one 16-byte store (the span array is 8-byte aligned)
+    *reinterpret_cast<i64x2a8 *>(res + (int64_t)s.count * 2) =
+        i64x2a8{base_of(c) + (int64_t)s.idx, base_of(c) + (int64_t)end_str};
+  ++s.count;
+  return true;
+}
+
 // Runs one query over each lane's record.  `live` marks lanes holding a
 // record (trimmed to line_start < line_end).  Fixed mode (reference): STORE i
 // writes slot i; any failure ends the record.  Variable mode (extension):
@@ -588,6 +605,12 @@ __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w,
     const bool then_end = op == kStoreResult && next_op == kEnd;
     const bool then_reset = op == kExpressionStringEquals && next_op == kResetPosition;
     if (then_down || then_end || then_reset) ++qn;
+    // a key or index step with its MOVE_DOWN followed by STORE_RESULT and END
+    // runs all four in one dispatch (not MOVE_TO_EACH: its frame keeps the
+    // position after its MOVE_DOWN)
+    const bool then_store = then_down && op != kMoveToEach && qn + 1 < ir.len &&
+                            ir.byte(qn) == kStoreResult && ir.byte(qn + 1) == kEnd;
+    if (then_store) qn += 2;
     if (!(live && s.qp == qpu)) continue;
 
     s.qp = (int32_t)qn;
@@ -602,19 +625,9 @@ __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w,
       }
     }
     if (!fail) switch (op) {
-    case kStoreResult: {
-      if (!variable && s.count >= cap) { fail = true; break; }
-      const P lend = s.level_end.get(s.lvl);
-      P end_str = lend;
-      while (byte_at(c, s.idx) == ' ' && s.idx < lend) ++s.idx;
-      while (end_str > s.idx && byte_at(c, (P)(end_str - 1)) == ' ') --end_str;
-      if (s.count < cap) // one 16-byte store (the span array is 8-byte aligned)
-        *reinterpret_cast<i64x2a8 *>(res + (int64_t)s.count * 2) =
-            i64x2a8{base_of(c) + (int64_t)s.idx, base_of(c) + (int64_t)end_str};
-      ++s.count;
-      if (then_end) fail = true; // END: the path is finished
+    case kStoreResult:
+      fail = !store_result(c, s, cap, variable, res) || then_end; // END: the path is finished
       break;
-    }
     case kMoveUp:
       if (s.lvl <= 0) { fail = true; break; }
       s.idx = s.level_end.get(s.lvl);
@@ -801,6 +814,10 @@ __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w,
     default: // END, unknown opcodes, running off the bytecode
       fail = true;
       break;
+    }
+    if (then_store && !fail && !c.wide) {
+      store_result(c, s, cap, variable, res);
+      fail = true; // END
     }
     if (c.wide) live = false; // the wide kernel redoes this record
     else if (fail) live = resume<MAXLV>(c, s);
