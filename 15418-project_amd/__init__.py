@@ -149,6 +149,8 @@ def lib():
         L.gpj_index_scratch_bytes.argtypes = [u64]
         L.gpj_index_scratch_bytes.restype = u64
         L.gpj_index_build_into.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp, vp]
+        L.gpj_index_status.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int32), vp]
+        L.gpj_debug_set_wait_limit.argtypes = [i64]
         L.gpj_debug_escape_quote.argtypes = [vp, u64, vp, vp, vp]
         L.gpj_debug_copy.argtypes = [vp, vp, u64, vp]
         L.gpj_debug_set_narrow_span.argtypes = [i64]
@@ -185,7 +187,8 @@ def lib():
         L.gpj_graph_destroy.argtypes = [vp]
         for name in ("gpj_alloc", "gpj_free", "gpj_h2d", "gpj_d2h", "gpj_stream_sync",
                      "gpj_device_sync", "gpj_index_build", "gpj_index_free",
-                     "gpj_index_build_into", "gpj_debug_escape_quote", "gpj_query_run",
+                     "gpj_index_build_into", "gpj_index_status", "gpj_debug_set_wait_limit",
+                     "gpj_debug_escape_quote", "gpj_query_run",
                      "gpj_query_run_ext", "gpj_query_run_batch", "gpj_query_compile",
                      "gpj_debug_copy", "gpj_row_bounds", "gpj_query_run_records",
                      "gpj_batch_tile_map", "gpj_batch_index", "gpj_batch_query",
@@ -372,6 +375,33 @@ class BuiltIndex:
             self.free()
         except Exception:
             pass
+
+
+def index_status(scratch: DeviceBuffer | int, n: int, stream=None) -> None:
+    """Raises DeviceError when the last index build over n bytes that used
+    this scratch failed (gpj_index_status: a tile's bounded wait gave up)."""
+    ptr = scratch.ptr if isinstance(scratch, DeviceBuffer) else int(scratch)
+    failed = ctypes.c_int32()
+    check(lib().gpj_index_status(ptr, int(n), ctypes.byref(failed), stream), "index status")
+    if failed.value:
+        raise DeviceError("index build: a tile's wait for its carry timed out")
+
+
+class wait_limit:
+    """Context manager: index tiles give up after `polls` unsuccessful polls
+    for their carry (default 2^24); 0 makes every waiting tile give up at once,
+    which drives the failure path without any kernel hanging."""
+
+    def __init__(self, polls: int):
+        self.polls = polls
+
+    def __enter__(self):
+        check(lib().gpj_debug_set_wait_limit(self.polls), "wait limit")
+        return self
+
+    def __exit__(self, *exc):
+        check(lib().gpj_debug_set_wait_limit(-1), "wait limit")
+        return False
 
 
 def build_index(dev: DeviceBuffer | int, n: int, levels: int, newline_capacity: int = 0,
@@ -731,10 +761,21 @@ class FileBatch:
                                        len(self.compiled), ctypes.byref(self.pf_results[0]),
                                        self.pf_scratch.ptr, self.stream), "per-file launches")
 
+    def check_status(self, per_file: bool = False):
+        """Raises DeviceError when the last index launch failed (a tile's
+        bounded wait for its carry gave up): the batched launch's flag, or a
+        per-file line count of -1."""
+        if per_file:
+            check(lib().gpj_stream_sync(self.stream))
+            if (self.pf_counts.to_host(np.int64, self.num_files, self.stream) < 0).any():
+                raise DeviceError("per-file index build: a tile's wait for its carry timed out")
+        else:
+            index_status(self.d_scratch, self.n, self.stream)
+
     def fetch(self, per_file: bool = False):
         """Host copies: (slot spans list, slot counts list), entries past a
         record's count set to -1."""
-        check(lib().gpj_stream_sync(self.stream))
+        self.check_status(per_file)
         r = self.pf_results if per_file else self.results
         out_sp, out_ct = [], []
         for k in range(2):
@@ -747,7 +788,7 @@ class FileBatch:
         return out_sp, out_ct
 
     def device_line_prefix(self) -> np.ndarray:
-        check(lib().gpj_stream_sync(self.stream))
+        self.check_status()
         return self.d_prefix.to_host(np.int64, self.num_files + 1, self.stream)
 
 

@@ -47,6 +47,12 @@ uint64_t tiles_for(uint64_t n) { return div_up(div_up(n, 64), gpj::kTileWords); 
 uint64_t scratch_bytes(uint64_t n) {
   return tiles_for(n) * 16 + 256;
 }
+uint32_t *scratch_counter(void *scratch, uint64_t n) {
+  return reinterpret_cast<uint32_t *>(static_cast<uint64_t *>(scratch) + 2 * tiles_for(n));
+}
+
+// Polls a waiting index tile makes before it gives up (tests lower it).
+std::atomic<uint32_t> g_wait_limit{gpj::kDefaultWaitLimit};
 
 __global__ void empty_index_kernel(int64_t *newline, int64_t cap, int64_t *num_lines) {
   if (cap > 0) newline[0] = 0;
@@ -63,8 +69,14 @@ gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int lev
   // single input: one extra workgroup (block 0) runs the prefix scan
   hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG>), dim3((uint32_t)(SEG ? tiles : tiles + 1)),
                      dim3(gpj::kTileWords), 0, st, d_in, n, words, levels, aligned, d_string,
-                     d_leveled, d_newline, cap, d_num_lines, desc, counter, tiles, seg);
+                     d_leveled, d_newline, cap, d_num_lines, desc, counter, tiles, seg,
+                     g_wait_limit.load());
   GPJ_HIP(hipGetLastError(), "index_kernel launch");
+  if (!SEG) { // a failed build reports -1 lines
+    hipLaunchKernelGGL(gpj::index_status_kernel, dim3(1), dim3(64), 0, st, counter + 1,
+                       d_num_lines);
+    GPJ_HIP(hipGetLastError(), "index_status_kernel launch");
+  }
   return GPJ_OK;
 }
 
@@ -231,6 +243,28 @@ gpj_status gpj_device_sync(void) {
 
 uint64_t gpj_index_scratch_bytes(uint64_t n) { return scratch_bytes(n); }
 
+gpj_status gpj_index_status(const void *d_scratch, uint64_t n, int32_t *failed, void *stream) {
+  if (!failed) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_status: null output");
+  *failed = 0;
+  if (n == 0) return GPJ_OK; // an empty input builds without scratch
+  if (!d_scratch) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_status: null scratch");
+  uint32_t flag = 0;
+  GPJ_HIP(hipMemcpyAsync(&flag, scratch_counter(const_cast<void *>(d_scratch), n) + 1, 4,
+                         hipMemcpyDeviceToHost, as_stream(stream)),
+          "copy index status");
+  GPJ_HIP(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
+  *failed = flag != 0;
+  return GPJ_OK;
+}
+
+gpj_status gpj_debug_set_wait_limit(int64_t polls) {
+  if (polls < 0) polls = gpj::kDefaultWaitLimit;
+  if (polls > (int64_t)gpj::kDefaultWaitLimit)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "wait limit: negative (default) or 0 .. 2^24 polls");
+  g_wait_limit.store((uint32_t)polls);
+  return GPJ_OK;
+}
+
 gpj_status gpj_debug_copy(void *d_dst, const void *d_src, uint64_t bytes, void *stream) {
   if (bytes % 16 != 0 || (reinterpret_cast<uintptr_t>(d_dst) & 15u) ||
       (reinterpret_cast<uintptr_t>(d_src) & 15u))
@@ -349,9 +383,8 @@ gpj_status gpj_index_build(const uint8_t *d_in, uint64_t n, int32_t levels,
   GPJ_TRY(hipStreamSynchronize(st), "index build");
   if (n > 0) {
     uint32_t failed = 0;
-    const uint32_t *flag =
-        reinterpret_cast<const uint32_t *>(static_cast<uint64_t *>(scratch) + 2 * tiles_for(n)) + 1;
-    GPJ_TRY(hipMemcpy(&failed, flag, 4, hipMemcpyDeviceToHost), "copy index status");
+    GPJ_TRY(hipMemcpy(&failed, scratch_counter(scratch, n) + 1, 4, hipMemcpyDeviceToHost),
+            "copy index status");
     if (failed) {
       cleanup();
       return fail(GPJ_ERR_DEVICE, "index build: tile look-back timed out");
@@ -490,8 +523,10 @@ gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index 
                                const gpj_query_desc *queries, int32_t num_queries,
                                int64_t max_lines, void *stream) {
   if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null index");
+  // index->num_lines: the entries the newline buffer holds
   return run_batch(d_in, n, index, index->d_newline, index->d_num_lines,
-                   gpj::RecordMode{0, 0, 64}, queries, num_queries, max_lines, stream);
+                   gpj::RecordMode{0, 0, 64, index->num_lines}, queries, num_queries, max_lines,
+                   stream);
 }
 
 uint64_t gpj_row_bounds_scratch_bytes(int64_t level_size) {
@@ -531,8 +566,10 @@ gpj_status gpj_query_run_records(const uint8_t *d_in, uint64_t n, const gpj_inde
   if (index->num_levels < 2)
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "array records need an index with at least 2 leveled rows");
-  return run_batch(d_in, n, index, d_bounds, d_num_bounds, gpj::RecordMode{1, 1, 64}, queries,
-                   num_queries, max_records, stream);
+  // the bounds buffer holds at least max_records + 1 entries
+  return run_batch(d_in, n, index, d_bounds, d_num_bounds,
+                   gpj::RecordMode{1, 1, 64, max_records + 1}, queries, num_queries, max_records,
+                   stream);
 }
 
 gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,
@@ -627,6 +664,10 @@ gpj_status gpj_batch_index(const uint8_t *d_in, uint64_t total_bytes,
                      reinterpret_cast<uint64_t *>(d_line_prefix), (uint64_t)num_files,
                      d_line_prefix + num_files, (int64_t)0);
   GPJ_HIP(hipGetLastError(), "line prefix launch");
+  // a failed build reports -1 total lines (its queries then run no record)
+  hipLaunchKernelGGL(gpj::index_status_kernel, dim3(1), dim3(64), 0, st, counter + 1,
+                     d_line_prefix + num_files);
+  GPJ_HIP(hipGetLastError(), "index_status_kernel launch");
   return GPJ_OK;
 }
 

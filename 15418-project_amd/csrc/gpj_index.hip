@@ -207,7 +207,6 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 constexpr uint64_t kFlagShift = 62;
 constexpr uint64_t kFlagAggregate = 1;
 constexpr uint64_t kFlagInclusive = 2;
-constexpr uint32_t kMaxSpins = 1u << 24;
 // Tables in registers are unpacked: bits 0..31 the output state of input s
 // in byte s, bits 32..63 the depth delta of input s in byte s, so that
 // composing two is two byte permutes and a bytewise add.
@@ -316,7 +315,8 @@ __device__ __forceinline__ void scan_step(uint64_t &tab, uint32_t &nl) {
 // instead of a look-back per tile.  Every wave takes the same branches (counts come
 // through LDS), so the barriers always match.
 __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tiles,
-                           uint32_t *error_flag, uint32_t lane, uint32_t wv) {
+                           uint32_t *error_flag, uint32_t max_spins, uint32_t lane,
+                           uint32_t wv) {
   __shared__ uint32_t s_ready[kScanWaves];
   __shared__ uint64_t s_tab[kScanWaves];
   __shared__ uint32_t s_nl[kScanWaves];
@@ -354,11 +354,12 @@ __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tile
     if (ready > num_tiles - next) ready = num_tiles - next;
     if (ready == 0) {
       lds_barrier(); // s_ready is rewritten next round
-      if (++spins > kMaxSpins) {
+      if (spins >= max_spins) {
         if (wv == 0 && lane == 0)
           __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
       }
+      ++spins;
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
@@ -435,7 +436,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
     uint64_t *__restrict__ leveled_out, int64_t *__restrict__ newline_out,
     int64_t newline_cap, int64_t *__restrict__ num_lines_out,
     uint64_t *__restrict__ desc, uint32_t *__restrict__ tile_counter,
-    uint64_t num_tiles, BatchSeg seg) {
+    uint64_t num_tiles, BatchSeg seg, uint32_t max_spins) {
   uint32_t *const error_flag = tile_counter + 1;
   __shared__ uint32_t s_tile;
   __shared__ uint8_t s_bsinfo[kTileWords];  // bit0 trailing parity, bit1 all-backslash
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
   //    costs ~11 ns per tile in series on MI355X.)
   if (!SEG) {
     if (blockIdx.x == 0) {
-      scan_tiles(desc, desc + num_tiles, num_tiles, error_flag, lane, wv);
+      scan_tiles(desc, desc + num_tiles, num_tiles, error_flag, max_spins, lane, wv);
       return;
     }
     if (t == 0) s_tile = blockIdx.x - 1;
@@ -632,6 +633,12 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
       uint64_t d = 0;
       uint64_t acc = kIdentityTable, acc_nl = 0;
       for (;;) {
+        if (spins >= max_spins) { // bounded wait: give up, flag the failure
+          if (lane == 0)
+            __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          d = kFlagInclusive << kFlagShift;
+          break;
+        }
         uint64_t v = 0;
         if (lane < need) v = desc_load(&desc[g0 + lane]);
         else if (lane == 63) v = g0 > 0 ? desc_load(&inc[g0 - 1]) : kInitialInclusive;
@@ -647,14 +654,10 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
           }
           break;
         }
-        if (++spins > kMaxSpins ||
-            ((spins & 1023u) == 0 &&
-             __hip_atomic_load(error_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
-          if (lane == 0)
-            __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          d = kFlagInclusive << kFlagShift;
-          break;
-        }
+        // another tile's failure ends this wait as well
+        if ((++spins & 1023u) == 0 &&
+            __hip_atomic_load(error_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+          spins = max_spins;
         __builtin_amdgcn_s_sleep(1);
       }
       excl = tbl_apply(acc, acc_nl, decode_inclusive(d));
@@ -673,9 +676,10 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
       // poll from every waiting tile slows them all (3 % of kernel time).
       if (base >= (int64_t)first_tile) {
         for (;;) {
+          if (spins >= max_spins) break; // the window loop flags the failure
           const uint64_t d0 = __builtin_amdgcn_readfirstlane((int)(uint32_t)(desc_load(&desc[base]) >> 32));
           if ((d0 >> (kFlagShift - 32)) != 0) break;
-          if (++spins > kMaxSpins) break;
+          ++spins;
           __builtin_amdgcn_s_sleep(1);
         }
       }
@@ -684,19 +688,20 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
         uint64_t d = 0;
         uint32_t first_incl = 64;
         for (;;) {
-          d = (p >= (int64_t)first_tile) ? desc_load(&desc[p]) : kInitialInclusive;
-          const uint64_t incl = __ballot((d >> kFlagShift) == kFlagInclusive);
-          const uint64_t zero = __ballot((d >> kFlagShift) == 0);
-          first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
-          const uint64_t needed = first_incl >= 63 ? ~0ull : ((2ull << first_incl) - 1ull);
-          if ((zero & needed) == 0) break;
-          if (++spins > kMaxSpins) {
+          if (spins >= max_spins) { // bounded wait: give up, flag the failure
             if (lane == 0)
               __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             d = kFlagInclusive << kFlagShift;
             first_incl = 0;
             break;
           }
+          d = (p >= (int64_t)first_tile) ? desc_load(&desc[p]) : kInitialInclusive;
+          const uint64_t incl = __ballot((d >> kFlagShift) == kFlagInclusive);
+          const uint64_t zero = __ballot((d >> kFlagShift) == 0);
+          first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+          const uint64_t needed = first_incl >= 63 ? ~0ull : ((2ull << first_incl) - 1ull);
+          if ((zero & needed) == 0) break;
+          ++spins;
           __builtin_amdgcn_s_sleep(1);
         }
         // tables of lanes [0, first_incl), identity beyond; only as many
@@ -831,16 +836,22 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
   }
 }
 
-template __global__ void index_kernel<2, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
-template __global__ void index_kernel<2, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
-template __global__ void index_kernel<4, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
-template __global__ void index_kernel<4, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
-template __global__ void index_kernel<8, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
-template __global__ void index_kernel<8, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
-template __global__ void index_kernel<16, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
-template __global__ void index_kernel<16, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
-template __global__ void index_kernel<22, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
-template __global__ void index_kernel<22, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg);
+template __global__ void index_kernel<2, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<2, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<4, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<4, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<8, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<8, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<16, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<16, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<22, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<22, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+
+__global__ void index_status_kernel(const uint32_t *error_flag, int64_t *lines_out) {
+  if (threadIdx.x == 0 && __hip_atomic_load(error_flag, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT) != 0)
+    *lines_out = -1;
+}
 
 // ---- exact newline index (fallback when the capacity hint was too small) ---
 

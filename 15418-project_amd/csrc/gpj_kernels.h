@@ -34,13 +34,25 @@ struct BatchSeg {
   int64_t *line_counts;      // [num_files] lines per file
 };
 
+// Every wait of the index kernel on another workgroup is bounded: after
+// max_spins unsuccessful polls (kDefaultWaitLimit unless a test lowers it) the
+// waiting tile sets the failure flag (tile_counter[1]), goes on with a dummy
+// carry and exits; index_status_kernel then turns the flag into a line count
+// of -1.  No wait is unbounded, so every launch ends.
+constexpr uint32_t kDefaultWaitLimit = 1u << 24;
+
 template <int LMAX, bool SEG>
 __global__ void index_kernel(const uint8_t *in, uint64_t n, uint64_t words,
                              int levels, int aligned, uint64_t *string_out,
                              uint64_t *leveled_out, int64_t *newline_out,
                              int64_t newline_cap, int64_t *num_lines_out,
                              uint64_t *desc, uint32_t *tile_counter,
-                             uint64_t num_tiles, BatchSeg seg);
+                             uint64_t num_tiles, BatchSeg seg, uint32_t max_spins);
+
+// After an index launch: when its failure flag is set, *lines_out = -1 (the
+// single-input line count, or the total of a batch's line prefix), so every
+// stream-ordered reader of the count sees the failure.
+__global__ void index_status_kernel(const uint32_t *error_flag, int64_t *lines_out);
 
 __global__ void newline_count_kernel(const uint8_t *in, uint64_t n, int aligned,
                                      uint64_t *tile_counts);
@@ -71,12 +83,15 @@ struct QueryBatch {
 
 // How records are cut: lines of a newline index (level_base 0, drop_last 0)
 // or top-level array elements between row-0 bounds (level_base 1,
-// drop_last 1); and how many consecutive records one wave takes at a time
-// (per_wave, 1..64; lanes past it idle).
+// drop_last 1); how many consecutive records one wave takes at a time
+// (per_wave, 1..64; lanes past it idle); and how many entries the start
+// array holds (starts_cap: a device count above it, e.g. a newline count
+// beyond the buffer's capacity, is clamped, so no entry past it is read).
 struct RecordMode {
   int32_t level_base;
   int32_t drop_last;
   int32_t per_wave;
+  int64_t starts_cap;
 };
 
 // How the query kernels hold positions.  The narrow kernels (NARROW = true)

@@ -860,7 +860,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
     const uint64_t *__restrict__ leveled, int64_t level_size, int32_t num_levels,
     RecordMode mode, PosMode pm, QueryBatch batch) {
   typedef typename std::conditional<NARROW, int32_t, int64_t>::type P;
-  const int64_t all = *num_lines;
+  const int64_t counted = *num_lines; // -1 after a failed index build: no record
+  const int64_t all = counted < mode.starts_cap ? counted : mode.starts_cap;
   const int64_t avail = all - mode.drop_last;
   const int64_t lines = avail < max_lines ? avail : max_lines;
   // one wave per block, mode.per_wave records per wave

@@ -45,6 +45,7 @@ struct Args {
   int capacity = 8;
   int verify = -1; // requests of the last rate checked against run_alone (-1 = all)
   int threads = 16;
+  int64_t wait_limit = -1; // index tiles' carry-wait polls (-1: default)
 };
 
 std::vector<double> parse_list(const std::string &s) {
@@ -84,6 +85,7 @@ Args parse(int argc, char **argv) {
     else if (k == "--capacity") a.capacity = std::stoi(val());
     else if (k == "--verify") a.verify = std::stoi(val());
     else if (k == "--threads") a.threads = std::stoi(val());
+    else if (k == "--debug-wait-limit") a.wait_limit = std::stoll(val());
     else throw std::invalid_argument("unknown option " + k);
   }
   return a;
@@ -118,6 +120,7 @@ int main(int argc, char **argv) {
   try {
     const Args a = parse(argc, argv);
     if (!gpj_device_available()) throw std::runtime_error("no HIP device");
+    if (gpj_debug_set_wait_limit(a.wait_limit) < 0) throw std::runtime_error(gpj_last_error());
     std::vector<gpjson::serve::QuerySpec> table;
     for (const auto &q : kQueries) table.push_back(gpjson::serve::compile_query_spec(q));
 

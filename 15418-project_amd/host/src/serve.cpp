@@ -3,6 +3,7 @@
 #include "gpjson/serve/server.hpp"
 
 #include "gpj_capi.h"
+#include "gpjson/device/device.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -53,6 +54,13 @@ bool uses_extension_opcodes(const std::vector<uint8_t> &ir) {
     else if (op == 5 || op == 6) (void)varint();
   }
   return false;
+}
+
+// A line count of -1 marks an index build whose bounded carry wait gave up
+// (gpj_index_build_into); its indices and results are not valid.
+void check_lines(int64_t lines) {
+  if (lines < 0)
+    throw error::device::DeviceError("index build: a tile's wait for its carry timed out");
 }
 
 double now_s(std::chrono::steady_clock::time_point t0) {
@@ -455,6 +463,7 @@ struct Server::Impl {
           progress = true;
           const Request &r = reqs[s.req];
           const int64_t lines = *s.h_lines;
+          check_lines(lines);
           if ((uint64_t)lines > s.doc->nl_cap) {
             // capacity hint too small: rebuild this document exactly, rerun
             ++stats.retries;
@@ -519,6 +528,7 @@ struct Server::Impl {
     uint64_t scratch_bytes = 0;
     ok(gpj_stream_create(&stream), "stream");
     uint64_t cap = newline_hint(r.doc_bytes);
+    int64_t counted = 0;
     for (;;) {
       Request rr = r;
       exact_caps[r.doc_id] = cap;
@@ -530,7 +540,8 @@ struct Server::Impl {
       enqueue_queries(r, d, d_results, stream);
       int64_t lines = 0;
       ok(gpj_d2h(&lines, d->d_lines, 8, stream), "line count");
-      if ((uint64_t)lines <= d->nl_cap) {
+      counted = lines;
+      if (lines >= 0 && (uint64_t)lines <= d->nl_cap) {
         std::vector<uint8_t> h(host_results_bytes(r, lines));
         enqueue_copy_back(r, d, d_results, lines, h.data(), stream);
         ok(gpj_stream_sync(stream), "copy back");
@@ -539,11 +550,12 @@ struct Server::Impl {
       (void)gpj_free_async(d_results, stream);
       release_doc(d, stream);
       ok(gpj_stream_sync(stream), "release");
-      if ((uint64_t)lines <= cap) break;
+      if (lines < 0 || (uint64_t)lines <= cap) break;
       cap = (uint64_t)lines;
     }
     (void)gpj_free(scratch);
     (void)gpj_stream_destroy(stream);
+    check_lines(counted);
     return res;
   }
 

@@ -231,6 +231,7 @@ def main():
     # way, orig_index_builder.cu:325-330) and size the query grids to it
     step()
     gpj.check(L.gpj_stream_sync(stream))
+    gpj.index_status(d_scratch, n, stream)  # raises if a tile's carry wait gave up
     num_lines = int(d_numlines.to_host(np.int64, 1)[0])
     nl_entries = num_lines
     if num_lines > nl_cap:
@@ -270,6 +271,8 @@ def main():
         query_ms.append(hip.elapsed_ms(a, b))
     run_queries(descs, len(compiled))
     gpj.check(L.gpj_stream_sync(stream))
+    # the last timed build succeeded (each build clears and re-arms the flag)
+    gpj.index_status(d_scratch, n, stream)
 
     # roofline per kernel: algorithmic bytes per launch over the launch time
     # measured with HIP events on the launch stream

@@ -117,12 +117,30 @@ uint64_t gpj_index_scratch_bytes(uint64_t n);
  * words, d_newline: newline_capacity entries, d_num_lines: 1 int64, d_scratch:
  * gpj_index_scratch_bytes(n) bytes.  If the input has more than
  * newline_capacity lines, *d_num_lines still receives the true count and only
- * the first newline_capacity entries are written; the caller checks. */
+ * the first newline_capacity entries are written; the caller checks.  If the
+ * build failed (a tile's bounded wait for its predecessors' carry gave up; the
+ * reference checks every launch instead, orig_index_builder.cu:107,115),
+ * *d_num_lines receives -1 and gpj_index_status reports it. */
 gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
                                 uint64_t *d_string, uint64_t *d_leveled,
                                 int64_t *d_newline, int64_t newline_capacity,
                                 int64_t *d_num_lines, void *d_scratch,
                                 void *stream);
+
+/* Whether the last index build that used d_scratch (gpj_index_build_into
+ * over n bytes, or gpj_batch_index over n total bytes) failed: *failed = 1
+ * when a tile's bounded wait gave up, so its indices are not valid (the
+ * stream-ordered line count reads -1 then).  Synchronizes `stream`.  Replaces
+ * the error check after each reference launch
+ * (src/gpjson/index/orig_index_builder.cu:107,115). */
+gpj_status gpj_index_status(const void *d_scratch, uint64_t n, int32_t *failed, void *stream);
+
+/* Debug (no counterpart in the reference): the number of polls a waiting
+ * index tile makes before it gives up and flags the build as failed; negative
+ * restores the default (2^24).  0 makes every tile that would wait give up at
+ * once, so tests drive the failure path without any kernel hanging.
+ * Process-wide; set it while no index build runs. */
+gpj_status gpj_debug_set_wait_limit(int64_t polls);
 
 /* Debug: escape and unescaped-quote bitmaps (reference kernels
  * src/gpjson/index/kernels/orig/uncombined/escape-index.cu and
@@ -152,8 +170,11 @@ gpj_status gpj_debug_set_narrow_span(int64_t bytes);
  * over every line of the index.  d_spans receives int64
  * [max_lines][num_results][2] partition-local {start, end} pairs, -1 for no
  * match; lines beyond *index->d_num_lines are not touched.  max_lines bounds
- * the grid (use index->num_lines when known).  Stream-ordered; allocates
- * nothing. */
+ * the grid (use index->num_lines when known).  index->num_lines is the number
+ * of entries d_newline holds (the exact count or the buffer's capacity): a
+ * device count above it (an overflowed capacity) is clamped to it, so no
+ * entry past the buffer is read, and a count of -1 (a failed index build)
+ * runs no line.  Stream-ordered; allocates nothing. */
 gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                          const uint8_t *d_ir, uint32_t ir_len,
                          int32_t num_results, int64_t max_lines,
@@ -259,7 +280,9 @@ uint64_t gpj_batch_scratch_bytes(uint64_t total_bytes);
  * GPJ_BATCH_ALIGN rounded up); tiles no file covers get the preceding file. */
 gpj_status gpj_batch_tile_map(const gpj_batch_file *files, int32_t num_files,
                               uint64_t total_bytes, uint32_t *tile_file);
-/* One index launch over every file (d_files, d_tile_file: device arrays). */
+/* One index launch over every file (d_files, d_tile_file: device arrays).
+ * A failed build writes -1 into d_line_prefix[num_files] (the total line
+ * count) and gpj_index_status(d_scratch, total_bytes, ...) reports it. */
 gpj_status gpj_batch_index(const uint8_t *d_in, uint64_t total_bytes,
                            const gpj_batch_file *d_files, int32_t num_files,
                            const uint32_t *d_tile_file, int32_t levels, uint64_t *d_string,
