@@ -35,6 +35,7 @@ COMPILE_EXTENSIONS = 1
 MAX_QUERY_IR_BYTES = 4096
 MAX_QUERY_LEVELS = 16
 MAX_INDEX_LEVELS = 22
+STATIC_MAX_BYTES = 16 << 30  # GPJ_STATIC_MAX_BYTES: inputs of gpj_index_build_static
 
 
 class GpJSONError(Exception):
@@ -150,6 +151,7 @@ def lib():
         L.gpj_index_scratch_bytes.restype = u64
         L.gpj_index_build_into.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp, vp]
         L.gpj_index_status.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int32), vp]
+        L.gpj_index_build_static.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp]
         L.gpj_debug_set_wait_limit.argtypes = [i64]
         L.gpj_debug_escape_quote.argtypes = [vp, u64, vp, vp, vp]
         L.gpj_debug_copy.argtypes = [vp, vp, u64, vp]
@@ -188,6 +190,7 @@ def lib():
         for name in ("gpj_alloc", "gpj_free", "gpj_h2d", "gpj_d2h", "gpj_stream_sync",
                      "gpj_device_sync", "gpj_index_build", "gpj_index_free",
                      "gpj_index_build_into", "gpj_index_status", "gpj_debug_set_wait_limit",
+                     "gpj_index_build_static",
                      "gpj_debug_escape_quote", "gpj_query_run",
                      "gpj_query_run_ext", "gpj_query_run_batch", "gpj_query_compile",
                      "gpj_debug_copy", "gpj_row_bounds", "gpj_query_run_records",

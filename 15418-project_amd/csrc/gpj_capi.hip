@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "gpj_capi.h"
@@ -316,6 +317,62 @@ gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
   if (levels <= 16) GPJ_DISPATCH(16);
   GPJ_DISPATCH(22);
 #undef GPJ_DISPATCH
+}
+
+} // extern "C"
+
+// ---- index build without allocation --------------------------------------
+// The index scratch of gpj_index_build_static (tile descriptors, the spare
+// word and the failure flag) for inputs up to GPJ_STATIC_MAX_BYTES: every GPU
+// the library runs on holds its own copy until the process ends.  In host code
+// this name is not a device address; the address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+static __device__ __align__(256) uint8_t scratch[GPJ_STATIC_SCRATCH_BYTES];
+static_assert(GPJ_STATIC_SCRATCH_BYTES >= (GPJ_STATIC_MAX_BYTES / gpj::kTileBytes) * 16 + 256,
+              "static scratch covers the largest input");
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+void *g_static_scratch[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (looked up on the first call on
+// that device, with the device current).
+gpj_status static_scratch(void **out) {
+  int dev = 0;
+  GPJ_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(GPJ_ERR_DEVICE, "gpj_index_build_static: device id beyond 63");
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_scratch[dev]) {
+    void *p = nullptr;
+    GPJ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(GPJ_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+    g_static_scratch[dev] = p;
+  }
+  *out = g_static_scratch[dev];
+  return GPJ_OK;
+}
+} // namespace
+
+extern "C" {
+
+gpj_status gpj_index_build_static(const uint8_t *d_in, uint64_t n, int32_t levels,
+                                  uint64_t *d_string, uint64_t *d_leveled,
+                                  int64_t *d_newline, int64_t newline_capacity,
+                                  int64_t *d_num_lines, void *stream) {
+  if (n > GPJ_STATIC_MAX_BYTES)
+    return fail(GPJ_ERR_CAPACITY, "gpj_index_build_static: input of " + std::to_string(n) +
+                                      " bytes is above the static scratch's limit of " +
+                                      std::to_string(GPJ_STATIC_MAX_BYTES) +
+                                      " bytes; use gpj_index_build_into");
+  void *ptr = nullptr;
+  const gpj_status st = static_scratch(&ptr);
+  if (st != GPJ_OK) return st;
+  // the same kernels as gpj_index_build_into; index_clear_kernel (a fill
+  // kernel) resets the descriptors and flags at the start of every call
+  return gpj_index_build_into(d_in, n, levels, d_string, d_leveled, d_newline,
+                              newline_capacity, d_num_lines, ptr, stream);
 }
 
 gpj_status gpj_index_free(gpj_index *index) {

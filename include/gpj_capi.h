@@ -127,6 +127,23 @@ gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
                                 int64_t *d_num_lines, void *d_scratch,
                                 void *stream);
 
+/* gpj_index_build_into without a scratch argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops and
+ * harnesses that forbid hipMalloc).  It has no counterpart in the reference.
+ * The scratch is a static device array of the library,
+ * GPJ_STATIC_SCRATCH_BYTES (16 MiB + 256 B) on every GPU it runs on, held
+ * until the process ends; inputs above GPJ_STATIC_MAX_BYTES (16 GiB) return
+ * GPJ_ERR_CAPACITY.  Calls nothing that allocates.  Every call on one GPU
+ * shares that GPU's array and resets it first, so calls that overlap on one
+ * GPU (on different streams or threads) race: serialize them.  Results and
+ * failure reporting (-1 lines) are those of gpj_index_build_into. */
+#define GPJ_STATIC_MAX_BYTES (16ull << 30)
+#define GPJ_STATIC_SCRATCH_BYTES ((16ull << 20) + 256)
+gpj_status gpj_index_build_static(const uint8_t *d_in, uint64_t n, int32_t levels,
+                                  uint64_t *d_string, uint64_t *d_leveled,
+                                  int64_t *d_newline, int64_t newline_capacity,
+                                  int64_t *d_num_lines, void *stream);
+
 /* Whether the last index build that used d_scratch (gpj_index_build_into
  * over n bytes, or gpj_batch_index over n total bytes) failed: *failed = 1
  * when a tile's bounded wait gave up, so its indices are not valid (the
