@@ -91,6 +91,14 @@ class QueryDescT(ctypes.Structure):
     ]
 
 
+class DenseResultT(ctypes.Structure):
+    _fields_ = [
+        ("d_offsets", ctypes.c_void_p),
+        ("d_spans", ctypes.c_void_p),
+        ("capacity", ctypes.c_int64),
+    ]
+
+
 class BatchFileT(ctypes.Structure):
     _fields_ = [
         ("offset", ctypes.c_uint64),
@@ -163,6 +171,14 @@ def lib():
                                           i32, i64, vp]
         L.gpj_query_compile.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint8), u32,
                                         u32p, u32p, u32p, u32]
+        L.gpj_query_dense_scratch_bytes.argtypes = [i64, i32, i64]
+        L.gpj_query_dense_scratch_bytes.restype = u64
+        L.gpj_query_run_dense.argtypes = [vp, u64, ctypes.POINTER(IndexT),
+                                          ctypes.POINTER(QueryDescT), i32, i64,
+                                          ctypes.POINTER(DenseResultT), vp, vp]
+        L.gpj_query_run_records_dense.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, vp,
+                                                  ctypes.POINTER(QueryDescT), i32, i64,
+                                                  ctypes.POINTER(DenseResultT), vp, vp]
         L.gpj_row_bounds_scratch_bytes.argtypes = [i64]
         L.gpj_row_bounds_scratch_bytes.restype = u64
         L.gpj_row_bounds.argtypes = [ctypes.POINTER(IndexT), vp, i64, vp, vp, vp]
@@ -194,6 +210,7 @@ def lib():
                      "gpj_debug_escape_quote", "gpj_query_run",
                      "gpj_query_run_ext", "gpj_query_run_batch", "gpj_query_compile",
                      "gpj_debug_copy", "gpj_row_bounds", "gpj_query_run_records",
+                     "gpj_query_run_dense", "gpj_query_run_records_dense",
                      "gpj_batch_tile_map", "gpj_batch_index", "gpj_batch_query",
                      "gpj_batch_query_shaped",
                      "gpj_batch_per_file", "gpj_graph_begin", "gpj_graph_end",
@@ -484,6 +501,71 @@ def run_query_batch(dev: DeviceBuffer | int, n: int, index: BuiltIndex, compiled
             spans[np.arange(cap)[None, :] >= counts[:, None]] = -1
         out.append((spans, counts))
     return out
+
+
+def run_query_dense(dev: DeviceBuffer | int, n: int, index: BuiltIndex, compiled,
+                    capacity: int | None = None, bounds=None, num_bounds: int = 0,
+                    max_records: int | None = None, stream=None, rerun: bool = True):
+    """Dense results (gpj_query_run_dense, or gpj_query_run_records_dense
+    over top-level array elements when `bounds` is given): per query, CSR
+    offsets int64 [records + 1] and spans int64 [total, 2] in record order.
+    `capacity` spans per query (default: 2 per record); when a total exceeds
+    it, the call reruns with the total (rerun=False returns the truncated
+    output and the offsets instead)."""
+    L = lib()
+    ptr = dev.ptr if isinstance(dev, DeviceBuffer) else int(dev)
+    if bounds is None:
+        records = index.num_lines if max_records is None else int(max_records)
+    else:
+        records = max(num_bounds - 1, 0) if max_records is None else int(max_records)
+    nq = len(compiled)
+    descs = (QueryDescT * max(nq, 1))()
+    irs = []
+    for i, (ir, depth, slots) in enumerate(compiled):
+        d_ir = DeviceBuffer.from_host(np.frombuffer(ir, dtype=np.uint8), stream)
+        irs.append(d_ir)
+        descs[i] = QueryDescT(d_ir.ptr, len(ir), slots, int(uses_extension_opcodes(ir)),
+                              int(depth), None, None)
+    caps = [2 * records + 1 if capacity is None else int(capacity)] * nq
+    d_m = None
+    if bounds is not None:
+        d_m = DeviceBuffer.from_host(np.array([num_bounds], dtype=np.int64), stream)
+    while True:
+        offs = [DeviceBuffer((records + 1) * 8) for _ in range(nq)]
+        spans = [DeviceBuffer(max(c * 16, 16)) for c in caps]
+        res = (DenseResultT * max(nq, 1))()
+        for i in range(nq):
+            res[i] = DenseResultT(offs[i].ptr, spans[i].ptr, caps[i])
+        scratch = DeviceBuffer(L.gpj_query_dense_scratch_bytes(records, nq, sum(caps)))
+        if bounds is None:
+            check(L.gpj_query_run_dense(ptr, int(n), ctypes.byref(index.raw), descs, nq, records,
+                                        res, scratch.ptr, stream), "dense query")
+        else:
+            d_b = bounds.ptr if isinstance(bounds, DeviceBuffer) else int(bounds)
+            check(L.gpj_query_run_records_dense(ptr, int(n), ctypes.byref(index.raw), d_b,
+                                                d_m.ptr, descs, nq, records, res, scratch.ptr,
+                                                stream), "dense record query")
+        check(L.gpj_stream_sync(stream))
+        out = [o.to_host(np.int64, records + 1, stream) for o in offs]
+        totals = [int(o[-1]) for o in out]
+        if rerun and any(t > c for t, c in zip(totals, caps)):
+            caps = [max(t, c) for t, c in zip(totals, caps)]
+            continue
+        return [(o, sp.to_host(np.int64, 2 * min(t, c), stream).reshape(-1, 2))
+                for o, sp, t, c in zip(out, spans, totals, caps)]
+
+
+def dense_from_slots(spans: np.ndarray, counts: np.ndarray | None = None):
+    """CSR form (offsets, spans [total, 2]) of per-record results
+    [records][cap][2]: the first counts[r] entries of record r (fixed mode:
+    its stored slots, those with start >= 0)."""
+    if counts is None:
+        counts = (spans[:, :, 0] >= 0).sum(axis=1)
+    counts = np.asarray(counts, dtype=np.int64)
+    offsets = np.zeros(counts.size + 1, dtype=np.int64)
+    np.cumsum(counts, out=offsets[1:])
+    keep = np.arange(spans.shape[1])[None, :] < counts[:, None]
+    return offsets, spans[keep].reshape(-1, 2)
 
 
 def compile_element_query(query: str):

@@ -502,31 +502,91 @@ int deepest_level(const gpj_query_desc *queries, int32_t num_queries) {
 }
 
 // Shared by the line and the top-level-array record entry points.
+// Scratch layout of dense results (see gpj_query_dense_scratch_bytes): region
+// cursors, tmp_off per query, scan chunk sums per query, staging regions.
+struct DenseLayout {
+  uint64_t tmp_off, sums, tmp, bytes;
+  int64_t blocks, region;
+};
+DenseLayout dense_layout(int64_t max_records, int32_t num_queries, int64_t total_capacity) {
+  DenseLayout l{};
+  const uint64_t rec = (uint64_t)std::max<int64_t>(max_records, 0);
+  const uint64_t nq = (uint64_t)std::max(num_queries, 0);
+  const uint64_t cap = (uint64_t)std::max<int64_t>(total_capacity, 0);
+  l.blocks = (int64_t)div_up(rec + 1, gpj::kScanChunk);
+  // room for the total plus an eighth, and a full stage per region
+  l.region = (int64_t)div_up(cap + cap / 8 + gpj::kDenseRegions * 64ull * gpj::kDenseStage,
+                             gpj::kDenseRegions);
+  auto r256 = [](uint64_t b) { return div_up(b, 256) * 256; };
+  l.tmp_off = 256;
+  l.sums = l.tmp_off + r256(nq * rec * 8);
+  l.tmp = l.sums + r256(nq * ((uint64_t)l.blocks + 1) * 8);
+  l.bytes = l.tmp + (uint64_t)gpj::kDenseRegions * (uint64_t)l.region * 16;
+  return l;
+}
+
 gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
                      const int64_t *d_starts, const int64_t *d_num_starts,
                      gpj::RecordMode mode, const gpj_query_desc *queries,
-                     int32_t num_queries, int64_t max_records, void *stream) {
+                     int32_t num_queries, int64_t max_records, void *stream,
+                     const gpj_dense_result *dense = nullptr, void *d_scratch = nullptr) {
   if (num_queries < 0 || num_queries > GPJ_MAX_BATCH_QUERIES || (num_queries > 0 && !queries))
     return fail(GPJ_ERR_INVALID_ARGUMENT,
                 "query batch holds 0.." + std::to_string(GPJ_MAX_BATCH_QUERIES) + " queries");
   gpj::QueryBatch batch{};
   uint32_t total_ir = 0;
+  int64_t total_cap = 0;
   for (int32_t q = 0; q < num_queries; ++q) {
     const gpj_query_desc &d = queries[q];
     if (d.ir_len == 0 || d.ir_len > GPJ_MAX_QUERY_IR_BYTES)
       return fail(GPJ_ERR_QUERY_EXECUTION, "Compiled query IR exceeds the 4096-byte IR buffer");
-    if (d.capacity < 1) return fail(GPJ_ERR_QUERY_EXECUTION, "query needs at least one result slot");
-    if (!d.d_ir || !d.d_spans || (d.variable && !d.d_counts))
+    if (!dense && d.capacity < 1)
+      return fail(GPJ_ERR_QUERY_EXECUTION, "query needs at least one result slot");
+    if (!d.d_ir || (!dense && (!d.d_spans || (d.variable && !d.d_counts))))
       return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null bytecode or result buffer");
     total_ir += d.ir_len;
-    batch.q[q] = gpj::QuerySlot{d.d_ir, d.ir_len, d.capacity, d.variable ? 1 : 0, d.d_spans,
-                                d.d_counts};
+    if (dense) {
+      const gpj_dense_result &r = dense[q];
+      if (!d.variable && d.capacity < 1)
+        return fail(GPJ_ERR_QUERY_EXECUTION, "fixed-slot query needs at least one result slot");
+      if (!r.d_offsets || r.capacity < 0 || (r.capacity > 0 && !r.d_spans))
+        return fail(GPJ_ERR_INVALID_ARGUMENT, "dense query: null offsets or spans");
+      batch.q[q] = gpj::QuerySlot{d.d_ir, d.ir_len, d.variable ? 1 : d.capacity,
+                                  d.variable ? 1 : 0, r.d_spans, r.d_offsets, 1, nullptr,
+                                  r.capacity};
+      total_cap += r.capacity;
+    } else {
+      batch.q[q] = gpj::QuerySlot{d.d_ir, d.ir_len, d.capacity, d.variable ? 1 : 0, d.d_spans,
+                                  d.d_counts};
+    }
   }
   if (total_ir > gpj::kBatchIrBytes)
     return fail(GPJ_ERR_QUERY_EXECUTION, "query batch bytecode exceeds 8192 bytes; split it");
   batch.count = num_queries;
   const int deepest = deepest_level(queries, num_queries);
-  if (max_records <= 0 || n == 0 || num_queries == 0) return GPJ_OK;
+  if (dense && num_queries > 0 && max_records >= 0) {
+    // every offset array gets its entries even when no record runs
+    if (!d_scratch) return fail(GPJ_ERR_INVALID_ARGUMENT, "dense queries: null scratch");
+    if (!d_num_starts) return fail(GPJ_ERR_QUERY_EXECUTION, "dense queries: null record count");
+    const DenseLayout lay = dense_layout(max_records, num_queries, total_cap);
+    uint8_t *sc = static_cast<uint8_t *>(d_scratch);
+    batch.ds = gpj::DenseStage{reinterpret_cast<int64_t *>(sc + lay.tmp),
+                               reinterpret_cast<uint64_t *>(sc), lay.region};
+    for (int32_t q = 0; q < num_queries; ++q)
+      batch.q[q].tmp_off = reinterpret_cast<int64_t *>(sc + lay.tmp_off) + (int64_t)q * max_records;
+  }
+  if (max_records <= 0 || n == 0 || num_queries == 0) {
+    if (dense && num_queries > 0 && max_records >= 0) {
+      // no record: every offset is 0
+      for (int32_t q = 0; q < num_queries; ++q) {
+        hipLaunchKernelGGL(fill_i64_kernel, dim3((uint32_t)std::min<uint64_t>(div_up((uint64_t)max_records + 1, 256), 1024)),
+                           dim3(256), 0, as_stream(stream), dense[q].d_offsets,
+                           max_records + 1, (int64_t)0);
+      }
+      GPJ_HIP(hipGetLastError(), "fill launch");
+    }
+    return GPJ_OK;
+  }
   if (n >= (1ull << 35) ||
       (uint64_t)std::max(index->num_levels, 1) * (uint64_t)index->level_size >= 0xFFFFFFF0ull)
     return fail(GPJ_ERR_QUERY_EXECUTION,
@@ -546,9 +606,15 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
   // the input is longer than the span) are rerun by the wide kernel
   const int64_t span = g_narrow_span.load();
   const bool fixup = (int64_t)n > span;
+  hipStream_t st = as_stream(stream);
+  if (dense) { // empty staging regions
+    hipLaunchKernelGGL(gpj::fill_u64_kernel, dim3(1), dim3(64), 0, st, batch.ds.cursors,
+                       (int64_t)gpj::kDenseRegions, (uint64_t)0);
+    GPJ_HIP(hipGetLastError(), "fill launch");
+  }
 #define GPJ_QLAUNCH(LV, NW, FIX)                                                          \
   hipLaunchKernelGGL((gpj::query_batch_kernel<LV, NW>), dim3((uint32_t)blocks), dim3(64), 0, \
-                     as_stream(stream), d_in, (int64_t)n, d_starts, d_num_starts,            \
+                     st, d_in, (int64_t)n, d_starts, d_num_starts,                           \
                      max_records, index->d_string, index->d_leveled, index->level_size,      \
                      index->num_levels, mode, gpj::PosMode{span, FIX}, batch)
 #define GPJ_QPAIR(LV)                                                                     \
@@ -556,13 +622,55 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
     GPJ_QLAUNCH(LV, true, 0);                                                             \
     if (fixup) GPJ_QLAUNCH(LV, false, 1);                                                 \
   } while (0)
-  if (deepest + 1 <= 4) GPJ_QPAIR(4);
-  else if (deepest + 1 <= 6) GPJ_QPAIR(6);
-  else if (deepest + 1 <= 8) GPJ_QPAIR(8);
-  else GPJ_QPAIR(16);
+#define GPJ_QDISPATCH(M)                                                                  \
+  do {                                                                                    \
+    if (deepest + 1 <= 4) M(4);                                                           \
+    else if (deepest + 1 <= 6) M(6);                                                      \
+    else if (deepest + 1 <= 8) M(8);                                                      \
+    else M(16);                                                                           \
+  } while (0)
+  GPJ_QDISPATCH(GPJ_QPAIR);
+  GPJ_HIP(hipGetLastError(), "query_batch_kernel launch");
+  if (dense) {
+    // counts -> CSR offsets (chunk sums, their scan, per-chunk scan), staged
+    // spans -> CSR order, then the records left to the fixup pass
+    const DenseLayout lay = dense_layout(max_records, num_queries, total_cap);
+    uint8_t *sc = static_cast<uint8_t *>(d_scratch);
+    gpj::DenseSet ds{};
+    for (int32_t q = 0; q < num_queries; ++q) {
+      ds.counts[q] = dense[q].d_offsets;
+      ds.tmp_off[q] = batch.q[q].tmp_off;
+      ds.spans[q] = dense[q].d_spans;
+      ds.out_cap[q] = dense[q].capacity;
+    }
+    ds.block_sums = reinterpret_cast<uint64_t *>(sc + lay.sums);
+    ds.blocks = lay.blocks;
+    ds.num_starts = d_num_starts;
+    ds.starts_cap = mode.starts_cap;
+    ds.drop_last = mode.drop_last;
+    ds.max_lines = max_records;
+    ds.tmp = batch.ds.tmp;
+    const dim3 chunks((uint32_t)lay.blocks, (uint32_t)num_queries);
+    hipLaunchKernelGGL(gpj::dense_reduce_kernel, chunks, dim3(256), 0, st, ds);
+    for (int32_t q = 0; q < num_queries; ++q) {
+      uint64_t *sums = ds.block_sums + (int64_t)q * (lay.blocks + 1);
+      hipLaunchKernelGGL(gpj::newline_scan_kernel, dim3(1), dim3(1024), 0, st, sums,
+                         (uint64_t)lay.blocks, reinterpret_cast<int64_t *>(sums + lay.blocks),
+                         (int64_t)0);
+    }
+    hipLaunchKernelGGL(gpj::dense_scan_kernel, chunks, dim3(256), 0, st, ds);
+    hipLaunchKernelGGL(gpj::dense_gather_kernel,
+                       dim3((uint32_t)div_up((uint64_t)max_records, 256), (uint32_t)num_queries),
+                       dim3(256), 0, st, ds);
+    GPJ_HIP(hipGetLastError(), "dense scan launch");
+#define GPJ_QFIX(LV) GPJ_QLAUNCH(LV, false, 2)
+    GPJ_QDISPATCH(GPJ_QFIX);
+#undef GPJ_QFIX
+    GPJ_HIP(hipGetLastError(), "dense fixup launch");
+  }
+#undef GPJ_QDISPATCH
 #undef GPJ_QPAIR
 #undef GPJ_QLAUNCH
-  GPJ_HIP(hipGetLastError(), "query_batch_kernel launch");
   return GPJ_OK;
 }
 
@@ -584,6 +692,24 @@ gpj_status gpj_query_run_batch(const uint8_t *d_in, uint64_t n, const gpj_index 
   return run_batch(d_in, n, index, index->d_newline, index->d_num_lines,
                    gpj::RecordMode{0, 0, 64, index->num_lines}, queries, num_queries, max_lines,
                    stream);
+}
+
+uint64_t gpj_query_dense_scratch_bytes(int64_t max_records, int32_t num_queries,
+                                       int64_t total_capacity) {
+  return dense_layout(max_records, num_queries, total_capacity).bytes;
+}
+
+gpj_status gpj_query_run_dense(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                               const gpj_query_desc *queries, int32_t num_queries,
+                               int64_t max_lines, const gpj_dense_result *results,
+                               void *d_scratch, void *stream) {
+  if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null index");
+  if (num_queries > 0 && !results)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "dense queries: null results");
+  if (max_lines < 0) return fail(GPJ_ERR_INVALID_ARGUMENT, "dense queries: negative line count");
+  return run_batch(d_in, n, index, index->d_newline, index->d_num_lines,
+                   gpj::RecordMode{0, 0, 64, index->num_lines}, queries, num_queries, max_lines,
+                   stream, results, d_scratch);
 }
 
 uint64_t gpj_row_bounds_scratch_bytes(int64_t level_size) {
@@ -627,6 +753,24 @@ gpj_status gpj_query_run_records(const uint8_t *d_in, uint64_t n, const gpj_inde
   return run_batch(d_in, n, index, d_bounds, d_num_bounds,
                    gpj::RecordMode{1, 1, 64, max_records + 1}, queries, num_queries, max_records,
                    stream);
+}
+
+gpj_status gpj_query_run_records_dense(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                                       const int64_t *d_bounds, const int64_t *d_num_bounds,
+                                       const gpj_query_desc *queries, int32_t num_queries,
+                                       int64_t max_records, const gpj_dense_result *results,
+                                       void *d_scratch, void *stream) {
+  if (!index) return fail(GPJ_ERR_INVALID_ARGUMENT, "query: null index");
+  if (index->num_levels < 2)
+    return fail(GPJ_ERR_QUERY_EXECUTION,
+                "array records need an index with at least 2 leveled rows");
+  if (num_queries > 0 && !results)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "dense queries: null results");
+  if (max_records < 0)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "dense queries: negative record count");
+  return run_batch(d_in, n, index, d_bounds, d_num_bounds,
+                   gpj::RecordMode{1, 1, 64, max_records + 1}, queries, num_queries, max_records,
+                   stream, results, d_scratch);
 }
 
 gpj_status gpj_query_run(const uint8_t *d_in, uint64_t n, const gpj_index *index,

@@ -73,13 +73,52 @@ struct QuerySlot {
   uint32_t ir_len;
   int32_t cap;        // result slots (fixed mode) or inline capacity (variable)
   int32_t variable;   // 0: reference fixed slots, 1: extension variable-length
-  int64_t *spans;     // [lines][cap][2]
-  int64_t *counts;    // [lines] or null (fixed mode)
+  int64_t *spans;     // [lines][cap][2]; dense: [out_cap][2] in record order
+  int64_t *counts;    // [lines] or null (fixed mode); dense: [lines + 1], the
+                      // counts, scanned in place into CSR offsets
+  int32_t dense;      // 1: dense results (any mode): spans of record r at
+                      // spans[offsets[r] .. offsets[r + 1])
+  int64_t *tmp_off;   // dense: [lines] where the record's spans sit in the
+                      // staging area, -1 = left to the dense fixup pass
+  int64_t out_cap;    // dense: spans the output holds
+};
+// Staging area of dense results: 8 regions of `region` spans (int64 pairs);
+// a wave reserves room in region blockIdx % 8 with cursors[region].
+constexpr int kDenseStage = 8; // spans per lane in the narrow kernels' LDS stage
+constexpr int kDenseRegions = 8;
+struct DenseStage {
+  int64_t *tmp;
+  uint64_t *cursors;
+  int64_t region;
 };
 struct QueryBatch {
   QuerySlot q[kMaxBatchQueries];
   int32_t count;
+  DenseStage ds;
 };
+
+// Dense results after the query passes: exclusive scan of each dense query's
+// counts into CSR offsets ([max_lines + 1] entries; the record count is
+// derived from the device line count as the query kernel does), and the copy
+// of staged spans into CSR order.  blockIdx.y = dense query.
+constexpr int kScanChunk = 4096; // counts per block of the scan (256 x 16)
+struct DenseSet {
+  int64_t *counts[kMaxBatchQueries];  // counts -> CSR offsets, in place
+  int64_t *tmp_off[kMaxBatchQueries];
+  int64_t *spans[kMaxBatchQueries];
+  int64_t out_cap[kMaxBatchQueries];
+  uint64_t *block_sums;               // [queries][blocks + 1]
+  int64_t blocks;                     // scan blocks per query
+  const int64_t *num_starts;          // device count of record starts
+  int64_t starts_cap;
+  int32_t drop_last;
+  int64_t max_lines;
+  const int64_t *tmp;                 // staging area (int64 pairs)
+};
+__global__ void dense_reduce_kernel(DenseSet d);
+__global__ void dense_scan_kernel(DenseSet d);
+__global__ void dense_gather_kernel(DenseSet d);
+__global__ void fill_u64_kernel(uint64_t *p, int64_t count, uint64_t v);
 
 // How records are cut: lines of a newline index (level_base 0, drop_last 0)
 // or top-level array elements between row-0 bounds (level_base 1,

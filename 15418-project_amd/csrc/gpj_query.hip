@@ -430,9 +430,17 @@ template <typename T, int N> struct LdsArray {
   }
 };
 
+// Spans a lane keeps in the wave's LDS stage for dense results (the narrow
+// kernels; the wide ones keep none): a record with more goes to the dense
+// fixup pass.
+template <typename P> struct StageSize {
+  static constexpr int value = sizeof(P) == 4 ? kDenseStage : 1;
+};
+
 // LDS home of the per-lane arrays of one wave.
 template <int MAXLV, typename P> struct WaveArrays {
   static constexpr int kF = MAXLV < kMaxFrames ? MAXLV : kMaxFrames;
+  P stage[StageSize<P>::value][64][2]; // dense results: {start, end} per lane
   P level_end[MAXLV][64];
   P marked_pos[MAXLV][64];
   P fr_sep[kF][64];
@@ -529,19 +537,35 @@ __device__ __forceinline__ bool resume(Ctx<P> &c, LaneState<MAXLV, P> &s) {
   }
 }
 
+// Where STORE_RESULT puts the spans of a lane's record: entries below `keep`
+// of `res` (file offsets, int64 pairs), or, with a stage, the wave's LDS stage
+// (positions, the first StageSize entries); the others are only counted.
+template <typename P> struct Sink {
+  int64_t *res;
+  int64_t keep;
+  P (*stage)[64][2];
+};
+
 // STORE_RESULT: the current value (spaces trimmed) goes to the next result
 // entry.  False when fixed mode has no slot left (the path fails).
 template <int MAXLV, typename P>
 __device__ __forceinline__ bool store_result(Ctx<P> &c, LaneState<MAXLV, P> &s, int32_t cap,
-                                             bool variable, int64_t *res) {
+                                             bool variable, const Sink<P> &out) {
   if (!variable && s.count >= cap) return false;
   const P lend = s.level_end.get(s.lvl);
   P end_str = lend;
   while (byte_at(c, s.idx) == ' ' && s.idx < lend) ++s.idx;
   while (end_str > s.idx && byte_at(c, (P)(end_str - 1)) == ' ') --end_str;
-  if (s.count < cap) // one 16-byte store (the span array is 8-byte aligned)
-    *reinterpret_cast<i64x2a8 *>(res + (int64_t)s.count * 2) =
+  if (out.stage) {
+    if (s.count < (P)StageSize<P>::value) {
+      const int lane = (int)(threadIdx.x & 63);
+      out.stage[s.count][lane][0] = s.idx;
+      out.stage[s.count][lane][1] = end_str;
+    }
+  } else if ((int64_t)s.count < out.keep) { // one 16-byte store (8-byte aligned array)
+    *reinterpret_cast<i64x2a8 *>(out.res + (int64_t)s.count * 2) =
         i64x2a8{base_of(c) + (int64_t)s.idx, base_of(c) + (int64_t)end_str};
+  }
   ++s.count;
   return true;
 }
@@ -556,7 +580,7 @@ __device__ __forceinline__ bool store_result(Ctx<P> &c, LaneState<MAXLV, P> &s, 
 template <int MAXLV, typename P>
 __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w, bool live,
                                              P line_start, P line_end, const IrReader ir,
-                                             int32_t cap, bool variable, int64_t *res) {
+                                             int32_t cap, bool variable, const Sink<P> &res) {
   LaneState<MAXLV, P> s(w, (int)(threadIdx.x & 63));
   s.idx = line_start;
   s.lvl = 0;
@@ -833,10 +857,59 @@ template <typename P> __device__ __forceinline__ bool trim_record(Ctx<P> &c, P &
   return start != end;
 }
 
-// Where a batch's marks live: the count of the first query in variable mode,
-// the start of its first span in fixed mode.
+// Where a batch's marks live: the count of the first query in variable and
+// dense mode, the start of its first span in fixed mode.
 __device__ __forceinline__ int64_t *mark_slot(const QuerySlot &q0, int64_t line) {
-  return q0.variable ? q0.counts + line : q0.spans + line * 2 * (int64_t)q0.cap;
+  return (q0.variable || q0.dense) ? q0.counts + line : q0.spans + line * 2 * (int64_t)q0.cap;
+}
+
+// Inclusive add-scan over the wave.
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
+  const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+// Dense results, first pass: the lanes whose record's spans all sit in the
+// wave's LDS stage move them to the staging area in one reservation (a wave
+// takes room from one of its 8 regions with one atomic add) and record where
+// (tmp_off); the others (more spans than the stage holds, a record left to the
+// wide kernel, a full region) record -1 and go to the dense fixup pass.  Every
+// valid lane writes its count (scanned into CSR offsets afterwards).
+template <typename P>
+__device__ __forceinline__ void flush_dense(const QuerySlot &qs, const DenseStage &ds,
+                                            P (*stage)[64][2], bool valid, bool to_fixup,
+                                            int64_t line, int64_t found, int64_t base) {
+  const int lane = (int)(threadIdx.x & 63);
+  const bool fits = valid && !to_fixup && found <= (int64_t)StageSize<P>::value;
+  const uint32_t cnt = fits ? (uint32_t)found : 0u;
+  const uint32_t incl = wave_incl_sum(cnt);
+  const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+  int64_t at = 0;
+  bool room = true;
+  if (total) {
+    const uint32_t region = blockIdx.x & 7u;
+    uint64_t got = 0;
+    if (lane == 0) got = atomicAdd(reinterpret_cast<unsigned long long *>(ds.cursors + region),
+                                   (unsigned long long)total);
+    got = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(got >> 32)) << 32) |
+          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)got);
+    room = got + total <= (uint64_t)ds.region;
+    at = (int64_t)region * ds.region + (int64_t)got + (int64_t)(incl - cnt);
+  }
+  const bool staged = fits && (cnt == 0 || room);
+  if (staged)
+    for (uint32_t k = 0; k < cnt; ++k)
+      *reinterpret_cast<i64x2a8 *>(ds.tmp + 2 * (at + (int64_t)k)) =
+          i64x2a8{base + (int64_t)stage[k][lane][0], base + (int64_t)stage[k][lane][1]};
+  if (valid) {
+    qs.counts[line] = found;
+    qs.tmp_off[line] = staged ? at : -1;
+  }
 }
 
 } // namespace
@@ -851,7 +924,12 @@ __device__ __forceinline__ int64_t *mark_slot(const QuerySlot &q0, int64_t line)
 // between consecutive leveled-row-0 characters, with interpreter level k on
 // leveled row k + 1 (mode.level_base = 1, mode.drop_last = 1: the last bound
 // opens no record).  Each wave takes mode.per_wave consecutive records at a
-// time.  Positions as PosMode describes.
+// time.  Positions as PosMode describes.  Dense queries (QuerySlot.dense) go
+// through the wave's LDS stage and the staging area (flush_dense); the wide
+// kernel's marked-record pass (fixup 1) counts their results and sends the
+// records to the dense fixup pass (fixup 2, wide kernel), which reruns the
+// records whose tmp_off is -1 and writes their spans straight to the CSR
+// output (spans + 2 * offsets[line], scanned offsets).
 template <int MAXLV, bool NARROW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ? 4 : 1))) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
@@ -873,9 +951,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
   for (int64_t wb = wave_base; wb < lines; wb += stride) {
     const int64_t line = wb + lane;
     bool valid = lane < mode.per_wave && line < lines;
-    if (!NARROW && pm.fixup) {
+    uint32_t redo = 0; // dense fixup: the dense queries this record reruns
+    if (!NARROW && pm.fixup == 1) {
       if (valid) valid = *mark_slot(batch.q[0], line) == kWideMark;
       if (!__ballot(valid)) continue; // (wave-uniform: lanes share wb)
+    } else if (!NARROW && pm.fixup == 2) {
+      if (valid)
+        for (int q = 0; q < batch.count; ++q)
+          if (batch.q[q].dense && batch.q[q].tmp_off[line] < 0 &&
+              batch.q[q].counts[line + 1] > batch.q[q].counts[line])
+            redo |= 1u << q;
+      valid = valid && redo != 0;
+      if (!__ballot(valid)) continue;
     }
     int64_t ls = 0, le = 0;
     bool record = false, mark = false;
@@ -896,14 +983,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
     if (record) record = trim_record(c, line_start, line_end);
     for (int q = 0; q < batch.count; ++q) {
       const QuerySlot &qs = batch.q[q];
-      int64_t *res = qs.spans + line * 2 * (int64_t)qs.cap;
-      if (valid && !qs.variable)
-        for (int i = 0; i < 2 * qs.cap; ++i) res[i] = -1;
+      const bool fix_dense = !NARROW && pm.fixup == 2;
+      if (fix_dense && !qs.dense) continue;
+      // where this query's spans go (one interpreter call site per kernel)
+      bool live = record && !c.wide;
+      Sink<P> out{nullptr, 0, nullptr};
+      if (fix_dense) { // dense fixup: straight into the CSR output
+        const bool mine = valid && ((redo >> q) & 1u);
+        const int64_t off = mine ? qs.counts[line] : 0;
+        const int64_t room = qs.out_cap - off;
+        out.res = qs.spans + 2 * off;
+        out.keep = mine ? (room < 0 ? 0 : room) : 0;
+        live = record && mine;
+      } else if (qs.dense) { // narrow kernel: the wave's stage; wide: counted only
+        out.stage = NARROW ? arrays.stage : nullptr;
+      } else {
+        out.res = qs.spans + line * 2 * (int64_t)qs.cap;
+        out.keep = qs.cap;
+        if (valid && !qs.variable)
+          for (int i = 0; i < 2 * qs.cap; ++i) out.res[i] = -1;
+      }
       const IrReader ir = IrReader::make(qs.ir, qs.ir_len);
-      const int64_t found =
-          run_query<MAXLV, P>(c, arrays, record && !c.wide, line_start, line_end, ir, qs.cap,
-                              qs.variable != 0, res);
-      if (valid && qs.counts) qs.counts[line] = found;
+      const int64_t found = run_query<MAXLV, P>(c, arrays, live, line_start, line_end, ir,
+                                                qs.cap, qs.variable != 0, out);
+      if (fix_dense) continue;
+      if (qs.dense) {
+        if (NARROW)
+          flush_dense<P>(qs, batch.ds, arrays.stage, valid, mark || c.wide, line, found,
+                         base_of(c));
+        else if (valid) {
+          qs.counts[line] = found;
+          qs.tmp_off[line] = -1;
+        }
+      } else if (valid && qs.counts) {
+        qs.counts[line] = found;
+      }
     }
     if (NARROW && (mark || c.wide)) *mark_slot(batch.q[0], line) = kWideMark;
   }
@@ -978,14 +1092,113 @@ __global__ __launch_bounds__(64) void query_files_kernel(
       if (mine && !qs.variable)
         for (int i = 0; i < 2 * fm.cap; ++i) res[i] = -1;
       const IrReader ir = IrReader::make(qs.ir, qs.ir_len);
+      const int32_t cap = qs.variable ? fm.cap : qs.cap;
+      const Sink<P> out{res, cap, nullptr};
       const int64_t found = run_query<MAXLV, P>(c, arrays, record && mine && !c.wide,
-                                                line_start, line_end, ir,
-                                                qs.variable ? fm.cap : qs.cap,
-                                                qs.variable != 0, res);
+                                                line_start, line_end, ir, cap,
+                                                qs.variable != 0, out);
       if (mine) (slot == 0 ? fm.counts[0] : fm.counts[1])[g] = found;
     }
     if (NARROW && (mark || c.wide)) fm.counts[0][g] = kWideMark;
   }
+}
+
+// ---- dense results: counts -> CSR offsets, staged spans -> CSR order ----
+
+namespace {
+// Records of the launch, as query_batch_kernel derives them.
+__device__ __forceinline__ int64_t dense_records(const DenseSet &d) {
+  const int64_t counted = *d.num_starts;
+  const int64_t all = counted < d.starts_cap ? counted : d.starts_cap;
+  const int64_t avail = all - d.drop_last;
+  const int64_t lines = avail < d.max_lines ? avail : d.max_lines;
+  return lines > 0 ? lines : 0;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_sum64(uint64_t v) {
+  const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+// Exclusive prefix over the block's 256 threads (4 waves); also the total.
+__device__ __forceinline__ uint64_t block_excl_sum(uint64_t v, uint64_t &total) {
+  __shared__ uint64_t s_wave[4];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  const uint64_t incl = wave_incl_sum64(v);
+  if (lane == 63) s_wave[wv] = incl;
+  __syncthreads();
+  uint64_t before = 0;
+  total = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    if (k < wv) before += s_wave[k];
+    total += s_wave[k];
+  }
+  __syncthreads();
+  return before + incl - v;
+}
+} // namespace
+
+// Per dense query (blockIdx.y) and chunk of kScanChunk entries: the sum of
+// the record counts in the chunk (entries past the record count read 0).
+__global__ __launch_bounds__(256) void dense_reduce_kernel(DenseSet d) {
+  const int q = (int)blockIdx.y;
+  const int64_t lines = dense_records(d);
+  const int64_t lo = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * 16;
+  const int64_t *c = d.counts[q];
+  uint64_t v = 0;
+  for (int k = 0; k < 16; ++k)
+    if (lo + k < lines) v += (uint64_t)c[lo + k];
+  uint64_t total;
+  (void)block_excl_sum(v, total);
+  if (threadIdx.x == 0) d.block_sums[(int64_t)q * (d.blocks + 1) + blockIdx.x] = total;
+}
+
+// Entries [0, max_lines] of the counts become exclusive prefixes (the CSR
+// offsets; entries from the record count on hold the total), given the
+// exclusive prefix of the chunk sums.
+__global__ __launch_bounds__(256) void dense_scan_kernel(DenseSet d) {
+  const int q = (int)blockIdx.y;
+  const int64_t lines = dense_records(d);
+  const int64_t lo = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * 16;
+  int64_t *c = d.counts[q];
+  uint64_t v[16], sum = 0;
+  for (int k = 0; k < 16; ++k) {
+    v[k] = lo + k < lines ? (uint64_t)c[lo + k] : 0;
+    sum += v[k];
+  }
+  uint64_t total;
+  uint64_t run = block_excl_sum(sum, total) +
+                 d.block_sums[(int64_t)q * (d.blocks + 1) + blockIdx.x];
+  for (int k = 0; k < 16; ++k) {
+    if (lo + k <= d.max_lines) c[lo + k] = (int64_t)run;
+    run += v[k];
+  }
+}
+
+// Staged spans of each record to its CSR position (records left to the
+// dense fixup pass are written there); nothing past the output's capacity.
+__global__ __launch_bounds__(256) void dense_gather_kernel(DenseSet d) {
+  const int q = (int)blockIdx.y;
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= dense_records(d)) return;
+  const int64_t t = d.tmp_off[q][r];
+  if (t < 0) return;
+  const int64_t o = d.counts[q][r], e = d.counts[q][r + 1];
+  const int64_t cap = d.out_cap[q];
+  for (int64_t k = 0; k < e - o && o + k < cap; ++k)
+    *reinterpret_cast<i64x2a8 *>(d.spans[q] + 2 * (o + k)) =
+        *reinterpret_cast<const i64x2a8 *>(d.tmp + 2 * (t + k));
+}
+
+__global__ void fill_u64_kernel(uint64_t *p, int64_t count, uint64_t v) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+    p[i] = v;
 }
 
 #define GPJ_QUERY_KERNELS(LV, NW)                                                              \

@@ -34,14 +34,55 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "kernel_traffic.json")
 
 
+KERNEL_SOURCES = ["csrc/gpj_index.hip", "csrc/gpj_query.hip", "csrc/gpj_kernels.h"]
+
+
+def kernel_sources_hash() -> str:
+    """sha256 over the kernel sources the traffic profile was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "15418-project_amd", rel), "rb") as f:
+            h.update(rel.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
 def profiled_traffic(workload: str, kernel: str):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE
-    (scaled by the copy-kernel calibration), or None when not profiled."""
+    (scaled by the copy-kernel calibration), or None when not profiled or
+    when the kernel sources changed since the profile was taken."""
     if not os.path.exists(PROFILE_TRAFFIC):
         return None
     with open(PROFILE_TRAFFIC) as f:
         prof = json.load(f)
+    if prof.get("kernel_sources_sha256") != kernel_sources_hash():
+        return None
     return prof.get("workloads", {}).get(workload, {}).get(kernel, {}).get("hbm_bytes_per_launch")
+
+
+def cpu_description() -> dict:
+    """Host CPU of this run: model name, logical CPUs and the CPUs this
+    process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
+def baseline_threads() -> int:
+    """All cores of this process's CPU share (at most 16: one GPU's share of
+    the host)."""
+    return max(1, min(16, cpu_description()["usable_cpus"]))
 
 
 def roofline_entry(kernel: str, algo_bytes: float, ms: float, traffic):
@@ -294,34 +335,39 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         orc = ge.load_oracle()
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        if array_mode:
-            # a prefix of the array: its complete elements are the records
-            sample = data[: min(data.size, args.cpu_sample_mib << 20 >> 3)]
-        else:
-            sample = cut_at_newline(data, args.cpu_sample_mib << 20)
-        t0 = time.time()
-        ix = orc.build_index(sample, levels, threads)
-        if array_mode:
-            b = orc.row_bounds(ix)
-            for c, v in zip(compiled, variable):
-                orc.query_records(sample, ix, b, c[0], ext_cap if v else c[2], v)
-        for c, v in zip(compiled, variable):
+
+        def cpu_run(sample_bytes, threads):
             if array_mode:
-                break
-            if v:
-                orc.query_ext(sample, ix, c[0], ext_cap, threads)
+                # a prefix of the array: its complete elements are the records
+                sample = data[: min(data.size, sample_bytes >> 3)]
             else:
-                orc.query(sample, ix, c[0], c[2], threads)
-        cpu_s = time.time() - t0
-        cpu = {"value": round(sample.size / cpu_s / 1e9, 4), "unit": "GB/s", "cores": threads,
-               "kind": "port",
-               "sample": f"first {sample.size} bytes of the same data"
-                         f"{'' if array_mode else ' (cut at a newline)'}, index + all "
-                         f"{len(compiled)} queries; CPU oracle (port of the reference "
-                         f"algorithm; index multi-threaded"
-                         f"{', element queries on one thread' if array_mode else ''}), "
-                         f"{cpu_s:.2f}s wall"}
+                sample = cut_at_newline(data, sample_bytes)
+            t0 = time.time()
+            ix = orc.build_index(sample, levels, threads)
+            if array_mode:
+                b = orc.row_bounds(ix)
+                for c, v in zip(compiled, variable):
+                    orc.query_records(sample, ix, b, c[0], ext_cap if v else c[2], v)
+            else:
+                for c, v in zip(compiled, variable):
+                    if v:
+                        orc.query_ext(sample, ix, c[0], ext_cap, threads)
+                    else:
+                        orc.query(sample, ix, c[0], c[2], threads)
+            secs = time.time() - t0
+            return {"value": round(sample.size / secs / 1e9, 4), "threads": threads,
+                    "sample": f"first {sample.size} bytes of the same data"
+                              f"{'' if array_mode else ' (cut at a newline)'}, index + all "
+                              f"{len(compiled)} queries, {secs:.2f}s wall"}
+
+        all_threads = args.cpu_threads or baseline_threads()
+        t_all = cpu_run(args.cpu_sample_mib << 20, all_threads)
+        t_one = cpu_run(args.cpu_sample_mib << 17, 1)  # 1/8 of the sample on one thread
+        cpu = {"value": t_all["value"], "unit": "GB/s", "cores": all_threads, "kind": "port",
+               "sample": t_all["sample"] + "; CPU oracle (port of the reference algorithm, "
+                         "index and line queries split over contiguous line ranges"
+                         f"{', element queries on one thread' if array_mode else ''})",
+               "threads_all": t_all, "threads_1": t_one, **cpu_description()}
 
     # per query: newline index read, one pass over the first leveled row,
     # 8 bytes per emitted span (index-walk and text reads are data dependent)

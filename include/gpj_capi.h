@@ -262,6 +262,40 @@ gpj_status gpj_query_run_records(const uint8_t *d_in, uint64_t n, const gpj_inde
                                  const gpj_query_desc *queries, int32_t num_queries,
                                  int64_t max_records, void *stream);
 
+/* ---- extension: dense results, counted, scanned and emitted on the device
+ * (no counterpart in the reference, which copies back a fixed
+ * [lines][slots] array per query,
+ * src/gpjson/query/kernels/orig/query_executor_host.cu:199-228).  Each
+ * query's results land in one span array in record order plus CSR offsets:
+ * record r's {start, end} pairs are d_spans[d_offsets[r] .. d_offsets[r + 1])
+ * and d_offsets[max_records] is the total (entries past the record count
+ * hold it too).  No per-record capacity: fixed-slot bytecode keeps its slot
+ * limit (desc.capacity = the compiled slot count; its stored slots are always
+ * the first ones, so its spans are those slots in order), extension bytecode
+ * (desc.variable = 1) any number.  If a total exceeds its `capacity`, the
+ * spans past it are not written (the offsets are complete) and the caller
+ * reruns with capacity = the total.  desc.d_spans / d_counts are ignored.
+ * d_scratch: gpj_query_dense_scratch_bytes(max_records, num_queries, sum of
+ * the capacities) bytes.  Stream-ordered; allocates nothing. */
+typedef struct {
+  int64_t *d_offsets; /* [max_records + 1] */
+  int64_t *d_spans;   /* [capacity][2] */
+  int64_t capacity;
+} gpj_dense_result;
+uint64_t gpj_query_dense_scratch_bytes(int64_t max_records, int32_t num_queries,
+                                       int64_t total_capacity);
+/* Dense results over the lines of the index (as gpj_query_run_batch). */
+gpj_status gpj_query_run_dense(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                               const gpj_query_desc *queries, int32_t num_queries,
+                               int64_t max_lines, const gpj_dense_result *results,
+                               void *d_scratch, void *stream);
+/* Dense results over top-level array elements (as gpj_query_run_records). */
+gpj_status gpj_query_run_records_dense(const uint8_t *d_in, uint64_t n, const gpj_index *index,
+                                       const int64_t *d_bounds, const int64_t *d_num_bounds,
+                                       const gpj_query_desc *queries, int32_t num_queries,
+                                       int64_t max_records, const gpj_dense_result *results,
+                                       void *d_scratch, void *stream);
+
 /* ---- extension: multi-file batches (no reference counterpart; the
  * reference engine loops over partitions one launch sequence at a time,
  * src/gpjson/engine.cpp:131-166).  Many independent files are indexed in one

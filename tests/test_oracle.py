@@ -406,3 +406,33 @@ def test_extension_mode_keeps_reference_results(orc, gpj):
             got = [tuple(x) for x in spans[li][: int(counts[li])].tolist()]
             want = [tuple(x) for x in fixed[li].tolist() if x[0] >= 0]
             assert got == want
+
+
+ENGINE_KAT = (b'{"name":"Ada","scores":[1,2],"lang":"en"}\n'
+              b'{"name":"Bob","scores":[3,4],"lang":"fr"}\n')
+
+
+@pytest.mark.parametrize("part_size", [0, 43])
+def test_engine_query_reference_kat(orc, gpj, part_size):
+    # tests/engine_test.cpp:41-58 (both files of the reference's engine tests;
+    # its options cut partitions of 43 bytes, engine_test.cpp:22-28)
+    compiled = [gpj.compile_query(q) for q in ("$.name", "$.scores[1]")]
+    out = orc.engine_query(ENGINE_KAT, compiled, part_size)
+    assert out == [[[b'"Ada"'], [b'"Bob"']], [[b"2"], [b"4"]]]
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_engine_query_partitioning_keeps_line_results(orc, gpj, seed):
+    # lines never straddle a partition cut, so the materialized result of the
+    # whole file equals that of any partitioning (the cut newline dropped)
+    data = jsongen.ndjson(seed, 60000, max_depth=5)
+    compiled = [gpj.compile_query(q) for q in ("$.a", "$.user.lang", "$.e[1]", "$.items[0].sku")]
+    whole = orc.engine_query(data, compiled, 0)
+    assert any(whole)
+    for part in (4096, 10007, 33333):
+        assert orc.engine_query(data, compiled, part) == whole
+    # and each value text is the span the per-line query returned
+    ix = orc.build_index(data, 2)
+    spans = orc.query(data, ix, compiled[0][0], compiled[0][2])
+    texts = [[data[a:b]] for a, b in spans[:, 0] if a >= 0]
+    assert whole[0] == texts
