@@ -518,7 +518,7 @@ DenseLayout dense_layout(int64_t max_records, int32_t num_queries, int64_t total
   l.region = (int64_t)div_up(cap + cap / 8 + gpj::kDenseRegions * 64ull * gpj::kDenseStage,
                              gpj::kDenseRegions);
   auto r256 = [](uint64_t b) { return div_up(b, 256) * 256; };
-  l.tmp_off = 256;
+  l.tmp_off = (uint64_t)gpj::kDenseRegions * gpj::kCursorStride * 8; // the cursors
   l.sums = l.tmp_off + r256(nq * rec * 8);
   l.tmp = l.sums + r256(nq * ((uint64_t)l.blocks + 1) * 8);
   l.bytes = l.tmp + (uint64_t)gpj::kDenseRegions * (uint64_t)l.region * 16;
@@ -608,8 +608,8 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
   const bool fixup = (int64_t)n > span;
   hipStream_t st = as_stream(stream);
   if (dense) { // empty staging regions
-    hipLaunchKernelGGL(gpj::fill_u64_kernel, dim3(1), dim3(64), 0, st, batch.ds.cursors,
-                       (int64_t)gpj::kDenseRegions, (uint64_t)0);
+    hipLaunchKernelGGL(gpj::fill_u64_kernel, dim3(16), dim3(256), 0, st, batch.ds.cursors,
+                       (int64_t)gpj::kDenseRegions * gpj::kCursorStride, (uint64_t)0);
     GPJ_HIP(hipGetLastError(), "fill launch");
   }
 #define GPJ_QLAUNCH(LV, NW, FIX)                                                          \
@@ -632,8 +632,9 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
   GPJ_QDISPATCH(GPJ_QPAIR);
   GPJ_HIP(hipGetLastError(), "query_batch_kernel launch");
   if (dense) {
-    // counts -> CSR offsets (chunk sums, their scan, per-chunk scan), staged
-    // spans -> CSR order, then the records left to the fixup pass
+    // counts -> CSR offsets (chunk sums, their prefix, per-chunk scan fused
+    // with the copy of staged spans into CSR order), then the records left
+    // to the fixup pass
     const DenseLayout lay = dense_layout(max_records, num_queries, total_cap);
     uint8_t *sc = static_cast<uint8_t *>(d_scratch);
     gpj::DenseSet ds{};
@@ -652,16 +653,9 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
     ds.tmp = batch.ds.tmp;
     const dim3 chunks((uint32_t)lay.blocks, (uint32_t)num_queries);
     hipLaunchKernelGGL(gpj::dense_reduce_kernel, chunks, dim3(256), 0, st, ds);
-    for (int32_t q = 0; q < num_queries; ++q) {
-      uint64_t *sums = ds.block_sums + (int64_t)q * (lay.blocks + 1);
-      hipLaunchKernelGGL(gpj::newline_scan_kernel, dim3(1), dim3(1024), 0, st, sums,
-                         (uint64_t)lay.blocks, reinterpret_cast<int64_t *>(sums + lay.blocks),
-                         (int64_t)0);
-    }
+    hipLaunchKernelGGL(gpj::dense_sums_kernel, dim3((uint32_t)num_queries), dim3(1024), 0, st,
+                       ds);
     hipLaunchKernelGGL(gpj::dense_scan_kernel, chunks, dim3(256), 0, st, ds);
-    hipLaunchKernelGGL(gpj::dense_gather_kernel,
-                       dim3((uint32_t)div_up((uint64_t)max_records, 256), (uint32_t)num_queries),
-                       dim3(256), 0, st, ds);
     GPJ_HIP(hipGetLastError(), "dense scan launch");
 #define GPJ_QFIX(LV) GPJ_QLAUNCH(LV, false, 2)
     GPJ_QDISPATCH(GPJ_QFIX);

@@ -83,9 +83,12 @@ struct QuerySlot {
   int64_t out_cap;    // dense: spans the output holds
 };
 // Staging area of dense results: 8 regions of `region` spans (int64 pairs);
-// a wave reserves room in region blockIdx % 8 with cursors[region].
+// a wave reserves room in region blockIdx % 8 with the cursor at
+// cursors[region * kCursorStride] (each in a 4 KiB block of its own, so the
+// 8 cursors' atomics do not queue on one cache line).
 constexpr int kDenseStage = 8; // spans per lane in the narrow kernels' LDS stage
 constexpr int kDenseRegions = 8;
+constexpr int kCursorStride = 512; // uint64 entries (4 KiB)
 struct DenseStage {
   int64_t *tmp;
   uint64_t *cursors;
@@ -99,8 +102,9 @@ struct QueryBatch {
 
 // Dense results after the query passes: exclusive scan of each dense query's
 // counts into CSR offsets ([max_lines + 1] entries; the record count is
-// derived from the device line count as the query kernel does), and the copy
-// of staged spans into CSR order.  blockIdx.y = dense query.
+// derived from the device line count as the query kernel does: chunk sums,
+// their prefix, the chunks' scan), fused with the copy of staged spans into
+// CSR order.
 constexpr int kScanChunk = 4096; // counts per block of the scan (256 x 16)
 struct DenseSet {
   int64_t *counts[kMaxBatchQueries];  // counts -> CSR offsets, in place
@@ -116,8 +120,8 @@ struct DenseSet {
   const int64_t *tmp;                 // staging area (int64 pairs)
 };
 __global__ void dense_reduce_kernel(DenseSet d);
+__global__ void dense_sums_kernel(DenseSet d);
 __global__ void dense_scan_kernel(DenseSet d);
-__global__ void dense_gather_kernel(DenseSet d);
 __global__ void fill_u64_kernel(uint64_t *p, int64_t count, uint64_t v);
 
 // How records are cut: lines of a newline index (level_base 0, drop_last 0)

@@ -894,7 +894,7 @@ __device__ __forceinline__ void flush_dense(const QuerySlot &qs, const DenseStag
   if (total) {
     const uint32_t region = blockIdx.x & 7u;
     uint64_t got = 0;
-    if (lane == 0) got = atomicAdd(reinterpret_cast<unsigned long long *>(ds.cursors + region),
+    if (lane == 0) got = atomicAdd(reinterpret_cast<unsigned long long *>(ds.cursors + region * kCursorStride),
                                    (unsigned long long)total);
     got = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(got >> 32)) << 32) |
           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)got);
@@ -1145,54 +1145,98 @@ __device__ __forceinline__ uint64_t block_excl_sum(uint64_t v, uint64_t &total) 
 
 // Per dense query (blockIdx.y) and chunk of kScanChunk entries: the sum of
 // the record counts in the chunk (entries past the record count read 0).
+// Coalesced: iteration k reads entries k * 256 + thread.
 __global__ __launch_bounds__(256) void dense_reduce_kernel(DenseSet d) {
   const int q = (int)blockIdx.y;
   const int64_t lines = dense_records(d);
-  const int64_t lo = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * 16;
+  const int64_t lo = (int64_t)blockIdx.x * kScanChunk + threadIdx.x;
   const int64_t *c = d.counts[q];
   uint64_t v = 0;
-  for (int k = 0; k < 16; ++k)
-    if (lo + k < lines) v += (uint64_t)c[lo + k];
+#pragma unroll
+  for (int k = 0; k < kScanChunk / 256; ++k)
+    if (lo + k * 256 < lines) v += (uint64_t)c[lo + k * 256];
   uint64_t total;
   (void)block_excl_sum(v, total);
   if (threadIdx.x == 0) d.block_sums[(int64_t)q * (d.blocks + 1) + blockIdx.x] = total;
 }
 
-// Entries [0, max_lines] of the counts become exclusive prefixes (the CSR
-// offsets; entries from the record count on hold the total), given the
-// exclusive prefix of the chunk sums.
-__global__ __launch_bounds__(256) void dense_scan_kernel(DenseSet d) {
-  const int q = (int)blockIdx.y;
-  const int64_t lines = dense_records(d);
-  const int64_t lo = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * 16;
-  int64_t *c = d.counts[q];
-  uint64_t v[16], sum = 0;
-  for (int k = 0; k < 16; ++k) {
-    v[k] = lo + k < lines ? (uint64_t)c[lo + k] : 0;
-    sum += v[k];
+// One workgroup per dense query (blockIdx.x): the chunk sums become their
+// exclusive prefix, the total goes after them.
+__global__ __launch_bounds__(1024) void dense_sums_kernel(DenseSet d) {
+  __shared__ uint64_t s_part[1024];
+  uint64_t *sums = d.block_sums + (int64_t)blockIdx.x * (d.blocks + 1);
+  const uint32_t t = threadIdx.x;
+  const int64_t per = (d.blocks + 1023) / 1024;
+  const int64_t lo = (int64_t)t * per, hi = lo + per < d.blocks ? lo + per : d.blocks;
+  uint64_t v = 0;
+  for (int64_t i = lo; i < hi; ++i) v += sums[i];
+  s_part[t] = v;
+  __syncthreads();
+  if (t == 0) {
+    uint64_t run = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const uint64_t x = s_part[i];
+      s_part[i] = run;
+      run += x;
+    }
+    sums[d.blocks] = run;
   }
-  uint64_t total;
-  uint64_t run = block_excl_sum(sum, total) +
-                 d.block_sums[(int64_t)q * (d.blocks + 1) + blockIdx.x];
-  for (int k = 0; k < 16; ++k) {
-    if (lo + k <= d.max_lines) c[lo + k] = (int64_t)run;
-    run += v[k];
+  __syncthreads();
+  uint64_t run = s_part[t];
+  for (int64_t i = lo; i < hi; ++i) {
+    const uint64_t x = sums[i];
+    sums[i] = run;
+    run += x;
   }
 }
 
-// Staged spans of each record to its CSR position (records left to the
-// dense fixup pass are written there); nothing past the output's capacity.
-__global__ __launch_bounds__(256) void dense_gather_kernel(DenseSet d) {
+// Per dense query and chunk: entries [0, max_lines] of the counts become
+// exclusive prefixes (the CSR offsets; entries from the record count on hold
+// the total), and each record whose spans were staged has them copied to its
+// CSR position (records left to the dense fixup pass are written there);
+// nothing past the output's capacity.  The chunk goes through LDS so that
+// global reads and writes stay coalesced while each thread scans 16
+// consecutive entries.
+__global__ __launch_bounds__(256) void dense_scan_kernel(DenseSet d) {
+  __shared__ uint64_t s_val[kScanChunk];
+  __shared__ uint64_t s_end;
   const int q = (int)blockIdx.y;
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= dense_records(d)) return;
-  const int64_t t = d.tmp_off[q][r];
-  if (t < 0) return;
-  const int64_t o = d.counts[q][r], e = d.counts[q][r + 1];
+  const int64_t lines = dense_records(d);
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+  const uint32_t t = threadIdx.x;
+  int64_t *c = d.counts[q];
+  for (int k = 0; k < kScanChunk / 256; ++k) {
+    const int64_t i = base + k * 256 + t;
+    s_val[k * 256 + t] = i < lines ? (uint64_t)c[i] : 0;
+  }
+  __syncthreads();
+  uint64_t sum = 0;
+  for (int j = 0; j < 16; ++j) sum += s_val[16 * t + j];
+  uint64_t total;
+  uint64_t run = block_excl_sum(sum, total) + d.block_sums[(int64_t)q * (d.blocks + 1) + blockIdx.x];
+  if (t == 255) s_end = run + sum; // the next chunk's first offset
+  for (int j = 0; j < 16; ++j) {
+    const uint64_t v = s_val[16 * t + j];
+    s_val[16 * t + j] = run;
+    run += v;
+  }
+  __syncthreads();
+  const int64_t *toff = d.tmp_off[q];
   const int64_t cap = d.out_cap[q];
-  for (int64_t k = 0; k < e - o && o + k < cap; ++k)
-    *reinterpret_cast<i64x2a8 *>(d.spans[q] + 2 * (o + k)) =
-        *reinterpret_cast<const i64x2a8 *>(d.tmp + 2 * (t + k));
+  int64_t *out = d.spans[q];
+  for (int k = 0; k < kScanChunk / 256; ++k) {
+    const int e = k * 256 + (int)t;
+    const int64_t i = base + e;
+    const int64_t o = (int64_t)s_val[e];
+    if (i <= d.max_lines) c[i] = o;
+    if (i >= lines) continue;
+    const int64_t src = toff[i];
+    if (src < 0) continue;
+    const int64_t end = e + 1 < kScanChunk ? (int64_t)s_val[e + 1] : (int64_t)s_end;
+    for (int64_t j = 0; j < end - o && o + j < cap; ++j)
+      *reinterpret_cast<i64x2a8 *>(out + 2 * (o + j)) =
+          *reinterpret_cast<const i64x2a8 *>(d.tmp + 2 * (src + j));
+  }
 }
 
 __global__ void fill_u64_kernel(uint64_t *p, int64_t count, uint64_t v) {

@@ -160,6 +160,13 @@ def main():
     ap.add_argument("--cpu-sample-mib", type=int, default=2048)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--results", choices=["dense", "slots"], default="slots",
+                    help="slots: per-record result arrays ([records][capacity][2], the "
+                         "reference's layout; capacity 8 for extension queries); dense: each "
+                         "query's spans in one array with CSR offsets (counted, scanned and "
+                         "emitted on the device).  The other layout is timed afterwards on "
+                         "the same index and compared in full")
+    ap.add_argument("--no-layout-check", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel over MIB MiB (profiler counter scale)")
     args = ap.parse_args()
@@ -224,15 +231,46 @@ def main():
     d_irs = [gpj.DeviceBuffer.from_host(np.frombuffer(c[0], dtype=np.uint8), stream)
              for c in compiled]
     caps = [ext_cap if v else c[2] for c, v in zip(compiled, variable)]
-    d_spans = [gpj.DeviceBuffer(nl_cap * k * 16) for k in caps]
-    d_counts = [gpj.DeviceBuffer(nl_cap * 8) if v else None for v in variable]
     L = gpj.lib()
     grid_lines = nl_cap
+    nq = len(compiled)
+    dense = args.results == "dense"
+    descs = (gpj.QueryDescT * nq)()
+    slot_bufs = []
 
-    descs = (gpj.QueryDescT * len(compiled))()
-    for qi, c in enumerate(compiled):
-        descs[qi] = gpj.QueryDescT(d_irs[qi].ptr, len(c[0]), caps[qi], int(variable[qi]), c[1],
-                                   d_spans[qi].ptr, d_counts[qi].ptr if variable[qi] else None)
+    def slot_results(rows):
+        # per-record result arrays ([rows][cap][2] spans, counts for extension
+        # bytecode), the reference's layout extended by a capacity
+        nonlocal slot_bufs
+        slot_bufs = [(gpj.DeviceBuffer(max(rows * k * 16, 16)),
+                      gpj.DeviceBuffer(max(rows * 8, 8)) if v else None)
+                     for k, v in zip(caps, variable)]
+        for qi, c in enumerate(compiled):
+            sp, ct = slot_bufs[qi]
+            descs[qi] = gpj.QueryDescT(d_irs[qi].ptr, len(c[0]), caps[qi], int(variable[qi]),
+                                       c[1], sp.ptr, ct.ptr if ct else None)
+
+    dense_res = (gpj.DenseResultT * nq)()
+    dense_bufs = {}
+
+    def dense_results(rows, spans_each):
+        # one CSR offsets array and one span array per query; scratch
+        dense_bufs.clear()
+        dense_bufs["offsets"] = [gpj.DeviceBuffer((rows + 1) * 8) for _ in range(nq)]
+        dense_bufs["spans"] = [gpj.DeviceBuffer(max(k * 16, 16)) for k in spans_each]
+        dense_bufs["scratch"] = gpj.DeviceBuffer(
+            L.gpj_query_dense_scratch_bytes(rows, nq, int(sum(spans_each))))
+        dense_bufs["rows"] = rows
+        for qi, c in enumerate(compiled):
+            dense_res[qi] = gpj.DenseResultT(dense_bufs["offsets"][qi].ptr,
+                                             dense_bufs["spans"][qi].ptr, int(spans_each[qi]))
+            descs[qi] = gpj.QueryDescT(d_irs[qi].ptr, len(c[0]), caps[qi], int(variable[qi]),
+                                       c[1], None, None)
+
+    if dense:
+        dense_results(nl_cap, [nl_cap] * nq)
+    else:
+        slot_results(nl_cap)
 
     bounds_cap = nl_cap
     d_bounds = gpj.DeviceBuffer(bounds_cap * 8) if array_mode else None
@@ -240,15 +278,29 @@ def main():
     d_bscratch = (gpj.DeviceBuffer(L.gpj_row_bounds_scratch_bytes(words))
                   if array_mode else None)
 
-    def run_queries(qdescs, nq):
+    def run_queries(qdescs, count, first=0):
         if array_mode:
             gpj.check(L.gpj_row_bounds(ctypes.byref(idx), d_bounds.ptr, bounds_cap,
                                        d_nbounds.ptr, d_bscratch.ptr, stream), "row bounds")
+        if dense:
+            res = ctypes.cast(ctypes.addressof(dense_res) + first * ctypes.sizeof(gpj.DenseResultT),
+                              ctypes.POINTER(gpj.DenseResultT))
+            if array_mode:
+                gpj.check(L.gpj_query_run_records_dense(d_in.ptr, n, ctypes.byref(idx),
+                                                        d_bounds.ptr, d_nbounds.ptr, qdescs,
+                                                        count, grid_lines, res,
+                                                        dense_bufs["scratch"].ptr, stream),
+                          "dense record query")
+            else:
+                gpj.check(L.gpj_query_run_dense(d_in.ptr, n, ctypes.byref(idx), qdescs, count,
+                                                grid_lines, res, dense_bufs["scratch"].ptr,
+                                                stream), "dense query")
+        elif array_mode:
             gpj.check(L.gpj_query_run_records(d_in.ptr, n, ctypes.byref(idx), d_bounds.ptr,
-                                              d_nbounds.ptr, qdescs, nq, grid_lines, stream),
+                                              d_nbounds.ptr, qdescs, count, grid_lines, stream),
                       "query records")
         else:
-            gpj.check(L.gpj_query_run_batch(d_in.ptr, n, ctypes.byref(idx), qdescs, nq,
+            gpj.check(L.gpj_query_run_batch(d_in.ptr, n, ctypes.byref(idx), qdescs, count,
                                             grid_lines, stream), "query batch")
 
     def step(ev=None):
@@ -266,7 +318,7 @@ def main():
 
     def single_query(qi):
         # one query alone, through the same batched entry point
-        run_queries(ctypes.byref(descs[qi]), 1)
+        run_queries(ctypes.byref(descs[qi]), 1, qi)
 
     # first build: learn the line count (the reference reads it back the same
     # way, orig_index_builder.cu:325-330) and size the query grids to it
@@ -284,6 +336,15 @@ def main():
             raise SystemExit("bench.py: record-bound capacity too small for this workload")
         num_lines = max(m - 1, 0)
     grid_lines = num_lines
+    if dense:
+        # exact outputs: each query's total from the first run
+        totals = [int(o.to_host(np.int64, dense_bufs["rows"] + 1)[min(num_lines,
+                                                                      dense_bufs["rows"])])
+                  for o in dense_bufs["offsets"]]
+        dense_results(num_lines, [max(t, 1) for t in totals])
+        step()
+    else:
+        slot_results(num_lines)
     for _ in range(args.warmup):
         step()
     gpj.check(L.gpj_stream_sync(stream))
@@ -320,17 +381,55 @@ def main():
     records = num_lines
     algo_index = n + n / 8 + levels * n / 8 + 8 * nl_entries
 
-    # matched values per query (spans >= 0; variable mode: the per-line counts)
-    matches = []
-    for qi, c in enumerate(compiled):
-        if variable[qi]:
-            cnt = d_counts[qi].to_host(np.int64, num_lines)
-            if int(cnt.max(initial=0)) > ext_cap:
-                raise SystemExit("bench.py: a record exceeded the inline result capacity")
-            matches.append(int(cnt.sum()))
+    # each query's results in CSR form (offsets [records + 1], spans [total, 2])
+    def slot_form(qi):
+        sp, ct = slot_bufs[qi]
+        spans = sp.to_host(np.int64, num_lines * caps[qi] * 2).reshape(num_lines, caps[qi], 2)
+        counts = ct.to_host(np.int64, num_lines) if ct else (spans[:, :, 0] >= 0).sum(axis=1)
+        if int(counts.max(initial=0)) > caps[qi]:
+            raise SystemExit("bench.py: a record exceeded the inline result capacity")
+        return gpj.dense_from_slots(spans, counts)
+
+    def csr_results():
+        if not dense:
+            return [slot_form(qi) for qi in range(nq)]
+        offs = [o.to_host(np.int64, num_lines + 1) for o in dense_bufs["offsets"]]
+        return [(o, sp.to_host(np.int64, 2 * int(o[-1])).reshape(-1, 2))
+                for sp, o in zip(dense_bufs["spans"], offs)]
+
+    got = csr_results()
+    matches = [int(o[-1]) for o, _ in got]
+    # the other result layout, timed over the same index (queries only) and
+    # compared with the timed layout in full
+    other = None
+    if num_lines and not args.no_layout_check:
+        dense = not dense
+        if dense:
+            dense_results(num_lines, [max(m, 1) for m in matches])
         else:
-            sp = d_spans[qi].to_host(np.int64, num_lines * c[2] * 2).reshape(num_lines, c[2], 2)
-            matches.append(int((sp[:, :, 0] >= 0).sum()))
+            dense_bufs.clear()
+            slot_results(num_lines)
+        run_queries(descs, nq)
+        a, b = hip.event(), hip.event()
+        gpj.check(L.gpj_stream_sync(stream))
+        hip.record(a, stream)
+        for _ in range(args.steps):
+            run_queries(descs, nq)
+        hip.record(b, stream)
+        hip.sync_event(b)
+        other_ms = hip.elapsed_ms(a, b) / args.steps
+        alt = csr_results()
+        same = all(np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+                   for x, y in zip(got, alt))
+        if not same:
+            raise SystemExit("bench.py: dense and per-record results differ")
+        result_bytes = {
+            "dense": int(sum((num_lines + 1) * 8 + 16 * m for m in matches)),
+            "slots": int(sum(num_lines * 16 * k + (8 * num_lines if v else 0)
+                             for k, v in zip(caps, variable)))}
+        other = {"results": "dense" if dense else "slots", "query_batch_ms": round(other_ms, 4),
+                 "identical_to_timed": same, "result_bytes": result_bytes}
+        dense = not dense
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -395,6 +494,7 @@ def main():
         "config": {"workload": args.workload, "bytes": n,
                    "records": num_lines if array_mode else num_lines - 1,
                    "queries": queries, "levels": levels, "matches": matches,
+                   "results": args.results, "other_results": other,
                    "parallelism": "single-gpu"},
         "roofline": dominant,
         "roofline_kernels": kernels,
