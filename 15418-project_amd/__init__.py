@@ -700,10 +700,12 @@ def generate(schema: int, target: int, seed: int, threads: int | None = None) ->
     return out[:n]
 
 
-def generate_files(schemas, targets, seed: int, threads: int | None = None):
-    """Many seeded NDJSON files (schemas 0, 2 or 3; each at most its target
-    size, at least one record), laid out GPJ_BATCH_ALIGN-aligned in one
-    zero-filled buffer.  Returns (buffer uint8, offsets uint64, sizes uint64)."""
+def generate_files(schemas, targets, seed: int, threads: int | None = None,
+                   align: int = BATCH_ALIGN):
+    """Many seeded files (NDJSON schemas 0, 2 or 3, each at most its target
+    size and at least one record; schema 1: one top-level array document),
+    laid out `align`-aligned in one zero-filled buffer.  Returns (buffer
+    uint8, offsets uint64, sizes uint64)."""
     g = _gen_lib()
     sch = np.ascontiguousarray(schemas, dtype=np.int32)
     tg = np.ascontiguousarray(targets, dtype=np.uint64)
@@ -712,7 +714,7 @@ def generate_files(schemas, targets, seed: int, threads: int | None = None):
     sizes = np.zeros(f, dtype=np.uint64)
     ptr = ctypes.c_void_p()
     threads = threads or min(16, os.cpu_count() or 4)
-    total = g.gpj_gen_files(f, sch.ctypes.data, tg.ctypes.data, seed, threads, BATCH_ALIGN,
+    total = g.gpj_gen_files(f, sch.ctypes.data, tg.ctypes.data, seed, threads, align,
                             offsets.ctypes.data, sizes.ctypes.data, ctypes.byref(ptr))
     if total == 0:
         raise RuntimeError("gpj_gen_files failed")

@@ -437,10 +437,12 @@ uint64_t gpj_gen(int schema, uint64_t target, uint64_t seed, int threads, uint8_
   return total <= cap ? total : 0;
 }
 
-/* ---- many files (the batched configuration) ------------------------------
+/* ---- many files (the batched and serving configurations) -----------------
  * File f: NDJSON of schema schemas[f] (0, 2 or 3) from its own generator
  * stream (seed, stream 1000 + f), records appended while the file stays
- * within targets[f] bytes, but at least one record.  The files are laid out
+ * within targets[f] bytes, but at least one record; or, for schema 1, one
+ * top-level array document of schema-1 elements within targets[f] bytes (at
+ * least one element).  The files are laid out
  * in one zero-filled buffer, each at an offset that is a multiple of
  * `align`; offsets_out / sizes_out receive the layout.  Returns the buffer
  * size (0 on allocation failure); free the buffer with gpj_gen_free. */
@@ -465,6 +467,19 @@ static void *files_worker(void *arg) {
     rng_t r;
     seed_rng(&r, j->seed, 1000u + (uint64_t)f);
     buf_t *out = &j->bufs[f];
+    if (j->schemas[f] == 1) {
+      /* one top-level array document: '[' elements ']', no newline */
+      putc_(out, '[');
+      for (int64_t id = 0;; ++id) {
+        tmp.len = 0;
+        element1(&tmp, &r, id);
+        if (id > 0 && out->len + 1 + tmp.len + 1 > j->targets[f]) break;
+        if (id) putc_(out, ',');
+        put(out, (const char *)tmp.p, tmp.len);
+      }
+      putc_(out, ']');
+      continue;
+    }
     for (;;) {
       tmp.len = 0;
       if (j->schemas[f] == 0) record0(&tmp, &r);

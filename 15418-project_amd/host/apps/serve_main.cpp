@@ -1,12 +1,17 @@
 // gpjson_serve: Poisson request replay against gpjson::serve::Server.
 //
-// Requests: documents of 4 KiB .. 64 MiB (log-uniform) from the NDJSON
-// generator (record schemas of the 64 MiB plain and the 8 GiB throughput
-// configurations), 1..8 queries each drawn from those configurations' query
-// sets, a share of requests (default 30%) re-querying an earlier document.
-// For every arrival rate it prints one JSON line: p50/p99 latency (arrival to
-// results on the host), sustained GB/s, resident hits, evictions, and whether
-// every verified request's results equal the same request served alone.
+// Requests: documents of 4 KiB .. 64 MiB (log-uniform) from the generator,
+// one of three record schemas with equal odds (NDJSON records of the 64 MiB
+// plain configuration, one top-level array of the 1 GiB single-document
+// configuration, NDJSON records of the 8 GiB throughput configuration), 1..8
+// queries each drawn from that configuration's query set, a share of
+// requests (default 30%) re-querying an earlier document.  For every arrival
+// rate it prints one JSON line: p50/p99 latency (arrival to results on the
+// host), sustained GB/s, resident hits, evictions, retries, and whether every
+// verified request's results equal the same request served alone (every
+// request at every rate unless --verify limits it).  --digest-out writes the
+// document manifest and one digest line per request and query, from which a
+// checker can regenerate the documents and compare every result.
 #include "gpjson/serve/server.hpp"
 
 #include "gpj_capi.h"
@@ -42,9 +47,11 @@ struct Args {
   double requery = 0.3;
   double budget_gib = 80;
   int slots = 8;
-  int capacity = 8;
-  int verify = -1; // requests of the last rate checked against run_alone (-1 = all)
+  int spans_per_record = 4;
+  uint64_t bytes_per_line = 256, bytes_per_element = 64;
+  int verify = -1; // requests per rate checked against run_alone (-1 = all)
   int threads = 16;
+  std::string digest_out;
   int64_t wait_limit = -1; // index tiles' carry-wait polls (-1: default)
 };
 
@@ -82,7 +89,10 @@ Args parse(int argc, char **argv) {
     else if (k == "--requery") a.requery = std::stod(val());
     else if (k == "--budget-gib") a.budget_gib = std::stod(val());
     else if (k == "--slots") a.slots = std::stoi(val());
-    else if (k == "--capacity") a.capacity = std::stoi(val());
+    else if (k == "--spans-per-record") a.spans_per_record = std::stoi(val());
+    else if (k == "--digest-out") a.digest_out = val();
+    else if (k == "--bytes-per-line") a.bytes_per_line = std::stoull(val());
+    else if (k == "--bytes-per-element") a.bytes_per_element = std::stoull(val());
     else if (k == "--verify") a.verify = std::stoi(val());
     else if (k == "--threads") a.threads = std::stoi(val());
     else if (k == "--debug-wait-limit") a.wait_limit = std::stoll(val());
@@ -91,12 +101,35 @@ Args parse(int argc, char **argv) {
   return a;
 }
 
-// query sets of the 64 MiB plain (schema 0) and 8 GiB throughput (schema 2)
-// configurations; indices into the server's table
-const std::vector<std::string> kQueries = {"$.a.b", "$.user.lang", "$.e[0].f", "$.h[*].t",
-                                           "$.h[?(@.k == \"v\")].id"};
+// query sets of the 64 MiB plain (schema 0), 1 GiB single-document (schema
+// 1, element queries) and 8 GiB throughput (schema 2) configurations;
+// indices into the server's table
+const std::vector<std::string> kQueries = {"$.a.b", "$[*].items[*].sku", "$.user.lang",
+                                           "$.e[0].f", "$.h[*].t", "$.h[?(@.k == \"v\")].id"};
 const std::vector<int> kSchema0 = {0};
-const std::vector<int> kSchema2 = {1, 2, 3, 4};
+const std::vector<int> kSchema1 = {1};
+const std::vector<int> kSchema2 = {2, 3, 4, 5};
+const uint64_t kAlign = 64;
+
+// Digest of a result array: sum of value * odd weight (SplitMix64 of the
+// index), modulo 2^64.
+uint64_t weight(uint64_t i) {
+  uint64_t z = i + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return (z ^ (z >> 31)) | 1ull;
+}
+uint64_t digest(const std::vector<int64_t> &v) {
+  uint64_t h = 0;
+  for (size_t i = 0; i < v.size(); ++i) h += (uint64_t)v[i] * weight(i);
+  return h;
+}
+
+template <typename T> std::string json_list(const std::vector<T> &v) {
+  std::string s = "[";
+  for (size_t i = 0; i < v.size(); ++i) s += (i ? "," : "") + std::to_string(v[i]);
+  return s + "]";
+}
 
 double pct(std::vector<double> v, double p) {
   if (v.empty()) return 0;
@@ -107,9 +140,9 @@ double pct(std::vector<double> v, double p) {
 }
 
 bool same(const RequestResult &a, const RequestResult &b) {
-  if (a.num_lines != b.num_lines || a.outputs.size() != b.outputs.size()) return false;
+  if (a.num_records != b.num_records || a.outputs.size() != b.outputs.size()) return false;
   for (size_t q = 0; q < a.outputs.size(); ++q)
-    if (a.outputs[q].spans != b.outputs[q].spans || a.outputs[q].counts != b.outputs[q].counts)
+    if (a.outputs[q].offsets != b.outputs[q].offsets || a.outputs[q].spans != b.outputs[q].spans)
       return false;
   return true;
 }
@@ -140,12 +173,14 @@ int main(int argc, char **argv) {
         s.doc = (int)(U(rng) * (double)schemas.size()) % (int)schemas.size();
       } else {
         s.doc = (int)schemas.size();
-        schemas.push_back(U(rng) < 0.5 ? 0 : 2);
+        const double pick = U(rng);
+        schemas.push_back(pick < 1.0 / 3 ? 0 : (pick < 2.0 / 3 ? 1 : 2));
         const double lg = std::log((double)a.min_doc) +
                           U(rng) * (std::log((double)a.max_doc) - std::log((double)a.min_doc));
         targets.push_back((uint64_t)std::exp(lg));
       }
-      const std::vector<int> &pool = schemas[s.doc] == 0 ? kSchema0 : kSchema2;
+      const std::vector<int> &pool =
+          schemas[s.doc] == 0 ? kSchema0 : (schemas[s.doc] == 1 ? kSchema1 : kSchema2);
       const int k = 1 + (int)(U(rng) * 8) % 8;
       for (int j = 0; j < k; ++j) s.queries.push_back(pool[(size_t)(U(rng) * pool.size()) % pool.size()]);
       shapes.push_back(std::move(s));
@@ -153,7 +188,8 @@ int main(int argc, char **argv) {
     std::vector<uint64_t> off(schemas.size()), sz(schemas.size());
     uint8_t *gen = nullptr;
     const uint64_t total = gpj_gen_files((int32_t)schemas.size(), schemas.data(), targets.data(),
-                                         a.seed, a.threads, 64, off.data(), sz.data(), &gen);
+                                         a.seed, a.threads, kAlign, off.data(), sz.data(),
+                                         &gen);
     if (!total) throw std::runtime_error("document generation failed");
     uint8_t *docs = nullptr; // pinned, so uploads overlap other slots' kernels
     if (gpj_host_alloc(reinterpret_cast<void **>(&docs), total) < 0)
@@ -164,6 +200,30 @@ int main(int argc, char **argv) {
     for (auto v : sz) doc_bytes_total += v;
     std::fprintf(stderr, "[serve] %d requests over %zu documents (%llu bytes)\n", a.requests,
                  schemas.size(), (unsigned long long)doc_bytes_total);
+    FILE *dig = nullptr;
+    if (!a.digest_out.empty()) {
+      dig = std::fopen(a.digest_out.c_str(), "w");
+      if (!dig) throw std::runtime_error("cannot write " + a.digest_out);
+      std::string qs = "[";
+      for (size_t i = 0; i < kQueries.size(); ++i) {
+        std::string esc;
+        for (char c : kQueries[i]) esc += (c == '"' ? std::string("\\\"") : std::string(1, c));
+        qs += (i ? ",\"" : "\"") + esc + "\"";
+      }
+      qs += "]";
+      std::fprintf(dig,
+                   "{\"manifest\": {\"seed\": %llu, \"align\": %llu, \"levels\": %d, "
+                   "\"queries\": %s, \"schemas\": %s, \"targets\": %s, \"offsets\": %s, "
+                   "\"sizes\": %s}}\n",
+                   (unsigned long long)a.seed, (unsigned long long)kAlign,
+                   [&] {
+                     int lv = 1;
+                     for (const auto &q : table) lv = std::max(lv, q.max_depth + (q.elements ? 1 : 0));
+                     return lv;
+                   }(),
+                   qs.c_str(), json_list(schemas).c_str(), json_list(targets).c_str(),
+                   json_list(off).c_str(), json_list(sz).c_str());
+    }
 
     for (size_t ri = 0; ri < a.rates.size(); ++ri) {
       const double rate = a.rates[ri];
@@ -189,7 +249,9 @@ int main(int argc, char **argv) {
       gpjson::serve::ServerOptions so;
       so.hbm_budget_bytes = (uint64_t)(a.budget_gib * (double)(1ull << 30));
       so.slots = a.slots;
-      so.result_capacity = a.capacity;
+      so.spans_per_record = a.spans_per_record;
+      so.bytes_per_line = a.bytes_per_line;
+      so.bytes_per_element = a.bytes_per_element;
       gpjson::serve::Server server(table, so);
       const std::vector<RequestResult> res = server.run(reqs);
       std::vector<double> lat;
@@ -203,10 +265,21 @@ int main(int argc, char **argv) {
       }
       const double span = last - first;
       int checked = 0, identical = 1;
-      const bool last_rate = ri + 1 == a.rates.size();
-      const int want = last_rate ? (a.verify < 0 ? a.requests : std::min(a.verify, a.requests))
-                                 : std::min(a.requests, 50);
+      const int want = a.verify < 0 ? a.requests : std::min(a.verify, a.requests);
       server.clear_resident();
+      if (dig)
+        for (int i = 0; i < a.requests; ++i) {
+          std::vector<uint64_t> d;
+          for (const auto &o : res[i].outputs) {
+            d.push_back(digest(o.offsets));
+            d.push_back(digest(o.spans));
+          }
+          std::fprintf(dig,
+                       "{\"rate\": %.0f, \"request\": %d, \"doc\": %d, \"queries\": %s, "
+                       "\"records\": %lld, \"digests\": %s}\n",
+                       rate, i, shapes[i].doc, json_list(shapes[i].queries).c_str(),
+                       (long long)res[i].num_records, json_list(d).c_str());
+        }
       for (int i = 0; i < want; ++i) {
         const RequestResult alone = server.run_alone(reqs[i]);
         if (!same(alone, res[i])) {
@@ -232,6 +305,7 @@ int main(int argc, char **argv) {
       std::fflush(stdout);
       if (!identical) return 3;
     }
+    if (dig) std::fclose(dig);
     gpj_host_free(docs);
     return 0;
   } catch (const std::exception &e) {

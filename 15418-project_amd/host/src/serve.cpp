@@ -1,5 +1,6 @@
-// Request server: slot admission under an HBM budget, resident documents,
-// overlapping per-slot streams.  See gpjson/serve/server.hpp.
+// Request server: slot admission under an HBM budget, resident documents
+// and indices, overlapping per-slot streams, dense results.  See
+// gpjson/serve/server.hpp.
 #include "gpjson/serve/server.hpp"
 
 #include "gpj_capi.h"
@@ -11,6 +12,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -28,10 +30,18 @@ void ok(gpj_status s, const char *what) {
 uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 uint64_t round256(uint64_t a) { return div_up(a, 256) * 256; }
 
-// Newline-index capacity assumed before the count is known: records of 256
-// bytes and more on average.  A document with more lines is rebuilt with its
-// exact count (counted as a retry).
-uint64_t newline_hint(uint64_t n) { return n / 256 + 2; }
+// Newline-index capacity assumed before the count is known: lines of
+// `per_line` bytes and more on average.  A document with more lines is
+// rebuilt with its exact count (counted as a retry).
+uint64_t newline_hint(uint64_t n, uint64_t per_line) {
+  return n / std::max<uint64_t>(per_line, 1) + 2;
+}
+
+// Record-bound capacity assumed for a top-level array before the count is
+// known: elements of `per_element` bytes and more on average, plus '[' and ']'.
+uint64_t bounds_hint(uint64_t n, uint64_t per_element) {
+  return n / std::max<uint64_t>(per_element, 1) + 3;
+}
 
 // True when the bytecode holds an extension opcode (MOVE_TO_EACH or
 // JUMP_IF_NOT_ARRAY), i.e. it runs in variable-length mode.
@@ -72,9 +82,15 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 QuerySpec compile_query_spec(const std::string &text) {
   QuerySpec q;
   q.text = text;
+  std::string src = text;
+  const size_t lead = text.find_first_not_of(" \t");
+  if (lead != std::string::npos && text.compare(lead, 4, "$[*]") == 0) {
+    q.elements = true; // runs as `$<rest>` on each element of a top-level array
+    src = "$" + text.substr(lead + 4);
+  }
   std::vector<uint8_t> buf(GPJ_MAX_QUERY_IR_BYTES * 4);
   uint32_t len = 0, depth = 0, slots = 0;
-  ok(gpj_query_compile(text.c_str(), buf.data(), (uint32_t)buf.size(), &len, &depth, &slots,
+  ok(gpj_query_compile(src.c_str(), buf.data(), (uint32_t)buf.size(), &len, &depth, &slots,
                        GPJ_COMPILE_EXTENSIONS),
      "query compile");
   q.ir.assign(buf.begin(), buf.begin() + len);
@@ -89,7 +105,8 @@ struct Server::Impl {
     uint8_t *p = nullptr;
     uint64_t bytes = 0;
   };
-  // A document on the device with its index (one allocation).
+  // A document on the device with its index (one allocation) and, once an
+  // element query needs them, the bounds of its top-level array's elements.
   struct Doc {
     uint64_t id = 0, bytes = 0, words = 0, nl_cap = 0, footprint = 0;
     uint8_t *block = nullptr;
@@ -97,19 +114,32 @@ struct Server::Impl {
     uint64_t *d_string = nullptr, *d_leveled = nullptr;
     int64_t *d_newline = nullptr, *d_lines = nullptr;
     void *ready = nullptr; // recorded after the index build
+    uint8_t *bounds_block = nullptr;
+    int64_t *d_bounds = nullptr, *d_nbounds = nullptr;
+    uint64_t bounds_cap = 0, bounds_bytes = 0;
+    void *bounds_ready = nullptr;
     int users = 0;
     uint64_t last_use = 0;
   };
+  // Layout of one request's device results: per query, CSR offsets
+  // [max_records + 1] and spans [caps[q]][2]; then the dense scratch.
+  struct Layout {
+    int64_t max_records = 0;
+    std::vector<uint64_t> caps;
+    uint64_t bytes = 0;
+  };
   struct Slot {
     void *stream = nullptr, *ev_a = nullptr, *ev_b = nullptr;
-    void *scratch = nullptr;
+    void *scratch = nullptr; // index scratch
     uint64_t scratch_bytes = 0;
     int stage = 0; // 0 idle, 1 computing, 2 copying results back
     size_t req = 0;
     Doc *doc = nullptr;
     uint8_t *d_results = nullptr;
-    uint64_t results_bytes = 0; // budget reserved for d_results
-    int64_t *h_lines = nullptr; // pinned
+    Layout lay;
+    uint64_t reserved = 0; // budget reserved for d_results and new bounds
+    int64_t *h_small = nullptr; // pinned: line count, bound count, totals[8]
+    int64_t records = 0;
     Pinned h_results;
     double start_s = 0;
     bool resident = false;
@@ -121,7 +151,9 @@ struct Server::Impl {
   int levels = 1;
   ServerStats stats;
   std::unordered_map<uint64_t, Doc *> docs;
-  std::unordered_map<uint64_t, uint64_t> exact_caps; // doc id -> line count
+  std::unordered_map<uint64_t, uint64_t> exact_caps;   // doc id -> line count
+  std::unordered_map<uint64_t, uint64_t> exact_bounds; // doc id -> bound count
+  std::map<std::pair<uint64_t, int>, uint64_t> exact_spans; // (doc, query) -> spans
   std::vector<Slot> slots;
   uint64_t used = 0, tick = 0;
 
@@ -189,9 +221,9 @@ struct Server::Impl {
 
   Impl(std::vector<QuerySpec> q, ServerOptions o) : queries(std::move(q)), opt(o) {
     if (opt.slots < 1) opt.slots = 1;
-    if (opt.result_capacity < 1) opt.result_capacity = 1;
+    if (opt.spans_per_record < 1) opt.spans_per_record = 1;
     int deepest = 1;
-    for (const auto &qs : queries) deepest = std::max(deepest, qs.max_depth);
+    for (const auto &qs : queries) deepest = std::max(deepest, qs.max_depth + (qs.elements ? 1 : 0));
     levels = opt.index_levels > 0 ? opt.index_levels : deepest;
     for (const auto &qs : queries) {
       void *d = nullptr;
@@ -205,7 +237,8 @@ struct Server::Impl {
       ok(gpj_stream_create(&s.stream), "stream");
       ok(gpj_event_create(&s.ev_a), "event");
       ok(gpj_event_create(&s.ev_b), "event");
-      ok(gpj_host_alloc(reinterpret_cast<void **>(&s.h_lines), 8), "pinned alloc");
+      ok(gpj_host_alloc(reinterpret_cast<void **>(&s.h_small), 8 * (2 + GPJ_MAX_BATCH_QUERIES)),
+         "pinned alloc");
     }
     for (int i = 0; i < 4; ++i) workers.emplace_back([this] { worker_loop(); });
   }
@@ -223,7 +256,7 @@ struct Server::Impl {
     docs.clear();
     for (auto &s : slots) {
       (void)gpj_free(s.scratch);
-      (void)gpj_host_free(s.h_lines);
+      (void)gpj_host_free(s.h_small);
       (void)gpj_event_destroy(s.ev_a);
       (void)gpj_event_destroy(s.ev_b);
       (void)gpj_stream_destroy(s.stream);
@@ -232,24 +265,65 @@ struct Server::Impl {
     (void)gpj_device_sync();
   }
 
+  bool elements_of(const Request &r) const {
+    bool el = false, line = false;
+    for (int qi : r.queries) (queries[qi].elements ? el : line) = true;
+    if (el && line)
+      throw std::invalid_argument("a request mixes element and line queries");
+    return el;
+  }
   uint64_t cap_of(const Request &r) const {
     auto it = exact_caps.find(r.doc_id);
-    return it != exact_caps.end() ? (uint64_t)it->second : newline_hint(r.doc_bytes);
+    return it != exact_caps.end() ? (uint64_t)it->second
+                                  : newline_hint(r.doc_bytes, opt.bytes_per_line);
+  }
+  uint64_t bounds_cap_of(const Request &r) const {
+    auto it = exact_bounds.find(r.doc_id);
+    return it != exact_bounds.end() ? std::max<uint64_t>(it->second, 1)
+                                    : bounds_hint(r.doc_bytes, opt.bytes_per_element);
   }
   uint64_t doc_footprint(uint64_t bytes, uint64_t nl_cap) const {
     const uint64_t w = div_up(bytes, 64);
     return round256(bytes) + round256(w * 8) + round256(w * 8 * (uint64_t)levels) +
            round256(nl_cap * 8) + 256;
   }
-  int cap_q(const QuerySpec &q) const { return q.variable ? opt.result_capacity : q.slots; }
-  uint64_t results_footprint(const Request &r, uint64_t nl_cap) const {
-    uint64_t b = 0;
-    for (int qi : r.queries) b += round256(nl_cap * 16 * (uint64_t)cap_q(queries[qi])) +
-                                  round256(nl_cap * 8);
-    return b;
+  uint64_t bounds_footprint(uint64_t doc_bytes, uint64_t cap) const {
+    return round256(cap * 8) + 256 +
+           round256(gpj_row_bounds_scratch_bytes((int64_t)div_up(doc_bytes, 64)));
+  }
+  // The results layout of request r over max_records records.
+  Layout layout(const Request &r, int64_t max_records) const {
+    Layout l;
+    l.max_records = max_records;
+    uint64_t total = 0;
+    for (int qi : r.queries) {
+      const QuerySpec &q = queries[qi];
+      uint64_t cap;
+      auto it = exact_spans.find({r.doc_id, qi});
+      if (it != exact_spans.end()) cap = it->second;
+      else if (!q.variable) cap = (uint64_t)max_records * (uint64_t)q.slots; // at most slots each
+      else cap = (uint64_t)max_records * (uint64_t)opt.spans_per_record + 16;
+      cap = std::max<uint64_t>(cap, 1);
+      l.caps.push_back(cap);
+      l.bytes += round256(((uint64_t)max_records + 1) * 8) + round256(cap * 16);
+      total += cap;
+    }
+    l.bytes += round256(gpj_query_dense_scratch_bytes(max_records, (int32_t)r.queries.size(),
+                                                      (int64_t)total));
+    return l;
   }
 
+  void release_bounds(Doc *d, void *stream) {
+    if (d->bounds_block) (void)gpj_free_async(d->bounds_block, stream);
+    if (d->bounds_ready) (void)gpj_event_destroy(d->bounds_ready);
+    used -= d->bounds_bytes;
+    d->bounds_block = nullptr;
+    d->d_bounds = d->d_nbounds = nullptr;
+    d->bounds_ready = nullptr;
+    d->bounds_bytes = d->bounds_cap = 0;
+  }
   void release_doc(Doc *d, void *stream) {
+    release_bounds(d, stream);
     if (d->block) (void)gpj_free_async(d->block, stream);
     if (d->ready) (void)gpj_event_destroy(d->ready);
     used -= d->footprint;
@@ -307,79 +381,140 @@ struct Server::Impl {
     return d;
   }
 
-  // Enqueues the query launch of request r over doc d into d_results.
-  void enqueue_queries(const Request &r, Doc *d, uint8_t *d_results, void *stream) {
-    std::vector<gpj_query_desc> descs;
-    uint8_t *p = d_results;
-    for (int qi : r.queries) {
-      const QuerySpec &q = queries[qi];
-      const int cap = cap_q(q);
-      gpj_query_desc desc{};
-      desc.d_ir = static_cast<const uint8_t *>(d_irs[qi]);
-      desc.ir_len = (uint32_t)q.ir.size();
-      desc.capacity = cap;
-      desc.variable = q.variable ? 1 : 0;
-      desc.max_level = q.max_depth;
-      desc.d_spans = reinterpret_cast<int64_t *>(p);
-      p += round256(d->nl_cap * 16 * (uint64_t)cap);
-      desc.d_counts = reinterpret_cast<int64_t *>(p);
-      p += round256(d->nl_cap * 8);
-      descs.push_back(desc);
-    }
-    gpj_index idx{d->d_newline, d->d_lines, (int64_t)d->nl_cap, d->d_string, d->d_leveled,
-                  (int64_t)d->words, levels, 0};
-    // up to 8 queries per launch
-    for (size_t b = 0; b < descs.size(); b += GPJ_MAX_BATCH_QUERIES) {
-      const int cnt = (int)std::min<size_t>(GPJ_MAX_BATCH_QUERIES, descs.size() - b);
-      ok(gpj_query_run_batch(d->d_in, d->bytes, &idx, descs.data() + b, cnt,
-                             (int64_t)d->nl_cap, stream),
-         "query launch");
-    }
+  gpj_index index_view(const Doc *d) const {
+    return gpj_index{d->d_newline, d->d_lines, (int64_t)d->nl_cap, d->d_string, d->d_leveled,
+                     (int64_t)d->words, levels, 0};
   }
 
-  uint64_t host_results_bytes(const Request &r, int64_t lines) const {
+  // Enqueues on `stream` (after the index): the bounds of the document's
+  // top-level array elements, cap entries at most.
+  void build_bounds(Doc *d, uint64_t cap, void *stream) {
+    d->bounds_cap = cap;
+    d->bounds_bytes = bounds_footprint(d->bytes, cap);
+    ok(gpj_alloc_async(reinterpret_cast<void **>(&d->bounds_block), d->bounds_bytes, stream),
+       "bounds alloc");
+    used += d->bounds_bytes;
+    uint8_t *p = d->bounds_block;
+    d->d_bounds = reinterpret_cast<int64_t *>(p);
+    p += round256(cap * 8);
+    d->d_nbounds = reinterpret_cast<int64_t *>(p);
+    p += 256;
+    const gpj_index idx = index_view(d);
+    ok(gpj_row_bounds(&idx, d->d_bounds, (int64_t)cap, d->d_nbounds, p, stream), "row bounds");
+    ok(gpj_event_create(&d->bounds_ready), "event");
+    ok(gpj_event_record(d->bounds_ready, stream), "event record");
+  }
+
+  // Enqueues the dense query launch of request r over doc d into d_results,
+  // then the copies of the line count, the bound count and each query's total.
+  void enqueue_queries(const Request &r, Doc *d, const Layout &lay, uint8_t *d_results,
+                       int64_t *h_small, void *stream) {
+    std::vector<gpj_query_desc> descs;
+    std::vector<gpj_dense_result> outs;
+    uint8_t *p = d_results;
+    for (size_t k = 0; k < r.queries.size(); ++k) {
+      const QuerySpec &q = queries[r.queries[k]];
+      gpj_query_desc desc{};
+      desc.d_ir = static_cast<const uint8_t *>(d_irs[r.queries[k]]);
+      desc.ir_len = (uint32_t)q.ir.size();
+      desc.capacity = q.variable ? 1 : q.slots;
+      desc.variable = q.variable ? 1 : 0;
+      desc.max_level = q.max_depth;
+      descs.push_back(desc);
+      gpj_dense_result o{};
+      o.d_offsets = reinterpret_cast<int64_t *>(p);
+      p += round256(((uint64_t)lay.max_records + 1) * 8);
+      o.d_spans = reinterpret_cast<int64_t *>(p);
+      p += round256(lay.caps[k] * 16);
+      o.capacity = (int64_t)lay.caps[k];
+      outs.push_back(o);
+    }
+    const gpj_index idx = index_view(d);
+    if (elements_of(r))
+      ok(gpj_query_run_records_dense(d->d_in, d->bytes, &idx, d->d_bounds, d->d_nbounds,
+                                     descs.data(), (int32_t)descs.size(), lay.max_records,
+                                     outs.data(), p, stream),
+         "element query launch");
+    else
+      ok(gpj_query_run_dense(d->d_in, d->bytes, &idx, descs.data(), (int32_t)descs.size(),
+                             lay.max_records, outs.data(), p, stream),
+         "query launch");
+    ok(gpj_d2h_async(h_small, d->d_lines, 8, stream), "line count copy");
+    if (d->d_nbounds) ok(gpj_d2h_async(h_small + 1, d->d_nbounds, 8, stream), "bound count copy");
+    for (size_t k = 0; k < outs.size(); ++k)
+      ok(gpj_d2h_async(h_small + 2 + k, outs[k].d_offsets + lay.max_records, 8, stream),
+         "total copy");
+  }
+
+  // Reads the small copies of a finished query launch: the record count, or
+  // -1 when a capacity guess was too small (the exact size is noted for the
+  // rerun).  A failed index build throws.
+  int64_t settle(const Request &r, const Doc *d, const Layout &lay, const int64_t *h_small,
+                 bool *drop_doc) {
+    const int64_t lines = h_small[0];
+    check_lines(lines);
+    *drop_doc = false;
+    if ((uint64_t)lines > d->nl_cap) { // rebuild the index with the exact count
+      exact_caps[r.doc_id] = (uint64_t)lines;
+      *drop_doc = true;
+      return -1;
+    }
+    int64_t records = lines;
+    if (elements_of(r)) {
+      const int64_t nb = h_small[1];
+      if ((uint64_t)nb > d->bounds_cap) { // rebuild the bounds with the exact count
+        exact_bounds[r.doc_id] = (uint64_t)nb;
+        *drop_doc = true;
+        return -1;
+      }
+      records = std::max<int64_t>(nb - 1, 0);
+    }
+    bool fits = true;
+    for (size_t k = 0; k < r.queries.size(); ++k)
+      if ((uint64_t)h_small[2 + k] > lay.caps[k]) {
+        exact_spans[{r.doc_id, r.queries[k]}] = (uint64_t)h_small[2 + k];
+        fits = false;
+      }
+    return fits ? records : -1;
+  }
+
+  uint64_t host_results_bytes(const Request &r, int64_t records, const int64_t *h_small) const {
     uint64_t b = 0;
-    for (int qi : r.queries) b += (uint64_t)lines * 16 * (uint64_t)cap_q(queries[qi]) +
-                                  (uint64_t)lines * 8;
+    for (size_t k = 0; k < r.queries.size(); ++k)
+      b += ((uint64_t)records + 1) * 8 + (uint64_t)h_small[2 + k] * 16;
     return b;
   }
 
-  // Enqueues the copy-back of the first `lines` rows of every query's results.
-  void enqueue_copy_back(const Request &r, Doc *d, const uint8_t *d_results, int64_t lines,
-                         uint8_t *h, void *stream) {
+  // Enqueues the copy-back of each query's offsets [records + 1] and spans.
+  void enqueue_copy_back(const Request &r, const Layout &lay, const uint8_t *d_results,
+                         int64_t records, const int64_t *h_small, uint8_t *h, void *stream) {
     const uint8_t *p = d_results;
-    for (int qi : r.queries) {
-      const int cap = cap_q(queries[qi]);
-      const uint64_t sp = (uint64_t)lines * 16 * (uint64_t)cap, ct = (uint64_t)lines * 8;
-      ok(gpj_d2h_async(h, p, sp, stream), "results copy");
-      h += sp;
-      p += round256(d->nl_cap * 16 * (uint64_t)cap);
-      ok(gpj_d2h_async(h, p, ct, stream), "counts copy");
-      h += ct;
-      p += round256(d->nl_cap * 8);
+    for (size_t k = 0; k < r.queries.size(); ++k) {
+      const uint64_t ob = ((uint64_t)records + 1) * 8, sb = (uint64_t)h_small[2 + k] * 16;
+      ok(gpj_d2h_async(h, p, ob, stream), "offsets copy");
+      h += ob;
+      p += round256(((uint64_t)lay.max_records + 1) * 8);
+      ok(gpj_d2h_async(h, p, sb, stream), "spans copy");
+      h += sb;
+      p += round256(lay.caps[k] * 16);
     }
   }
 
-  void unpack(const Request &r, int64_t lines, const uint8_t *h, RequestResult &out) const {
-    out.num_lines = lines;
+  void unpack(const Request &r, int64_t records, const int64_t *totals, const uint8_t *h,
+              RequestResult &out) const {
+    out.num_records = records;
     out.outputs.clear();
-    for (int qi : r.queries) {
-      const int cap = cap_q(queries[qi]);
+    for (size_t k = 0; k < r.queries.size(); ++k) {
       QueryOutput o;
-      o.spans.resize((size_t)lines * cap * 2);
-      o.counts.resize((size_t)lines);
+      o.offsets.resize((size_t)records + 1);
+      o.spans.resize((size_t)totals[k] * 2);
+      std::memcpy(o.offsets.data(), h, o.offsets.size() * 8);
+      h += o.offsets.size() * 8;
       std::memcpy(o.spans.data(), h, o.spans.size() * 8);
       h += o.spans.size() * 8;
-      std::memcpy(o.counts.data(), h, o.counts.size() * 8);
-      h += o.counts.size() * 8;
-      if (queries[qi].variable) // entries past a line's count are unspecified
-        for (int64_t l = 0; l < lines; ++l)
-          for (int64_t k = std::max<int64_t>(0, o.counts[l]); k < cap; ++k)
-            o.spans[(size_t)(l * cap + k) * 2] = o.spans[(size_t)(l * cap + k) * 2 + 1] = -1;
       out.outputs.push_back(std::move(o));
     }
   }
-
 
   std::vector<RequestResult> run(const std::vector<Request> &reqs) {
     std::vector<RequestResult> out(reqs.size());
@@ -390,8 +525,10 @@ struct Server::Impl {
     for (size_t i = 0; i < reqs.size(); ++i) {
       out[i].id = reqs[i].id;
       out[i].arrival_s = reqs[i].arrival_s;
-      if (reqs[i].queries.empty() || reqs[i].doc_bytes == 0)
-        throw std::invalid_argument("request without queries or document bytes");
+      if (reqs[i].queries.empty() || reqs[i].queries.size() > GPJ_MAX_BATCH_QUERIES ||
+          reqs[i].doc_bytes == 0)
+        throw std::invalid_argument("request without 1..8 queries or document bytes");
+      (void)elements_of(reqs[i]);
     }
     std::deque<size_t> waiting;
     size_t next = 0, done = 0;
@@ -413,11 +550,16 @@ struct Server::Impl {
           }
         if (!s) break;
         const Request &r = reqs[waiting.front()];
+        const bool el = elements_of(r);
         auto it = docs.find(r.doc_id);
         Doc *d = it != docs.end() ? it->second : nullptr;
-        const uint64_t cap = d ? d->nl_cap : cap_of(r);
-        const uint64_t res_bytes = results_footprint(r, cap);
-        const uint64_t need = res_bytes + (d ? 0 : doc_footprint(r.doc_bytes, cap));
+        const uint64_t nl_cap = d ? d->nl_cap : cap_of(r);
+        const bool new_bounds = el && (!d || !d->d_bounds);
+        const uint64_t b_cap = el ? (new_bounds ? bounds_cap_of(r) : d->bounds_cap) : 0;
+        const int64_t max_records = el ? (int64_t)b_cap - 1 : (int64_t)nl_cap;
+        Layout lay = layout(r, max_records);
+        const uint64_t extra = lay.bytes + (new_bounds ? bounds_footprint(r.doc_bytes, b_cap) : 0);
+        const uint64_t need = extra + (d ? 0 : doc_footprint(r.doc_bytes, nl_cap));
         if (d) ++d->users; // pin before making room
         const bool fits = make_room(need, s->stream);
         if (!fits) {
@@ -433,20 +575,22 @@ struct Server::Impl {
         if (d) {
           ++stats.resident_hits;
           ok(gpj_stream_wait_event(s->stream, d->ready), "wait for index");
+          if (d->bounds_ready) ok(gpj_stream_wait_event(s->stream, d->bounds_ready), "wait");
         } else {
           d = build_doc(r, s->stream, &s->scratch, &s->scratch_bytes);
           d->users = 1;
           if (opt.keep_resident) docs[d->id] = d;
         }
+        if (new_bounds) build_bounds(d, b_cap, s->stream);
         s->doc = d;
         d->last_use = ++tick;
-        ok(gpj_alloc_async(reinterpret_cast<void **>(&s->d_results), res_bytes, s->stream),
+        ok(gpj_alloc_async(reinterpret_cast<void **>(&s->d_results), lay.bytes, s->stream),
            "results alloc");
-        s->results_bytes = res_bytes;
-        used += res_bytes;
+        s->lay = std::move(lay);
+        s->reserved = s->lay.bytes;
+        used += s->reserved;
         stats.peak_bytes = std::max(stats.peak_bytes, used);
-        enqueue_queries(r, d, s->d_results, s->stream);
-        ok(gpj_d2h_async(s->h_lines, d->d_lines, 8, s->stream), "line count copy");
+        enqueue_queries(r, d, s->lay, s->d_results, s->h_small, s->stream);
         ok(gpj_event_record(s->ev_a, s->stream), "event record");
         s->stage = 1;
         s->start_s = now_s(t0);
@@ -462,18 +606,17 @@ struct Server::Impl {
           if (q == 0) continue;
           progress = true;
           const Request &r = reqs[s.req];
-          const int64_t lines = *s.h_lines;
-          check_lines(lines);
-          if ((uint64_t)lines > s.doc->nl_cap) {
-            // capacity hint too small: rebuild this document exactly, rerun
+          bool drop_doc = false;
+          const int64_t records = settle(r, s.doc, s.lay, s.h_small, &drop_doc);
+          if (records < 0) { // a capacity guess was too small: rerun with exact sizes
             ++stats.retries;
-            exact_caps[r.doc_id] = (uint64_t)lines;
-            finish_slot(s, /*drop_doc=*/true);
+            finish_slot(s, drop_doc);
             waiting.push_front(&r - reqs.data());
             continue;
           }
-          s.h_results = take_pinned(host_results_bytes(r, lines));
-          enqueue_copy_back(r, s.doc, s.d_results, lines, s.h_results.p, s.stream);
+          s.records = records;
+          s.h_results = take_pinned(host_results_bytes(r, records, s.h_small));
+          enqueue_copy_back(r, s.lay, s.d_results, records, s.h_small, s.h_results.p, s.stream);
           ok(gpj_event_record(s.ev_b, s.stream), "event record");
           s.stage = 2;
         } else if (s.stage == 2) {
@@ -485,11 +628,12 @@ struct Server::Impl {
           RequestResult &res = out[s.req];
           res.resident = s.resident;
           res.done_s = now_s(t0); // results are in host memory
-          const int64_t lines = *s.h_lines;
+          const int64_t records = s.records;
           const Pinned buf = s.h_results;
+          std::vector<int64_t> totals(s.h_small + 2, s.h_small + 2 + r.queries.size());
           s.h_results = Pinned{};
-          post([this, &r, &res, lines, buf] {
-            unpack(r, lines, buf.p, res);
+          post([this, &r, &res, records, totals, buf] {
+            unpack(r, records, totals.data(), buf.p, res);
             give_pinned(buf);
           });
           finish_slot(s, /*drop_doc=*/false);
@@ -505,9 +649,9 @@ struct Server::Impl {
 
   void finish_slot(Slot &s, bool drop_doc) {
     (void)gpj_free_async(s.d_results, s.stream);
-    used -= s.results_bytes;
+    used -= s.reserved;
     s.d_results = nullptr;
-    s.results_bytes = 0;
+    s.reserved = 0;
     Doc *d = s.doc;
     --d->users;
     if (drop_doc) {
@@ -524,38 +668,50 @@ struct Server::Impl {
   RequestResult run_alone(const Request &r) {
     RequestResult res;
     res.id = r.id;
+    if (r.queries.empty() || r.queries.size() > GPJ_MAX_BATCH_QUERIES || r.doc_bytes == 0)
+      throw std::invalid_argument("request without 1..8 queries or document bytes");
+    const bool el = elements_of(r);
     void *stream = nullptr, *scratch = nullptr;
     uint64_t scratch_bytes = 0;
     ok(gpj_stream_create(&stream), "stream");
-    uint64_t cap = newline_hint(r.doc_bytes);
-    int64_t counted = 0;
+    // nothing shared: sizes start from the hints and are learned here
+    std::unordered_map<uint64_t, uint64_t> caps_keep, bounds_keep;
+    std::map<std::pair<uint64_t, int>, uint64_t> spans_keep;
+    std::swap(caps_keep, exact_caps);
+    std::swap(bounds_keep, exact_bounds);
+    std::swap(spans_keep, exact_spans);
+    int64_t failed_lines = 0;
+    std::vector<int64_t> h_small(2 + GPJ_MAX_BATCH_QUERIES, 0);
     for (;;) {
-      Request rr = r;
-      exact_caps[r.doc_id] = cap;
-      Doc *d = build_doc(rr, stream, &scratch, &scratch_bytes);
-      exact_caps.erase(r.doc_id);
-      const uint64_t res_bytes = results_footprint(r, d->nl_cap);
+      Doc *d = build_doc(r, stream, &scratch, &scratch_bytes);
+      if (el) build_bounds(d, bounds_cap_of(r), stream);
+      const Layout lay = layout(r, el ? (int64_t)d->bounds_cap - 1 : (int64_t)d->nl_cap);
       uint8_t *d_results = nullptr;
-      ok(gpj_alloc_async(reinterpret_cast<void **>(&d_results), res_bytes, stream), "alloc");
-      enqueue_queries(r, d, d_results, stream);
-      int64_t lines = 0;
-      ok(gpj_d2h(&lines, d->d_lines, 8, stream), "line count");
-      counted = lines;
-      if (lines >= 0 && (uint64_t)lines <= d->nl_cap) {
-        std::vector<uint8_t> h(host_results_bytes(r, lines));
-        enqueue_copy_back(r, d, d_results, lines, h.data(), stream);
+      ok(gpj_alloc_async(reinterpret_cast<void **>(&d_results), lay.bytes, stream), "alloc");
+      std::fill(h_small.begin(), h_small.end(), 0);
+      enqueue_queries(r, d, lay, d_results, h_small.data(), stream);
+      ok(gpj_stream_sync(stream), "query");
+      bool drop_doc = false;
+      int64_t records = -1;
+      if (h_small[0] < 0) failed_lines = h_small[0];
+      else records = settle(r, d, lay, h_small.data(), &drop_doc);
+      if (records >= 0) {
+        std::vector<uint8_t> h(host_results_bytes(r, records, h_small.data()));
+        enqueue_copy_back(r, lay, d_results, records, h_small.data(), h.data(), stream);
         ok(gpj_stream_sync(stream), "copy back");
-        unpack(r, lines, h.data(), res);
+        unpack(r, records, h_small.data() + 2, h.data(), res);
       }
       (void)gpj_free_async(d_results, stream);
       release_doc(d, stream);
       ok(gpj_stream_sync(stream), "release");
-      if (lines < 0 || (uint64_t)lines <= cap) break;
-      cap = (uint64_t)lines;
+      if (records >= 0 || failed_lines < 0) break;
     }
+    std::swap(caps_keep, exact_caps);
+    std::swap(bounds_keep, exact_bounds);
+    std::swap(spans_keep, exact_spans);
     (void)gpj_free(scratch);
     (void)gpj_stream_destroy(stream);
-    check_lines(counted);
+    check_lines(failed_lines);
     return res;
   }
 
