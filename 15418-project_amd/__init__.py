@@ -188,6 +188,8 @@ def lib():
         L.gpj_batch_scratch_bytes.restype = u64
         L.gpj_batch_tile_map.argtypes = [ctypes.POINTER(BatchFileT), i32, u64, vp]
         L.gpj_batch_index.argtypes = [vp, u64, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp]
+        L.gpj_batch_line_counts.argtypes = [vp, u64, vp, i32, vp, vp, vp, vp]
+        L.gpj_gather_words.argtypes = [vp, ctypes.POINTER(vp), i32, vp]
         L.gpj_batch_query.argtypes = [vp, u64, vp, i32, i32, vp, vp, vp, vp, i64,
                                       ctypes.POINTER(QueryDescT), i32,
                                       ctypes.POINTER(BatchResultsT), vp]
@@ -212,6 +214,7 @@ def lib():
                      "gpj_debug_copy", "gpj_row_bounds", "gpj_query_run_records",
                      "gpj_query_run_dense", "gpj_query_run_records_dense",
                      "gpj_batch_tile_map", "gpj_batch_index", "gpj_batch_query",
+                     "gpj_batch_line_counts", "gpj_gather_words",
                      "gpj_batch_query_shaped",
                      "gpj_batch_per_file", "gpj_graph_begin", "gpj_graph_end",
                      "gpj_graph_launch", "gpj_graph_destroy", "gpj_stream_create",
@@ -744,19 +747,10 @@ class FileBatch:
         self.compiled = compiled
         self.capacity = capacity
         self.levels = max(1, max(c[1] for c in compiled))
-        # exact newline slices (count + 1 entries per file)
-        nl = np.array([np.count_nonzero(buf[int(o):int(o) + int(z)] == 10)
-                       for o, z in zip(self.offsets, self.sizes)], dtype=np.int64)
-        self.line_counts = nl + 1
-        self.line_prefix = np.zeros(f + 1, dtype=np.int64)
-        np.cumsum(self.line_counts, out=self.line_prefix[1:])
-        self.total_lines = int(self.line_prefix[-1])
         files = (BatchFileT * f)()
         for i in range(f):
-            files[i] = BatchFileT(int(self.offsets[i]), int(self.sizes[i]),
-                                  int(self.line_prefix[i]), int(self.line_counts[i]),
+            files[i] = BatchFileT(int(self.offsets[i]), int(self.sizes[i]), 0, 1,
                                   int(self.masks[i]), 0)
-        self.files = files
         self.words = (self.n + 63) // 64
         tiles = (self.n + BATCH_ALIGN - 1) // BATCH_ALIGN
         tile_file = np.zeros(max(tiles, 1), dtype=np.uint32)
@@ -765,11 +759,26 @@ class FileBatch:
         self.d_files = DeviceBuffer.from_host(
             np.frombuffer(bytes(files), dtype=np.uint8), stream)
         self.d_tile_file = DeviceBuffer.from_host(tile_file, stream)
+        self.d_prefix = DeviceBuffer((f + 1) * 8)
+        self.d_scratch = DeviceBuffer(L.gpj_batch_scratch_bytes(self.n))
+        # each file's lines, counted on the device; its newline slice holds
+        # exactly that many entries
+        check(L.gpj_batch_line_counts(self.d_in.ptr, self.n, self.d_files.ptr, f,
+                                      self.d_tile_file.ptr, self.d_prefix.ptr,
+                                      self.d_scratch.ptr, stream), "batch line counts")
+        check(L.gpj_stream_sync(stream))
+        self.line_prefix = self.d_prefix.to_host(np.int64, f + 1, stream)
+        self.line_counts = np.diff(self.line_prefix)
+        self.total_lines = int(self.line_prefix[-1])
+        for i in range(f):
+            files[i].newline_offset = int(self.line_prefix[i])
+            files[i].newline_capacity = int(self.line_counts[i])
+        self.files = files
+        self.d_files = DeviceBuffer.from_host(
+            np.frombuffer(bytes(files), dtype=np.uint8), stream)
         self.d_string = DeviceBuffer(self.words * 8)
         self.d_leveled = DeviceBuffer(self.words * 8 * self.levels)
         self.d_newline = DeviceBuffer(self.total_lines * 8)
-        self.d_prefix = DeviceBuffer((f + 1) * 8)
-        self.d_scratch = DeviceBuffer(L.gpj_batch_scratch_bytes(self.n))
         self.descs = (QueryDescT * len(compiled))()
         self._irs = []
         for i, (ir, depth, slots) in enumerate(compiled):
@@ -783,6 +792,17 @@ class FileBatch:
         self.grid_records = self.total_lines
         self.per_wave = 64  # records per one-wave workgroup of the query launch
         self.graph = None
+
+    def set_levels(self, levels: int):
+        """Leveled rows the batch index builds (at least the queries'
+        deepest level; more rows give the same results)."""
+        need = max(1, max(c[1] for c in self.compiled))
+        if levels < need:
+            raise ValueError(f"the queries need {need} leveled rows")
+        if levels != self.levels:
+            self.release_graph()
+            self.levels = levels
+            self.d_leveled = DeviceBuffer(self.words * 8 * levels)
 
     def _alloc_results(self):
         sp = [DeviceBuffer(max(self.total_lines * self.capacity * 16, 16)) for _ in range(2)]
@@ -880,14 +900,15 @@ class FileBatch:
 
 
 def autotune_batch(batch: FileBatch, time_fn, cache_path: str | None = None):
-    """Pick the query launch of a file batch by measurement: its size (one
-    grid pass over all records, or 4..32 waves per CU on 256 CUs looping over
-    the records) and its workgroup shape (64, 32 or 16 records per one-wave
-    workgroup).  The index launch has no free parameter (one workgroup per
-    16 KiB tile, levels = the queries' deepest level, the fewest that answer
-    them).  `time_fn(batch)` returns the time of one batched run in ms.  The
-    choice is cached by (record-count bucket, levels, query count) in
-    cache_path (JSON) when given."""
+    """Pick the launches of a file batch by measurement: the query launch's
+    size (one grid pass over all records, or 4..32 waves per CU on 256 CUs
+    looping over the records) and workgroup shape (64, 32 or 16 records per
+    one-wave workgroup), then the number of leveled rows (the queries'
+    deepest level or one more).  The index tile (16 KiB, one 256-lane
+    workgroup) is fixed by the batch layout: files start on tiles.
+    `time_fn(batch)` returns the time of one batched run in ms.  The choice
+    is cached by (record-count bucket, levels, query count) in cache_path
+    (JSON) when given."""
     import json as _json
     key = (f"lines~2^{max(batch.total_lines, 1).bit_length()}/levels{batch.levels}"
            f"/q{len(batch.compiled)}/shape")
@@ -898,6 +919,7 @@ def autotune_batch(batch: FileBatch, time_fn, cache_path: str | None = None):
     if key in cache:
         batch.grid_records = int(cache[key]["grid_records"])
         batch.per_wave = int(cache[key]["per_wave"])
+        batch.set_levels(int(cache[key].get("levels", batch.levels)))
         return cache[key]
     trials = []
     for per_wave in (64, 32, 16):
@@ -909,9 +931,21 @@ def autotune_batch(batch: FileBatch, time_fn, cache_path: str | None = None):
             trials.append((time_fn(batch), batch.grid_records, per_wave))
     best = min(trials)
     batch.grid_records, batch.per_wave = best[1], best[2]
+    # leveled rows: the queries' deepest level, or one more
+    base = batch.levels
+    level_trials = [(best[0], base)]
+    batch.set_levels(base + 1)
+    level_trials.append((time_fn(batch), base + 1))
+    best_levels = min(level_trials)[1]
+    batch.set_levels(best_levels)
     choice = {"grid_records": best[1], "per_wave": best[2], "ms": round(best[0], 4),
               "trials": [{"grid_records": g, "per_wave": p, "ms": round(t, 4)}
-                         for t, g, p in trials]}
+                         for t, g, p in trials],
+              "levels": best_levels,
+              "level_trials": [{"levels": lv, "ms": round(t, 4)} for t, lv in level_trials],
+              "index_tile_bytes": BATCH_ALIGN,
+              "index_tile_note": "fixed: files start on index tiles (GPJ_BATCH_ALIGN), one "
+                                 "workgroup of 256 lanes per tile"}
     if cache_path:
         cache[key] = choice
         with open(cache_path, "w") as fh:

@@ -87,6 +87,16 @@ __global__ void fill_i64_kernel(int64_t *p, int64_t count, int64_t v) {
     p[i] = v;
 }
 
+// Up to 16 single words from anywhere to one place (so a host needs one copy
+// for a handful of device counts).
+struct WordSources {
+  const uint64_t *src[16];
+};
+__global__ void gather_words_kernel(WordSources s, int32_t count, uint64_t *dst) {
+  const int i = (int)threadIdx.x;
+  if (i < count) dst[i] = s.src[i] ? *s.src[i] : 0;
+}
+
 // Plain 16-byte-per-lane grid-stride copy; the calibration kernel for HBM
 // traffic counters.
 __global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src,
@@ -184,6 +194,19 @@ gpj_status gpj_d2h_async(void *h_dst, const void *d_src, uint64_t bytes, void *s
   if (!h_dst || !d_src) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_d2h_async: null pointer");
   GPJ_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, as_stream(stream)),
           "hipMemcpyAsync D2H");
+  return GPJ_OK;
+}
+
+gpj_status gpj_gather_words(uint64_t *d_dst, const uint64_t *const *d_srcs, int32_t count,
+                            void *stream) {
+  if (count < 0 || count > 16 || (count > 0 && (!d_dst || !d_srcs)))
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_gather_words: 0..16 words, non-null pointers");
+  if (count == 0) return GPJ_OK;
+  WordSources w{};
+  for (int32_t i = 0; i < count; ++i) w.src[i] = d_srcs[i];
+  hipLaunchKernelGGL(gather_words_kernel, dim3(1), dim3(64), 0, as_stream(stream), w, count,
+                     d_dst);
+  GPJ_HIP(hipGetLastError(), "gather_words_kernel launch");
   return GPJ_OK;
 }
 
@@ -815,6 +838,34 @@ gpj_status gpj_batch_tile_map(const gpj_batch_file *files, int32_t num_files,
     while (f + 1 < num_files && files[f + 1].offset / gpj::kTileBytes <= t) ++f;
     tile_file[t] = (uint32_t)f;
   }
+  return GPJ_OK;
+}
+
+gpj_status gpj_batch_line_counts(const uint8_t *d_in, uint64_t total_bytes,
+                                 const gpj_batch_file *d_files, int32_t num_files,
+                                 const uint32_t *d_tile_file, int64_t *d_line_prefix,
+                                 void *d_scratch, void *stream) {
+  if (num_files <= 0 || total_bytes == 0 || !d_in || !d_files || !d_tile_file ||
+      !d_line_prefix || !d_scratch)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_batch_line_counts: empty batch or null pointer");
+  if (reinterpret_cast<uintptr_t>(d_in) % 64 != 0)
+    return fail(GPJ_ERR_INVALID_ARGUMENT,
+                "gpj_batch_line_counts: batch buffer must be 64-byte aligned");
+  const uint64_t tiles = tiles_for(total_bytes);
+  if (tiles > 0xFFFFFFFFull)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "batch too large for one count launch");
+  hipStream_t st = as_stream(stream);
+  uint64_t *counts = static_cast<uint64_t *>(d_scratch);
+  const auto *files = reinterpret_cast<const gpj::BatchFile *>(d_files);
+  hipLaunchKernelGGL(gpj::batch_count_kernel, dim3((uint32_t)tiles), dim3(gpj::kTileWords), 0, st,
+                     d_in, files, d_tile_file, counts);
+  hipLaunchKernelGGL(gpj::batch_file_lines_kernel, dim3((uint32_t)div_up((uint64_t)num_files, 256)),
+                     dim3(256), 0, st, files, num_files, counts, d_line_prefix);
+  // line counts -> exclusive prefix, total in entry num_files
+  hipLaunchKernelGGL(gpj::newline_scan_kernel, dim3(1), dim3(1024), 0, st,
+                     reinterpret_cast<uint64_t *>(d_line_prefix), (uint64_t)num_files,
+                     d_line_prefix + num_files, (int64_t)0);
+  GPJ_HIP(hipGetLastError(), "batch line count launch");
   return GPJ_OK;
 }
 

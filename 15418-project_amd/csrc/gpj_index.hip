@@ -940,6 +940,48 @@ __global__ __launch_bounds__(kTileWords) void newline_write_kernel(
   }
 }
 
+// ---- line counts of a multi-file batch -------------------------------------
+
+// Per tile: the newlines among its bytes that lie inside its file (tiles past
+// a file's end belong to that file and count nothing beyond it).
+__global__ __launch_bounds__(kTileWords) void batch_count_kernel(
+    const uint8_t *__restrict__ in, const BatchFile *__restrict__ files,
+    const uint32_t *__restrict__ tile_file, uint64_t *__restrict__ tile_counts) {
+  __shared__ uint64_t s_sum[kTileWords / 64];
+  const BatchFile bf = files[tile_file[blockIdx.x]];
+  const uint64_t pos = ((uint64_t)blockIdx.x * kTileWords + threadIdx.x) * 64;
+  const uint64_t end = bf.offset + bf.bytes;
+  uint64_t c = 0;
+  if (pos >= bf.offset && pos < end) {
+    uint32_t x[16];
+    load_word(in, end, pos, true, x); // bytes from `end` on read as 0
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      c += __popc(eq_bytes(x[i] & 0x7F7F7F7Fu, '\n') & ~x[i] & 0x80808080u);
+  }
+  c = wave_sum((uint32_t)c); // at most 64 * 64 per wave
+  if ((threadIdx.x & 63u) == 0) s_sum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t sum = 0;
+    for (int k = 0; k < kTileWords / 64; ++k) sum += s_sum[k];
+    tile_counts[blockIdx.x] = sum;
+  }
+}
+
+// Per file: 1 + the newlines of its tiles (its line count).
+__global__ void batch_file_lines_kernel(const BatchFile *__restrict__ files, int32_t num_files,
+                                        const uint64_t *__restrict__ tile_counts,
+                                        int64_t *__restrict__ lines) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= num_files) return;
+  const BatchFile bf = files[f];
+  uint64_t sum = 1;
+  for (uint64_t t = bf.offset / kTileBytes; t <= (bf.offset + bf.bytes - 1) / kTileBytes; ++t)
+    sum += tile_counts[t];
+  lines[f] = (int64_t)sum;
+}
+
 // ---- set-bit positions of one bitmap row ----------------------------------
 // Record bounds of a top-level array document are the set bits of leveled
 // row 0 (its '[', the ',' between elements and its ']').

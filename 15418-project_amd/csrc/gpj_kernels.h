@@ -62,6 +62,12 @@ __global__ void newline_scan_kernel(uint64_t *tile_counts, uint64_t tiles,
 __global__ void newline_write_kernel(const uint8_t *in, uint64_t n, int aligned,
                                      const uint64_t *tile_offsets,
                                      int64_t *newline_out);
+// Line counts of a multi-file batch: newlines per tile inside its file, then
+// 1 + the sum over each file's tiles.
+__global__ void batch_count_kernel(const uint8_t *in, const BatchFile *files,
+                                   const uint32_t *tile_file, uint64_t *tile_counts);
+__global__ void batch_file_lines_kernel(const BatchFile *files, int32_t num_files,
+                                        const uint64_t *tile_counts, int64_t *lines);
 __global__ void escape_quote_kernel(const uint8_t *in, uint64_t n, uint64_t words,
                                     uint64_t *escape_out, uint64_t *quote_out);
 

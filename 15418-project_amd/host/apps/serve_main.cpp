@@ -4,7 +4,10 @@
 // one of three record schemas with equal odds (NDJSON records of the 64 MiB
 // plain configuration, one top-level array of the 1 GiB single-document
 // configuration, NDJSON records of the 8 GiB throughput configuration), 1..8
-// queries each drawn from that configuration's query set, a share of
+// distinct queries each (at most the pool's size) drawn from the query sets
+// of the configurations: the five line queries of the 64 MiB and 8 GiB
+// configurations for NDJSON documents, the element query of the 1 GiB
+// configuration for array documents, a share of
 // requests (default 30%) re-querying an earlier document.  For every arrival
 // rate it prints one JSON line: p50/p99 latency (arrival to results on the
 // host), sustained GB/s, resident hits, evictions, retries, and whether every
@@ -47,8 +50,8 @@ struct Args {
   double requery = 0.3;
   double budget_gib = 80;
   int slots = 8;
-  int spans_per_record = 4;
-  uint64_t bytes_per_line = 256, bytes_per_element = 64;
+  int spans_per_record = 1;
+  uint64_t bytes_per_line = 1024, bytes_per_element = 512;
   int verify = -1; // requests per rate checked against run_alone (-1 = all)
   int threads = 16;
   std::string digest_out;
@@ -106,9 +109,11 @@ Args parse(int argc, char **argv) {
 // indices into the server's table
 const std::vector<std::string> kQueries = {"$.a.b", "$[*].items[*].sku", "$.user.lang",
                                            "$.e[0].f", "$.h[*].t", "$.h[?(@.k == \"v\")].id"};
-const std::vector<int> kSchema0 = {0};
-const std::vector<int> kSchema1 = {1};
-const std::vector<int> kSchema2 = {2, 3, 4, 5};
+// NDJSON documents (either schema) draw from the line queries of both
+// NDJSON configurations (a query of the other configuration finds nothing
+// there); array documents run the single-document configuration's query.
+const std::vector<int> kLineQueries = {0, 2, 3, 4, 5};
+const std::vector<int> kElementQueries = {1};
 const uint64_t kAlign = 64;
 
 // Digest of a result array: sum of value * odd weight (SplitMix64 of the
@@ -179,10 +184,14 @@ int main(int argc, char **argv) {
                           U(rng) * (std::log((double)a.max_doc) - std::log((double)a.min_doc));
         targets.push_back((uint64_t)std::exp(lg));
       }
-      const std::vector<int> &pool =
-          schemas[s.doc] == 0 ? kSchema0 : (schemas[s.doc] == 1 ? kSchema1 : kSchema2);
-      const int k = 1 + (int)(U(rng) * 8) % 8;
-      for (int j = 0; j < k; ++j) s.queries.push_back(pool[(size_t)(U(rng) * pool.size()) % pool.size()]);
+      // 1..8 distinct queries (at most the pool's size), in random order
+      std::vector<int> pool = schemas[s.doc] == 1 ? kElementQueries : kLineQueries;
+      const int k = std::min<int>(1 + (int)(U(rng) * 8) % 8, (int)pool.size());
+      for (int j = 0; j < k; ++j) {
+        const size_t pick = j + (size_t)(U(rng) * (double)(pool.size() - j)) % (pool.size() - j);
+        std::swap(pool[j], pool[pick]);
+        s.queries.push_back(pool[j]);
+      }
       shapes.push_back(std::move(s));
     }
     std::vector<uint64_t> off(schemas.size()), sz(schemas.size());

@@ -71,16 +71,18 @@ struct RequestResult {
 struct ServerOptions {
   uint64_t hbm_budget_bytes = 80ull << 30;
   int slots = 8;            // requests in flight at once
-  int spans_per_record = 4; // first guess of an extension query's spans per
-                            // record (the output is sized from it; a larger
-                            // total reruns the request with the exact size)
+  int spans_per_record = 1; // first guess of an extension query's spans per
+                            // record (the output is sized from it and from
+                            // the most seen since; a larger total reruns the
+                            // request with the exact size)
   bool keep_resident = true;
   int index_levels = 0;     // 0: the deepest the table's queries need
-  // First guesses of a new document's line and array-element sizes (the
-  // newline index and the element bounds are sized from them; a document
-  // with more lines or elements is rebuilt with the exact count).
-  uint64_t bytes_per_line = 256;
-  uint64_t bytes_per_element = 64;
+  // First guesses of a document's line and array-element sizes (the
+  // newline index and the element bounds are sized from them, and later from
+  // the densest document seen; a document with more lines or elements is
+  // rebuilt with the exact count).
+  uint64_t bytes_per_line = 1024;
+  uint64_t bytes_per_element = 512;
 };
 
 struct ServerStats {

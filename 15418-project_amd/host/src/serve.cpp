@@ -30,17 +30,12 @@ void ok(gpj_status s, const char *what) {
 uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 uint64_t round256(uint64_t a) { return div_up(a, 256) * 256; }
 
-// Newline-index capacity assumed before the count is known: lines of
-// `per_line` bytes and more on average.  A document with more lines is
-// rebuilt with its exact count (counted as a retry).
-uint64_t newline_hint(uint64_t n, uint64_t per_line) {
-  return n / std::max<uint64_t>(per_line, 1) + 2;
-}
-
-// Record-bound capacity assumed for a top-level array before the count is
-// known: elements of `per_element` bytes and more on average, plus '[' and ']'.
-uint64_t bounds_hint(uint64_t n, uint64_t per_element) {
-  return n / std::max<uint64_t>(per_element, 1) + 3;
+// Capacities assumed before a count is known: `density` items per byte
+// (the densest document served so far, or the configured first guess) with a
+// quarter of slack.  A document with more lines or elements is rebuilt with
+// its exact count (counted as a retry).
+uint64_t density_hint(uint64_t n, double density, uint64_t extra) {
+  return (uint64_t)((double)n * density * 1.25) + extra;
 }
 
 // True when the bytecode holds an extension opcode (MOVE_TO_EACH or
@@ -126,6 +121,7 @@ struct Server::Impl {
   struct Layout {
     int64_t max_records = 0;
     std::vector<uint64_t> caps;
+    uint64_t outputs = 0; // bytes of the offsets and spans (they come first)
     uint64_t bytes = 0;
   };
   struct Slot {
@@ -135,7 +131,8 @@ struct Server::Impl {
     int stage = 0; // 0 idle, 1 computing, 2 copying results back
     size_t req = 0;
     Doc *doc = nullptr;
-    uint8_t *d_results = nullptr;
+    uint8_t *d_results = nullptr; // kept across requests, grown when needed
+    uint64_t results_cap = 0;
     Layout lay;
     uint64_t reserved = 0; // budget reserved for d_results and new bounds
     int64_t *h_small = nullptr; // pinned: line count, bound count, totals[8]
@@ -156,6 +153,10 @@ struct Server::Impl {
   std::map<std::pair<uint64_t, int>, uint64_t> exact_spans; // (doc, query) -> spans
   std::vector<Slot> slots;
   uint64_t used = 0, tick = 0;
+  // learned sizes: lines and array elements per document byte, spans per
+  // record of each query (the largest seen; they only size first guesses)
+  double line_density = 0, element_density = 0;
+  std::vector<double> spans_ratio;
 
   // Pinned result buffers: a completed request's buffer goes to a worker
   // that unpacks it into the caller's vectors and then back to the pool, so
@@ -171,16 +172,21 @@ struct Server::Impl {
 
   Pinned take_pinned(uint64_t bytes) {
     {
+      // the smallest pooled buffer that holds `bytes`
       std::lock_guard<std::mutex> g(pool_mu);
+      size_t best = pool.size();
       for (size_t i = 0; i < pool.size(); ++i)
-        if (pool[i].bytes >= bytes) {
-          Pinned b = pool[i];
-          pool.erase(pool.begin() + (long)i);
-          return b;
-        }
+        if (pool[i].bytes >= bytes && (best == pool.size() || pool[i].bytes < pool[best].bytes))
+          best = i;
+      if (best < pool.size()) {
+        Pinned b = pool[best];
+        pool.erase(pool.begin() + (long)best);
+        return b;
+      }
     }
     Pinned b;
-    b.bytes = std::max<uint64_t>(bytes, 1 << 20);
+    b.bytes = 1 << 20; // powers of two: few distinct buffers in the pool
+    while (b.bytes < bytes) b.bytes <<= 1;
     ok(gpj_host_alloc(reinterpret_cast<void **>(&b.p), b.bytes), "pinned alloc");
     return b;
   }
@@ -222,6 +228,9 @@ struct Server::Impl {
   Impl(std::vector<QuerySpec> q, ServerOptions o) : queries(std::move(q)), opt(o) {
     if (opt.slots < 1) opt.slots = 1;
     if (opt.spans_per_record < 1) opt.spans_per_record = 1;
+    line_density = 1.0 / (double)std::max<uint64_t>(opt.bytes_per_line, 1);
+    element_density = 1.0 / (double)std::max<uint64_t>(opt.bytes_per_element, 1);
+    spans_ratio.assign(queries.size(), (double)opt.spans_per_record);
     int deepest = 1;
     for (const auto &qs : queries) deepest = std::max(deepest, qs.max_depth + (qs.elements ? 1 : 0));
     levels = opt.index_levels > 0 ? opt.index_levels : deepest;
@@ -240,6 +249,11 @@ struct Server::Impl {
       ok(gpj_host_alloc(reinterpret_cast<void **>(&s.h_small), 8 * (2 + GPJ_MAX_BATCH_QUERIES)),
          "pinned alloc");
     }
+    // page-locked result buffers for the slots' first requests (allocating
+    // them later would stall the scheduler thread)
+    std::vector<Pinned> warm;
+    for (int i = 0; i < 2 * opt.slots; ++i) warm.push_back(take_pinned(4u << 20));
+    for (auto &b : warm) give_pinned(b);
     for (int i = 0; i < 4; ++i) workers.emplace_back([this] { worker_loop(); });
   }
 
@@ -256,6 +270,7 @@ struct Server::Impl {
     docs.clear();
     for (auto &s : slots) {
       (void)gpj_free(s.scratch);
+      (void)gpj_free(s.d_results);
       (void)gpj_host_free(s.h_small);
       (void)gpj_event_destroy(s.ev_a);
       (void)gpj_event_destroy(s.ev_b);
@@ -275,12 +290,12 @@ struct Server::Impl {
   uint64_t cap_of(const Request &r) const {
     auto it = exact_caps.find(r.doc_id);
     return it != exact_caps.end() ? (uint64_t)it->second
-                                  : newline_hint(r.doc_bytes, opt.bytes_per_line);
+                                  : density_hint(r.doc_bytes, line_density, 2);
   }
   uint64_t bounds_cap_of(const Request &r) const {
     auto it = exact_bounds.find(r.doc_id);
     return it != exact_bounds.end() ? std::max<uint64_t>(it->second, 1)
-                                    : bounds_hint(r.doc_bytes, opt.bytes_per_element);
+                                    : density_hint(r.doc_bytes, element_density, 3);
   }
   uint64_t doc_footprint(uint64_t bytes, uint64_t nl_cap) const {
     const uint64_t w = div_up(bytes, 64);
@@ -302,14 +317,16 @@ struct Server::Impl {
       auto it = exact_spans.find({r.doc_id, qi});
       if (it != exact_spans.end()) cap = it->second;
       else if (!q.variable) cap = (uint64_t)max_records * (uint64_t)q.slots; // at most slots each
-      else cap = (uint64_t)max_records * (uint64_t)opt.spans_per_record + 16;
+      else cap = (uint64_t)((double)max_records * spans_ratio[qi] * 1.25) + 16;
       cap = std::max<uint64_t>(cap, 1);
       l.caps.push_back(cap);
       l.bytes += round256(((uint64_t)max_records + 1) * 8) + round256(cap * 16);
       total += cap;
     }
+    l.outputs = l.bytes;
     l.bytes += round256(gpj_query_dense_scratch_bytes(max_records, (int32_t)r.queries.size(),
                                                       (int64_t)total));
+    l.bytes += 256; // the gathered counts (last 256 bytes)
     return l;
   }
 
@@ -366,11 +383,13 @@ struct Server::Impl {
     d->d_lines = reinterpret_cast<int64_t *>(p);
     used += d->footprint;
     ok(gpj_h2d(d->d_in, r.doc, r.doc_bytes, stream), "doc upload");
-    const uint64_t sb = gpj_index_scratch_bytes(r.doc_bytes);
+    // index scratch of the slot: sized for 1 GiB documents at first (64 KiB
+    // per 4 GiB), grown stream-ordered for larger ones
+    const uint64_t sb = std::max<uint64_t>(gpj_index_scratch_bytes(r.doc_bytes),
+                                           gpj_index_scratch_bytes(1ull << 30));
     if (sb > *scratch_bytes) {
-      ok(gpj_stream_sync(stream), "scratch");
-      ok(gpj_free(*scratch), "scratch free");
-      ok(gpj_alloc(scratch, sb), "scratch alloc");
+      if (*scratch) ok(gpj_free_async(*scratch, stream), "scratch free");
+      ok(gpj_alloc_async(scratch, sb, stream), "scratch alloc");
       *scratch_bytes = sb;
     }
     ok(gpj_index_build_into(d->d_in, d->bytes, levels, d->d_string, d->d_leveled, d->d_newline,
@@ -439,11 +458,15 @@ struct Server::Impl {
       ok(gpj_query_run_dense(d->d_in, d->bytes, &idx, descs.data(), (int32_t)descs.size(),
                              lay.max_records, outs.data(), p, stream),
          "query launch");
-    ok(gpj_d2h_async(h_small, d->d_lines, 8, stream), "line count copy");
-    if (d->d_nbounds) ok(gpj_d2h_async(h_small + 1, d->d_nbounds, 8, stream), "bound count copy");
+    // line count, bound count and each query's total in one copy
+    const uint64_t *srcs[2 + GPJ_MAX_BATCH_QUERIES] = {};
+    srcs[0] = reinterpret_cast<const uint64_t *>(d->d_lines);
+    srcs[1] = reinterpret_cast<const uint64_t *>(d->d_nbounds);
     for (size_t k = 0; k < outs.size(); ++k)
-      ok(gpj_d2h_async(h_small + 2 + k, outs[k].d_offsets + lay.max_records, 8, stream),
-         "total copy");
+      srcs[2 + k] = reinterpret_cast<const uint64_t *>(outs[k].d_offsets + lay.max_records);
+    uint64_t *d_small = reinterpret_cast<uint64_t *>(d_results + lay.bytes - 256);
+    ok(gpj_gather_words(d_small, srcs, (int32_t)(2 + outs.size()), stream), "count gather");
+    ok(gpj_d2h_async(h_small, d_small, 8 * (2 + outs.size()), stream), "count copy");
   }
 
   // Reads the small copies of a finished query launch: the record count, or
@@ -454,6 +477,10 @@ struct Server::Impl {
     const int64_t lines = h_small[0];
     check_lines(lines);
     *drop_doc = false;
+    // the next documents' first guesses follow the densest seen so far
+    line_density = std::max(line_density, (double)lines / (double)r.doc_bytes);
+    if (elements_of(r))
+      element_density = std::max(element_density, (double)h_small[1] / (double)r.doc_bytes);
     if ((uint64_t)lines > d->nl_cap) { // rebuild the index with the exact count
       exact_caps[r.doc_id] = (uint64_t)lines;
       *drop_doc = true;
@@ -475,33 +502,21 @@ struct Server::Impl {
         exact_spans[{r.doc_id, r.queries[k]}] = (uint64_t)h_small[2 + k];
         fits = false;
       }
+    for (size_t k = 0; k < r.queries.size(); ++k)
+      if (records > 0 && queries[r.queries[k]].variable)
+        spans_ratio[r.queries[k]] = std::max(spans_ratio[r.queries[k]],
+                                             (double)h_small[2 + k] / (double)records);
     return fits ? records : -1;
   }
 
-  uint64_t host_results_bytes(const Request &r, int64_t records, const int64_t *h_small) const {
-    uint64_t b = 0;
-    for (size_t k = 0; k < r.queries.size(); ++k)
-      b += ((uint64_t)records + 1) * 8 + (uint64_t)h_small[2 + k] * 16;
-    return b;
+  // One copy brings every query's outputs (offsets and spans, as laid out
+  // on the device) to the host.
+  void enqueue_copy_back(const Layout &lay, const uint8_t *d_results, uint8_t *h, void *stream) {
+    ok(gpj_d2h_async(h, d_results, lay.outputs, stream), "results copy");
   }
 
-  // Enqueues the copy-back of each query's offsets [records + 1] and spans.
-  void enqueue_copy_back(const Request &r, const Layout &lay, const uint8_t *d_results,
-                         int64_t records, const int64_t *h_small, uint8_t *h, void *stream) {
-    const uint8_t *p = d_results;
-    for (size_t k = 0; k < r.queries.size(); ++k) {
-      const uint64_t ob = ((uint64_t)records + 1) * 8, sb = (uint64_t)h_small[2 + k] * 16;
-      ok(gpj_d2h_async(h, p, ob, stream), "offsets copy");
-      h += ob;
-      p += round256(((uint64_t)lay.max_records + 1) * 8);
-      ok(gpj_d2h_async(h, p, sb, stream), "spans copy");
-      h += sb;
-      p += round256(lay.caps[k] * 16);
-    }
-  }
-
-  void unpack(const Request &r, int64_t records, const int64_t *totals, const uint8_t *h,
-              RequestResult &out) const {
+  void unpack(const Request &r, const Layout &lay, int64_t records, const int64_t *totals,
+              const uint8_t *h, RequestResult &out) const {
     out.num_records = records;
     out.outputs.clear();
     for (size_t k = 0; k < r.queries.size(); ++k) {
@@ -509,11 +524,22 @@ struct Server::Impl {
       o.offsets.resize((size_t)records + 1);
       o.spans.resize((size_t)totals[k] * 2);
       std::memcpy(o.offsets.data(), h, o.offsets.size() * 8);
-      h += o.offsets.size() * 8;
+      h += round256(((uint64_t)lay.max_records + 1) * 8);
       std::memcpy(o.spans.data(), h, o.spans.size() * 8);
-      h += o.spans.size() * 8;
+      h += round256(lay.caps[k] * 16);
       out.outputs.push_back(std::move(o));
     }
+  }
+
+  // The slot's device results buffer, grown to `bytes` when smaller (the
+  // slot is idle: its stream has no work pending on the old buffer).
+  void slot_results(Slot &s, uint64_t bytes) {
+    if (s.results_cap >= bytes) return;
+    if (s.d_results) (void)gpj_free_async(s.d_results, s.stream);
+    const uint64_t grown = std::max<uint64_t>(bytes, s.results_cap + s.results_cap / 2);
+    ok(gpj_alloc_async(reinterpret_cast<void **>(&s.d_results), grown, s.stream),
+       "results alloc");
+    s.results_cap = grown;
   }
 
   std::vector<RequestResult> run(const std::vector<Request> &reqs) {
@@ -584,8 +610,7 @@ struct Server::Impl {
         if (new_bounds) build_bounds(d, b_cap, s->stream);
         s->doc = d;
         d->last_use = ++tick;
-        ok(gpj_alloc_async(reinterpret_cast<void **>(&s->d_results), lay.bytes, s->stream),
-           "results alloc");
+        slot_results(*s, lay.bytes);
         s->lay = std::move(lay);
         s->reserved = s->lay.bytes;
         used += s->reserved;
@@ -615,8 +640,8 @@ struct Server::Impl {
             continue;
           }
           s.records = records;
-          s.h_results = take_pinned(host_results_bytes(r, records, s.h_small));
-          enqueue_copy_back(r, s.lay, s.d_results, records, s.h_small, s.h_results.p, s.stream);
+          s.h_results = take_pinned(s.lay.outputs);
+          enqueue_copy_back(s.lay, s.d_results, s.h_results.p, s.stream);
           ok(gpj_event_record(s.ev_b, s.stream), "event record");
           s.stage = 2;
         } else if (s.stage == 2) {
@@ -632,8 +657,8 @@ struct Server::Impl {
           const Pinned buf = s.h_results;
           std::vector<int64_t> totals(s.h_small + 2, s.h_small + 2 + r.queries.size());
           s.h_results = Pinned{};
-          post([this, &r, &res, records, totals, buf] {
-            unpack(r, records, totals.data(), buf.p, res);
+          post([this, &r, &res, records, totals, buf, lay = s.lay] {
+            unpack(r, lay, records, totals.data(), buf.p, res);
             give_pinned(buf);
           });
           finish_slot(s, /*drop_doc=*/false);
@@ -648,9 +673,7 @@ struct Server::Impl {
   }
 
   void finish_slot(Slot &s, bool drop_doc) {
-    (void)gpj_free_async(s.d_results, s.stream);
-    used -= s.reserved;
-    s.d_results = nullptr;
+    used -= s.reserved; // the buffer stays with the slot for its next request
     s.reserved = 0;
     Doc *d = s.doc;
     --d->users;
@@ -696,10 +719,10 @@ struct Server::Impl {
       if (h_small[0] < 0) failed_lines = h_small[0];
       else records = settle(r, d, lay, h_small.data(), &drop_doc);
       if (records >= 0) {
-        std::vector<uint8_t> h(host_results_bytes(r, records, h_small.data()));
-        enqueue_copy_back(r, lay, d_results, records, h_small.data(), h.data(), stream);
+        std::vector<uint8_t> h(lay.outputs);
+        enqueue_copy_back(lay, d_results, h.data(), stream);
         ok(gpj_stream_sync(stream), "copy back");
-        unpack(r, records, h_small.data() + 2, h.data(), res);
+        unpack(r, lay, records, h_small.data() + 2, h.data(), res);
       }
       (void)gpj_free_async(d_results, stream);
       release_doc(d, stream);

@@ -8,19 +8,25 @@ Default workload (the throughput configuration): 8 GiB of seeded synthetic
 NDJSON, ~1 KiB records of depth <= 4, four queries over one shared 3-level
 index: $.user.lang, $.e[0].f, $.h[*].t and $.h[?(@.k == "v")].id.
 
-Prints one JSON line (see the keys at the end of main()).  Timing uses HIP
-events recorded on the stream the kernels run on; each kernel's launch time
-gives its roofline figure (`roofline` is the kernel that takes longest,
-`roofline_kernels` lists both).  The CPU baseline is the
-multi-threaded CPU oracle (a port of the reference algorithm) on a bounded
-prefix sample of the same data, rank 0 only.
+Prints one JSON line.  Timing uses HIP events recorded on the stream the
+kernels run on; each kernel's launch time gives its roofline figure
+(`roofline` is the kernel that takes longest, `roofline_kernels` lists
+both).  The CPU baseline is the CPU oracle (a port of the reference
+algorithm) on a bounded prefix sample of the same data, at all cores of the
+process's CPU share and at one thread.  The default run also summarizes, in
+the same line, the batched configuration (10,000 files: one graph launch
+against plain per-file launches, `batched`) and the serving configuration
+(p50/p99 latency and sustained GB/s per arrival rate, `serving`); they run
+after the timed steps.  `--workload serving` prints the serving line alone.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes
+import gc
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -91,6 +97,7 @@ def roofline_entry(kernel: str, algo_bytes: float, ms: float, traffic):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
             "kernel_ms": round(ms, 4), "algorithmic_bytes": int(algo_bytes)}
 
+DEFAULT_WORKLOAD = "ndjson_8GiB_4q"
 WORKLOADS = {
     # name: (schema, bytes, seed, queries, extension queries allowed)
     "ndjson_8GiB_4q": (2, 8 << 30, 0x15418002,
@@ -156,7 +163,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ndjson_8GiB_4q", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD,
+                    choices=sorted(WORKLOADS) + ["serving"])
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload only: leave out the batched (10,000 files) and "
+                         "serving summaries")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-verify", type=int, default=100,
+                    help="requests per rate the replay compares with solo runs (-1 = all)")
     ap.add_argument("--cpu-sample-mib", type=int, default=2048)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -190,10 +204,29 @@ def main():
         gpj.check(gpj.lib().gpj_stream_sync(None))
         print(json.dumps({"copy_calibration_bytes": nb, "launches": 3}))
         return
+    if args.workload == "serving":
+        print(json.dumps(serving_line(args)), flush=True)
+        return
     schema, nbytes, seed, queries = WORKLOADS[args.workload]
     if schema == -1:
-        return run_file_batch(args, gpj, hip, nbytes, seed)
+        print(json.dumps(run_file_batch(args, gpj, hip, nbytes, seed)), flush=True)
+        return
+    out = run_single(args, gpj, hip, schema, nbytes, seed, queries)
+    if args.workload == DEFAULT_WORKLOAD and not args.no_extras:
+        # the batched and serving configurations, summarized in the same line
+        gc.collect()
+        b = run_file_batch(args, gpj, hip, WORKLOADS["batch_10k_files"][1],
+                           WORKLOADS["batch_10k_files"][2], summary=True)
+        out["batched"] = {k: b[k] for k in ("config", "ms_per_step", "value", "unit",
+                                            "per_file_launches", "batched_over_per_file",
+                                            "identical_results", "kernels_ms", "autotune")}
+        gc.collect()
+        out["serving"] = run_serving(args)
+    print(json.dumps(out), flush=True)
 
+
+def run_single(args, gpj, hip, schema, nbytes, seed, queries):
+    """One input indexed and queried per step (the single-input workloads)."""
     t0 = time.time()
     data = gpj.generate(schema, nbytes, seed)
     gen_s = time.time() - t0
@@ -505,10 +538,10 @@ def main():
         "end_to_end_hbm_frac": round(
             (algo_index + algo_queries) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def run_file_batch(args, gpj, hip, num_files: int, seed: int):
+def run_file_batch(args, gpj, hip, num_files: int, seed: int, summary: bool = False):
     """Many files: one graph launch (batched index + batched query) per step,
     against plain per-file launches over the same device-resident files."""
     rng = np.random.default_rng(seed)
@@ -564,7 +597,7 @@ def run_file_batch(args, gpj, hip, num_files: int, seed: int):
     batched_ms = timed(b.launch_graph, args.steps)
     b.per_file_setup()
     b.launch_per_file()
-    per_file_ms = timed(b.launch_per_file, max(1, args.steps // 4))
+    per_file_ms = timed(b.launch_per_file, 2 if summary else max(1, args.steps // 4))
     bs, bc = b.fetch()
     ps, pc = b.fetch(per_file=True)
     identical = all(np.array_equal(bs[k], ps[k]) and np.array_equal(bc[k], pc[k])
@@ -585,9 +618,9 @@ def run_file_batch(args, gpj, hip, num_files: int, seed: int):
                               profiled_traffic(args.workload, "gpj::query_files_kernel"))]
     dominant = max(kernels, key=lambda k: k["kernel_ms"])
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not summary:
         orc = ge.load_oracle()
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or baseline_threads()
         budget, done, t0 = args.cpu_sample_mib << 19, 0, time.time()
         nfiles = 0
         for f in range(num_files):
@@ -608,7 +641,7 @@ def run_file_batch(args, gpj, hip, num_files: int, seed: int):
         cpu = {"value": round(done / cpu_s / 1e9, 4), "unit": "GB/s", "cores": threads,
                "kind": "port",
                "sample": f"first {nfiles} files ({done} bytes), index + their 2 queries each; "
-                         f"multi-threaded CPU oracle, {cpu_s:.2f}s wall"}
+                         f"multi-threaded CPU oracle, {cpu_s:.2f}s wall", **cpu_description()}
     out = {
         "metric": "JSON GB/s indexed+queried (fraction of HBM peak); p50/p99 request latency",
         "value": round(total / (batched_ms * 1e-3) / 1e9, 3),
@@ -622,7 +655,7 @@ def run_file_batch(args, gpj, hip, num_files: int, seed: int):
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": args.workload, "files": num_files, "bytes": total,
+        "config": {"workload": "batch_10k_files", "files": num_files, "bytes": total,
                    "buffer_bytes": b.n, "records": lines - num_files, "queries": BATCH_QUERIES,
                    "levels": b.levels, "parallelism": "single-gpu"},
         "roofline": dominant,
@@ -634,8 +667,55 @@ def run_file_batch(args, gpj, hip, num_files: int, seed: int):
         "batched_over_per_file": round(per_file_ms / batched_ms, 2),
         "identical_results": identical,
         "autotune": tune,
+        "line_counts": "on the device (gpj_batch_line_counts), no host pass over the bytes",
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+SERVE_BIN = os.path.join(ROOT, "15418-project_amd", "_build", "gpjson_serve")
+SERVING_RATES = "500,1000,2000,5000"
+
+
+def run_serving(args) -> dict:
+    """The serving configuration through the gpjson_serve replay (a child
+    process): Poisson arrivals at each rate, documents of 4 KiB .. 64 MiB
+    (log-uniform) from the three record schemas, 1..8 queries each from that
+    schema's configuration, 30% re-queries, an 80 GiB HBM budget, 8 slots."""
+    cmd = [SERVE_BIN, "--requests", str(args.serve_requests), "--rates", SERVING_RATES,
+           "--min-doc", "4KiB", "--max-doc", "64MiB", "--requery", "0.3", "--budget-gib", "80",
+           "--slots", "8", "--verify", str(args.serve_verify), "--seed", "0x15418004"]
+    print(f"[bench] serving: {' '.join(cmd[1:])}", file=sys.stderr, flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    if p.returncode != 0:
+        raise SystemExit("bench.py: serving replay failed: " + p.stderr[-1500:])
+    rows = [json.loads(x) for x in p.stdout.splitlines() if x.startswith("{")]
+    keep = ("rate_per_s", "p50_ms", "p99_ms", "mean_ms", "sustained_GBps", "new_document_GBps",
+            "makespan_s", "resident_hits", "evictions", "retries", "peak_device_bytes",
+            "verified", "identical")
+    return {"workload": f"serving: {args.serve_requests} requests per rate, Poisson arrivals, "
+                        "documents 4 KiB-64 MiB log-uniform from the 64 MiB / 1 GiB / 8 GiB "
+                        "record schemas, 1-8 queries each, 30% re-queries, 80 GiB HBM budget, "
+                        "8 slots; latency = arrival to results in host memory",
+            "documents": rows[0]["documents"] if rows else 0,
+            "rates": [{k: r[k] for k in keep} for r in rows]}
+
+
+def serving_line(args) -> dict:
+    """The bench line of the serving workload: value = sustained GB/s at the
+    highest arrival rate, with p50/p99 latency at every rate."""
+    srv = run_serving(args)
+    top = max(srv["rates"], key=lambda r: r["rate_per_s"])
+    return {
+        "metric": "JSON GB/s indexed+queried (fraction of HBM peak); p50/p99 request latency",
+        "value": top["sustained_GBps"], "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(top["makespan_s"] * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "serving", "requests_per_rate": args.serve_requests,
+                   "rates_per_s": [r["rate_per_s"] for r in srv["rates"]],
+                   "parallelism": "single-gpu"},
+        "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "serving": srv,
+        "roofline": None, "cpu_baseline": None,
+    }
 
 
 if __name__ == "__main__":

@@ -63,6 +63,11 @@ gpj_status gpj_host_free(void *h_ptr);
 gpj_status gpj_alloc_async(void **d_ptr, uint64_t bytes, void *stream);
 gpj_status gpj_free_async(void *d_ptr, void *stream);
 gpj_status gpj_d2h_async(void *h_dst, const void *d_src, uint64_t bytes, void *stream);
+/* d_dst[i] = *d_srcs[i] for i < count <= 16 (a null source gives 0): one
+ * launch gathers a handful of device words so that one copy brings them to
+ * the host.  d_srcs is a host array of device pointers.  Stream-ordered. */
+gpj_status gpj_gather_words(uint64_t *d_dst, const uint64_t *const *d_srcs, int32_t count,
+                            void *stream);
 gpj_status gpj_event_create(void **event);
 gpj_status gpj_event_record(void *event, void *stream);
 gpj_status gpj_event_query(void *event);
@@ -331,6 +336,17 @@ uint64_t gpj_batch_scratch_bytes(uint64_t total_bytes);
  * GPJ_BATCH_ALIGN rounded up); tiles no file covers get the preceding file. */
 gpj_status gpj_batch_tile_map(const gpj_batch_file *files, int32_t num_files,
                               uint64_t total_bytes, uint32_t *tile_file);
+/* Line counts of every file, counted on the device (no host pass over the
+ * bytes): d_line_prefix[f] receives the exclusive prefix of the files' line
+ * counts (1 + newlines inside the file) and d_line_prefix[num_files] the
+ * total, from which a caller sizes the newline slices (newline_offset,
+ * newline_capacity) before gpj_batch_index.  Those two fields of d_files are
+ * not read here.  d_scratch: gpj_batch_scratch_bytes(total_bytes) bytes;
+ * 64-byte aligned buffer.  Stream-ordered. */
+gpj_status gpj_batch_line_counts(const uint8_t *d_in, uint64_t total_bytes,
+                                 const gpj_batch_file *d_files, int32_t num_files,
+                                 const uint32_t *d_tile_file, int64_t *d_line_prefix,
+                                 void *d_scratch, void *stream);
 /* One index launch over every file (d_files, d_tile_file: device arrays).
  * A failed build writes -1 into d_line_prefix[num_files] (the total line
  * count) and gpj_index_status(d_scratch, total_bytes, ...) reports it. */

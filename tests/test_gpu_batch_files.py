@@ -45,6 +45,15 @@ def test_line_prefix(batch):
     np.testing.assert_array_equal(batch.device_line_prefix(), batch.line_prefix)
 
 
+def test_device_line_counts_equal_host_count(batch):
+    # gpj_batch_line_counts (the sizes FileBatch uses) against a host count of
+    # each file's newlines
+    want = np.array([np.count_nonzero(batch.buf[int(o):int(o) + int(z)] == 10) + 1
+                     for o, z in zip(batch.offsets, batch.sizes)], dtype=np.int64)
+    np.testing.assert_array_equal(batch.line_counts, want)
+    assert batch.line_prefix[0] == 0 and batch.total_lines == int(want.sum())
+
+
 def test_batched_equals_oracle_per_file(gpu, orc, batch):
     spans, counts = batch.fetch()
     string = batch.d_string.to_host(np.uint64, batch.words)

@@ -197,6 +197,10 @@ def test_batch_10k_files_sample(gpu, orc):
     b.launch()
     b.buf = buf
     np.testing.assert_array_equal(b.device_line_prefix(), b.line_prefix)
+    # the device line counts FileBatch sized its buffers with, against the host
+    host = np.array([np.count_nonzero(buf[int(o):int(o) + int(z)] == 10) + 1
+                     for o, z in zip(off, sz)], dtype=np.int64)
+    np.testing.assert_array_equal(b.line_counts, host)
     spans, counts = b.fetch()
     rng = np.random.default_rng(17)
     files = set(rng.choice(b.num_files, 30, replace=False).tolist())
