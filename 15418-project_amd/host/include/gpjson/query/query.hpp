@@ -81,11 +81,28 @@ private:
 
 class MaterializedBatchResult;
 
+/* Results of one query over the lines of one partition in compact form (what
+ * the executor produces): spans [lines][slots][2], partition-local {start,
+ * end} (-1 = no value), global offsets = base + local, and the texts of the
+ * values in one buffer, in span order.  No string per value is made until
+ * the results are read per line (BatchQueryResult::queries) or materialized,
+ * and then only for values a line holds (extension; the reference builds a
+ * string per match in the executor, query_executor_host.cu:111-114). */
+struct CompactLines {
+  size_t base = 0;
+  size_t lines = 0;
+  int slots = 1;
+  std::vector<int64_t> spans;
+  std::string texts;
+};
+
 class BatchQueryResult {
 public:
   explicit BatchQueryResult(size_t num_queries)
-      : queries_(num_queries), query_texts_(num_queries) {}
+      : queries_(num_queries), pending_(num_queries), query_texts_(num_queries) {}
   void add_line_result(size_t query_index, LineQueryResult line_result);
+  /* Appends a partition's results of one query in compact form. */
+  void add_compact(size_t query_index, CompactLines lines);
   void merge(const BatchQueryResult &other);
   void set_query_text(size_t query_index, std::string query_text);
   /* Drops unmatched slots and lines without any value (query.cpp:87-109).
@@ -93,10 +110,14 @@ public:
   MaterializedBatchResult materialize() const &;
   MaterializedBatchResult materialize() &&;
   size_t num_queries() const { return queries_.size(); }
-  const std::vector<QueryResult> &queries() const { return queries_; }
+  /* Per-line results (QueryOffset with start, end and json_text); compact
+   * results are turned into this form on the first call. */
+  const std::vector<QueryResult> &queries() const;
 
 private:
-  std::vector<QueryResult> queries_;
+  void realize() const;
+  mutable std::vector<QueryResult> queries_;
+  mutable std::vector<std::vector<CompactLines>> pending_; // after queries_, in order
   std::vector<std::string> query_texts_;
 };
 
@@ -124,6 +145,16 @@ public:
   const std::string &query_text() const { return query_text_; }
   const std::vector<MaterializedLineResult> &lines() const { return lines_; }
   void add_line_result(MaterializedLineResult line) { lines_.push_back(std::move(line)); }
+  /* Moves the lines of `other` to the end of this result. */
+  void append(MaterializedQueryResult &&other) {
+    if (lines_.empty()) {
+      lines_ = std::move(other.lines_);
+    } else {
+      lines_.reserve(lines_.size() + other.lines_.size());
+      for (auto &l : other.lines_) lines_.push_back(std::move(l));
+    }
+    other.lines_.clear();
+  }
 
 private:
   std::string query_text_;
@@ -134,6 +165,8 @@ class MaterializedBatchResult {
 public:
   const std::vector<MaterializedQueryResult> &queries() const { return queries_; }
   void add_query_result(MaterializedQueryResult result) { queries_.push_back(std::move(result)); }
+  /* The results of query q, to move them out (e.g. MaterializedQueryResult::append). */
+  MaterializedQueryResult &&take_query(size_t q) { return std::move(queries_.at(q)); }
 
 private:
   std::vector<MaterializedQueryResult> queries_;

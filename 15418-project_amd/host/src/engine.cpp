@@ -89,8 +89,7 @@ query::MaterializedBatchResult Engine::query(const std::string &file_path,
     if (part.queries().size() != merged.size())
       throw error::common::ImplementationError(
           "Partition query results do not match the compiled query batch.");
-    for (size_t q = 0; q < merged.size(); ++q)
-      for (const auto &line : part.queries()[q].lines()) merged[q].add_line_result(line);
+    for (size_t q = 0; q < merged.size(); ++q) merged[q].append(part.take_query(q));
     profiler.end(pseg);
   }
 

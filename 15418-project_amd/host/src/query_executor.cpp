@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <exception>
 #include <string>
 #include <thread>
@@ -100,67 +101,62 @@ BatchQueryResult QueryExecutor::execute_batch(const BatchCompiledQuery &compiled
     }
     for (size_t q = first; q < last; ++q) {
       const int slots = descs[q - first].capacity;
-      const size_t count = static_cast<size_t>(num_lines) * slots * 2;
-      std::vector<int64_t> host_spans(count, -1);
-      spans[q - first].copy_to_host(host_spans.data(), count * sizeof(int64_t));
-      // one host string per match (query_executor_host.cu:104-119), built by
-      // up to 16 threads over contiguous line ranges, appended in order
       const size_t lines_n = static_cast<size_t>(num_lines);
-      std::vector<LineQueryResult> lines(lines_n);
+      const size_t count = lines_n * slots * 2;
+      CompactLines out;
+      out.base = base;
+      out.lines = lines_n;
+      out.slots = slots;
+      out.spans.assign(count, -1);
+      spans[q - first].copy_to_host(out.spans.data(), count * sizeof(int64_t));
+      // the values' texts, in span order, in one buffer (no string per value:
+      // those are made only when a line's results are read or materialized);
+      // ranges of lines on up to 16 threads, each at its prefix of the bytes
+      const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+      const size_t workers = std::min({hw, size_t{16}, std::max<size_t>(1, (lines_n + 65535) / 65536)});
+      const size_t per = (lines_n + workers - 1) / std::max<size_t>(workers, 1);
+      std::vector<size_t> bytes(workers + 1, 0);
       std::atomic<bool> bad_range{false};
-      auto build = [&](size_t lo, size_t hi) {
-        for (size_t line = lo; line < hi && !bad_range.load(std::memory_order_relaxed); ++line) {
-          LineQueryResult &lr = lines[line];
-          for (int s = 0; s < slots; ++s) {
-            const int64_t a = host_spans[(line * slots + s) * 2];
-            const int64_t b = host_spans[(line * slots + s) * 2 + 1];
-            if (a < 0 || b < 0) {
-              lr.add_offset(QueryOffset{});
-              continue;
-            }
-            if (a > b || static_cast<size_t>(b) > partition.size_bytes()) {
-              bad_range = true;
-              return;
-            }
-            QueryOffset off;
-            off.start = base + static_cast<size_t>(a);
-            off.end = base + static_cast<size_t>(b);
-            off.json_text = std::string(host + a, static_cast<size_t>(b - a));
-            lr.add_offset(std::move(off));
-          }
+      auto measure = [&](size_t t) {
+        size_t sum = 0;
+        for (size_t i = t * per * slots; i < std::min(lines_n, (t + 1) * per) * slots; ++i) {
+          const int64_t a = out.spans[2 * i], b = out.spans[2 * i + 1];
+          if (a < 0 || b < 0) continue;
+          if (a > b || static_cast<size_t>(b) > partition.size_bytes()) bad_range = true;
+          else sum += static_cast<size_t>(b - a);
+        }
+        bytes[t + 1] = sum;
+      };
+      auto copy = [&](size_t t) {
+        char *dst = out.texts.data() + bytes[t];
+        for (size_t i = t * per * slots; i < std::min(lines_n, (t + 1) * per) * slots; ++i) {
+          const int64_t a = out.spans[2 * i], b = out.spans[2 * i + 1];
+          if (a < 0 || b < 0) continue;
+          std::memcpy(dst, host + a, static_cast<size_t>(b - a));
+          dst += b - a;
         }
       };
-      const size_t hw = std::max(1u, std::thread::hardware_concurrency());
-      const size_t workers = std::min({hw, size_t{16}, (lines_n + 65535) / 65536});
-      if (workers <= 1) {
-        build(0, lines_n);
-      } else {
-        const size_t per = (lines_n + workers - 1) / workers;
-        std::vector<std::exception_ptr> errors(workers);
+      auto run = [&](auto &&fn) {
+        if (workers <= 1) {
+          fn(0);
+          return;
+        }
         std::vector<std::thread> pool;
         try {
-          for (size_t t = 0; t < workers; ++t)
-            pool.emplace_back([&, t] {
-              try {
-                build(t * per, std::min(lines_n, (t + 1) * per));
-              } catch (...) {
-                errors[t] = std::current_exception();
-                bad_range = true;
-              }
-            });
-        } catch (...) { // a thread could not start: stop and join the others
-          bad_range = true;
+          for (size_t t = 0; t < workers; ++t) pool.emplace_back([&, t] { fn(t); });
+        } catch (...) { // a thread could not start: join the others, then fail
           for (auto &th : pool) th.join();
           throw;
         }
         for (auto &th : pool) th.join();
-        for (auto &e : errors)
-          if (e) std::rethrow_exception(e);
-      }
+      };
+      run(measure);
       if (bad_range)
         throw error::query::QueryExecutionError("Query kernel produced an invalid result range");
-      for (size_t line = 0; line < lines_n; ++line)
-        batch.add_line_result(q, std::move(lines[line]));
+      for (size_t t = 0; t < workers; ++t) bytes[t + 1] += bytes[t];
+      out.texts.resize(bytes[workers]);
+      run(copy);
+      batch.add_compact(q, std::move(out));
     }
   }
   return batch;

@@ -354,6 +354,61 @@ TEST(Profiler, RecordsNestedSegments) {
 }
 
 // ---- device (tests/cuda/test_cuda.cpp) ------------------------------------
+// ---- compact executor results (texts made only when read) -----------------
+namespace {
+query::CompactLines sample_chunk(size_t base) {
+  // "abcdefghij": line 0 holds [1,3) and nothing; line 1 nothing; line 2
+  // holds [4,7) and [0,1)
+  query::CompactLines c;
+  c.base = base;
+  c.lines = 3;
+  c.slots = 2;
+  c.spans = {1, 3, -1, -1, -1, -1, -1, -1, 4, 7, 0, 1};
+  c.texts = "bc" "efg" "a";
+  return c;
+}
+} // namespace
+
+TEST(BatchQueryResult, CompactResultsReadPerLineAndMaterialized) {
+  query::BatchQueryResult r(1);
+  r.set_query_text(0, "$.x");
+  r.add_compact(0, sample_chunk(100));
+  const auto &lines = r.queries()[0].lines();
+  EXPECT_EQ(lines.size(), 3u);
+  EXPECT_EQ(*lines[0].offsets()[0].start, size_t{101});
+  EXPECT_EQ(*lines[0].offsets()[0].end, size_t{103});
+  EXPECT_EQ(*lines[0].offsets()[0].json_text, std::string("bc"));
+  EXPECT_TRUE(!lines[0].offsets()[1].json_text.has_value());
+  EXPECT_TRUE(!lines[1].offsets()[0].start.has_value());
+  EXPECT_EQ(*lines[2].offsets()[0].json_text, std::string("efg"));
+  EXPECT_EQ(*lines[2].offsets()[1].json_text, std::string("a"));
+  const auto m = r.materialize();
+  EXPECT_EQ(m.queries()[0].lines().size(), 2u);
+  EXPECT_EQ(m.queries()[0].lines()[1].values()[1].json_text(), std::string("a"));
+}
+
+TEST(BatchQueryResult, CompactMaterializeWithoutPerLineForm) {
+  // materialized straight from the compact form, in order with eager lines
+  query::BatchQueryResult r(1);
+  query::LineQueryResult eager;
+  query::QueryOffset off;
+  off.start = 0;
+  off.end = 1;
+  off.json_text = "z";
+  eager.add_offset(off);
+  r.add_line_result(0, eager);
+  r.add_compact(0, sample_chunk(0));
+  query::BatchQueryResult other(1);
+  other.add_compact(0, sample_chunk(50));
+  r.merge(other);
+  const auto m = std::move(r).materialize();
+  const auto &l = m.queries()[0].lines();
+  EXPECT_EQ(l.size(), 5u); // "z", then 2 lines of each chunk
+  EXPECT_EQ(l[0].values()[0].json_text(), std::string("z"));
+  EXPECT_EQ(l[1].values()[0].json_text(), std::string("bc"));
+  EXPECT_EQ(l[4].values()[0].json_text(), std::string("efg"));
+}
+
 GPU_TEST(DeviceArray, MoveAndOversizeCopy) {
   device::DeviceArray a(64);
   void *p = a.data();
