@@ -636,10 +636,20 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
     GPJ_HIP(hipGetLastError(), "fill launch");
   }
 #define GPJ_QLAUNCH(LV, NW, FIX)                                                          \
-  hipLaunchKernelGGL((gpj::query_batch_kernel<LV, NW>), dim3((uint32_t)blocks), dim3(64), 0, \
-                     st, d_in, (int64_t)n, d_starts, d_num_starts,                           \
-                     max_records, index->d_string, index->d_leveled, index->level_size,      \
-                     index->num_levels, mode, gpj::PosMode{span, FIX}, batch)
+  do {                                                                                    \
+    if (dense)                                                                            \
+      hipLaunchKernelGGL((gpj::query_batch_kernel<LV, NW, true>), dim3((uint32_t)blocks),  \
+                         dim3(64), 0, st, d_in, (int64_t)n, d_starts, d_num_starts,          \
+                         max_records, index->d_string, index->d_leveled,                    \
+                         index->level_size, index->num_levels, mode,                        \
+                         gpj::PosMode{span, FIX}, batch);                                    \
+    else                                                                                  \
+      hipLaunchKernelGGL((gpj::query_batch_kernel<LV, NW, false>), dim3((uint32_t)blocks), \
+                         dim3(64), 0, st, d_in, (int64_t)n, d_starts, d_num_starts,          \
+                         max_records, index->d_string, index->d_leveled,                    \
+                         index->level_size, index->num_levels, mode,                        \
+                         gpj::PosMode{span, FIX}, batch);                                    \
+  } while (0)
 #define GPJ_QPAIR(LV)                                                                     \
   do {                                                                                    \
     GPJ_QLAUNCH(LV, true, 0);                                                             \

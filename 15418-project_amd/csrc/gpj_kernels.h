@@ -159,7 +159,9 @@ struct PosMode {
   int32_t fixup;
 };
 
-template <int MAXLV, bool NARROW>
+// DENSE: the kernel can run dense queries (QuerySlot.dense); without it the
+// dense paths are compiled out.
+template <int MAXLV, bool NARROW, bool DENSE>
 __global__ void query_batch_kernel(const uint8_t *file, int64_t n, const int64_t *newline,
                                    const int64_t *num_lines, int64_t max_lines,
                                    const uint64_t *string, const uint64_t *leveled,

@@ -437,10 +437,11 @@ template <typename P> struct StageSize {
   static constexpr int value = sizeof(P) == 4 ? kDenseStage : 1;
 };
 
-// LDS home of the per-lane arrays of one wave.
-template <int MAXLV, typename P> struct WaveArrays {
+// LDS home of the per-lane arrays of one wave (with a dense-results stage of
+// STAGE entries per lane in the kernels that run dense queries).
+template <int MAXLV, typename P, int STAGE = 1> struct WaveArrays {
   static constexpr int kF = MAXLV < kMaxFrames ? MAXLV : kMaxFrames;
-  P stage[StageSize<P>::value][64][2]; // dense results: {start, end} per lane
+  P stage[STAGE][64][2]; // dense results: {start, end} per lane
   P level_end[MAXLV][64];
   P marked_pos[MAXLV][64];
   P fr_sep[kF][64];
@@ -476,7 +477,8 @@ template <int MAXLV, typename P> struct LaneState {
                                   // depth + 1, and whether a MOVE_DOWN follows
                                   // MOVE_TO_EACH
   LdsArray<P, kF> fr_sep;
-  __device__ __forceinline__ LaneState(WaveArrays<MAXLV, P> &w, int lane)
+  template <int S>
+  __device__ __forceinline__ LaneState(WaveArrays<MAXLV, P, S> &w, int lane)
       : level_end{w.level_end, lane}, curr_index{w.curr_index, lane},
         marked_pos{w.marked_pos, lane}, marked_lvl{w.marked_lvl, lane},
         fr_pack{w.fr_pack, lane}, fr_sep{w.fr_sep, lane} {}
@@ -577,8 +579,8 @@ __device__ __forceinline__ bool store_result(Ctx<P> &c, LaneState<MAXLV, P> &s, 
 // resumes the innermost MOVE_TO_EACH at its next element.  A lane whose
 // record needs the wide kernel (c.wide) stops.  Returns the lane's result
 // count.  Stored spans are file offsets (base + position).
-template <int MAXLV, typename P>
-__device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w, bool live,
+template <int MAXLV, typename P, int S>
+__device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P, S> &w, bool live,
                                              P line_start, P line_end, const IrReader ir,
                                              int32_t cap, bool variable, const Sink<P> &res) {
   LaneState<MAXLV, P> s(w, (int)(threadIdx.x & 63));
@@ -716,6 +718,10 @@ __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P> &w,
         if (sep_byte == '}') {
           idx = sep;
           fail = true;
+          break;
+        }
+        if (sep == idx) { // no progress: a ':' at the level end (malformed input;
+          fail = true;    // the reference would search it forever)
           break;
         }
         idx = nxt;
@@ -930,7 +936,7 @@ __device__ __forceinline__ void flush_dense(const QuerySlot &qs, const DenseStag
 // records to the dense fixup pass (fixup 2, wide kernel), which reruns the
 // records whose tmp_off is -1 and writes their spans straight to the CSR
 // output (spans + 2 * offsets[line], scanned offsets).
-template <int MAXLV, bool NARROW>
+template <int MAXLV, bool NARROW, bool DENSE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ? 4 : 1))) void query_batch_kernel(
     const uint8_t *__restrict__ file, int64_t n,
     const int64_t *__restrict__ newline, const int64_t *__restrict__ num_lines,
@@ -947,7 +953,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
   const int64_t wave_base = (int64_t)blockIdx.x * mode.per_wave;
   const int32_t lane = (int32_t)(threadIdx.x & 63);
   const int64_t span = NARROW ? pm.span : INT64_MAX;
-  __shared__ WaveArrays<MAXLV, P> arrays;
+  __shared__ WaveArrays<MAXLV, P, (DENSE && NARROW) ? kDenseStage : 1> arrays;
   for (int64_t wb = wave_base; wb < lines; wb += stride) {
     const int64_t line = wb + lane;
     bool valid = lane < mode.per_wave && line < lines;
@@ -955,7 +961,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
     if (!NARROW && pm.fixup == 1) {
       if (valid) valid = *mark_slot(batch.q[0], line) == kWideMark;
       if (!__ballot(valid)) continue; // (wave-uniform: lanes share wb)
-    } else if (!NARROW && pm.fixup == 2) {
+    } else if (DENSE && !NARROW && pm.fixup == 2) {
       if (valid)
         for (int q = 0; q < batch.count; ++q)
           if (batch.q[q].dense && batch.q[q].tmp_off[line] < 0 &&
@@ -983,8 +989,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
     if (record) record = trim_record(c, line_start, line_end);
     for (int q = 0; q < batch.count; ++q) {
       const QuerySlot &qs = batch.q[q];
-      const bool fix_dense = !NARROW && pm.fixup == 2;
-      if (fix_dense && !qs.dense) continue;
+      const bool dense = DENSE && qs.dense; // (compile-time false without DENSE)
+      const bool fix_dense = DENSE && !NARROW && pm.fixup == 2;
+      if (fix_dense && !dense) continue;
       // where this query's spans go (one interpreter call site per kernel)
       bool live = record && !c.wide;
       Sink<P> out{nullptr, 0, nullptr};
@@ -995,7 +1002,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
         out.res = qs.spans + 2 * off;
         out.keep = mine ? (room < 0 ? 0 : room) : 0;
         live = record && mine;
-      } else if (qs.dense) { // narrow kernel: the wave's stage; wide: counted only
+      } else if (dense) { // narrow kernel: the wave's stage; wide: counted only
         out.stage = NARROW ? arrays.stage : nullptr;
       } else {
         out.res = qs.spans + line * 2 * (int64_t)qs.cap;
@@ -1007,7 +1014,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
       const int64_t found = run_query<MAXLV, P>(c, arrays, live, line_start, line_end, ir,
                                                 qs.cap, qs.variable != 0, out);
       if (fix_dense) continue;
-      if (qs.dense) {
+      if (dense) {
         if (NARROW)
           flush_dense<P>(qs, batch.ds, arrays.stage, valid, mark || c.wide, line, found,
                          base_of(c));
@@ -1250,11 +1257,12 @@ __global__ void fill_u64_kernel(uint64_t *p, int64_t count, uint64_t v) {
                                                       const uint64_t *, const uint64_t *,       \
                                                       int64_t, int32_t, FilesMode, PosMode,     \
                                                       QueryBatch);                              \
-  template __global__ void query_batch_kernel<LV, NW>(const uint8_t *, int64_t, const int64_t *, \
-                                                      const int64_t *, int64_t,                 \
-                                                      const uint64_t *, const uint64_t *,       \
-                                                      int64_t, int32_t, RecordMode, PosMode,    \
-                                                      QueryBatch);
+  template __global__ void query_batch_kernel<LV, NW, false>(                                  \
+      const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *,    \
+      const uint64_t *, int64_t, int32_t, RecordMode, PosMode, QueryBatch);                     \
+  template __global__ void query_batch_kernel<LV, NW, true>(                                   \
+      const uint8_t *, int64_t, const int64_t *, const int64_t *, int64_t, const uint64_t *,    \
+      const uint64_t *, int64_t, int32_t, RecordMode, PosMode, QueryBatch);
 GPJ_QUERY_KERNELS(4, true)
 GPJ_QUERY_KERNELS(6, true)
 GPJ_QUERY_KERNELS(8, true)

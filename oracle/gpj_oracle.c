@@ -370,6 +370,7 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
         ++idx;
         idx = next_struct(c, level_end[lvl], idx, lvl);
         if (qbyte(c, idx) != ':') break;
+        const int64_t cand = idx;
         int64_t string_end = prev_bit(c->string, line_start + 1, idx - 1);
         int64_t string_start = string_end - key_len;
         int match = (string_start >= line_start && qbyte(c, string_start) == '"');
@@ -380,6 +381,10 @@ static void query_line(const qctx *c, const int64_t *nl, int64_t nl_size,
         ++idx;
         idx = next_struct(c, level_end[lvl], idx, lvl);
         if (qbyte(c, idx) == '}') goto stop;
+        /* no progress: the candidate was the level end itself, a ':' that
+         * only malformed input puts there; the reference would search it
+         * again forever (query.cu:254-302); the path fails instead */
+        if (idx == cand) goto stop;
       }
       ++idx;
       break;

@@ -115,3 +115,15 @@ def test_key_lookup_forms(gpu, orc):
         print(f"[query] q={q}", flush=True)
         got, want = run_both(gpu, orc, data, q, levels=4)
         np.testing.assert_array_equal(got, want)
+
+
+def test_key_search_ends_at_a_colon_level_end(gpu, orc):
+    # malformed records whose level end is a ':' (see the oracle test of the
+    # same name): the key search must end (the path fails) and match the
+    # oracle; well-formed records around them are unaffected
+    data = (b'{"a":{"x":1]:2}\n{"a":{"y":5},"b":1}\n{"a":{"x":1,"z":[3]]:7}\n'
+            b'{"a":{"q":"w"}}\n') * 50
+    for q in ("$.a.y", "$.a.z", "$.a.q"):
+        print(f"[query] malformed level end q={q}", flush=True)
+        got, want = run_both(gpu, orc, data, q)
+        np.testing.assert_array_equal(got, want)

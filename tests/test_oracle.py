@@ -436,3 +436,16 @@ def test_engine_query_partitioning_keeps_line_results(orc, gpj, seed):
     spans = orc.query(data, ix, compiled[0][0], compiled[0][2])
     texts = [[data[a:b]] for a, b in spans[:, 0] if a >= 0]
     assert whole[0] == texts
+
+
+def test_key_search_ends_at_a_colon_level_end(orc, gpj):
+    # malformed input whose level end is a ':' (a ']' closes the inner object,
+    # then a top-level ':'): the key search reaches the level end without a
+    # '}' separator; the reference's search_key loop (query.cu:254-302) would
+    # test that ':' forever, here the path fails
+    data = b'{"a":{"x":1]:2}\n{"a":{"y":5},"b":1}\n'
+    ir, depth, slots = gpj.compile_query("$.a.y")
+    ix = orc.build_index(data, max(depth, 1))
+    got = orc.query(data, ix, ir, slots)
+    assert got[0, 0, 0] == -1 and got[0, 0, 1] == -1
+    assert data[got[1, 0, 0]:got[1, 0, 1]] == b"5"
