@@ -164,6 +164,7 @@ def lib():
         L.gpj_debug_escape_quote.argtypes = [vp, u64, vp, vp, vp]
         L.gpj_debug_copy.argtypes = [vp, vp, u64, vp]
         L.gpj_debug_set_narrow_span.argtypes = [i64]
+        L.gpj_query_set_walk.argtypes = [i32]
         L.gpj_query_run.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp, vp]
         L.gpj_query_run_ext.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp,
                                         vp, vp]
@@ -218,7 +219,7 @@ def lib():
                      "gpj_batch_query_shaped",
                      "gpj_batch_per_file", "gpj_graph_begin", "gpj_graph_end",
                      "gpj_graph_launch", "gpj_graph_destroy", "gpj_stream_create",
-                     "gpj_stream_destroy", "gpj_debug_set_narrow_span"):
+                     "gpj_stream_destroy", "gpj_debug_set_narrow_span", "gpj_query_set_walk"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -649,6 +650,35 @@ class narrow_span:
 
     def __exit__(self, *exc):
         check(lib().gpj_debug_set_narrow_span(0), "narrow span")
+        return False
+
+
+WALK_AUTO, WALK_LANE, WALK_WAVE = 0, 1, 2
+WAVE_WALK_RECORDS, WAVE_WALK_LONG_RECORDS, WAVE_WALK_LONG_BYTES = 4096, 24576, 2048
+
+
+def auto_walk_is_wave(n: int, records: int) -> bool:
+    """The walk GPJ_WALK_AUTO picks for a 16-byte aligned input of n bytes and
+    a record bound `records` (True: one record per wave)."""
+    return records <= WAVE_WALK_RECORDS or (
+        records <= WAVE_WALK_LONG_RECORDS and n >= WAVE_WALK_LONG_BYTES * records)
+
+
+class walk:
+    """Context manager: which query walk the query calls use -- WALK_LANE
+    (one record per lane), WALK_WAVE (one record per wave, staged in LDS) or
+    WALK_AUTO (the default, see auto_walk_is_wave).  Results never depend on
+    it."""
+
+    def __init__(self, mode: int):
+        self.mode = mode
+
+    def __enter__(self):
+        check(lib().gpj_query_set_walk(self.mode), "query walk")
+        return self
+
+    def __exit__(self, *exc):
+        check(lib().gpj_query_set_walk(WALK_AUTO), "query walk")
         return False
 
 

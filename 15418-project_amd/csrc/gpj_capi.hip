@@ -513,6 +513,19 @@ namespace {
 // Record extent the narrow query kernels take (gpj::PosMode); tests lower it
 // through gpj_debug_set_narrow_span to drive the wide kernel on small inputs.
 std::atomic<int64_t> g_narrow_span{gpj::kNarrowSpan};
+// Query walk (GPJ_WALK_*).
+std::atomic<int32_t> g_walk{GPJ_WALK_AUTO};
+
+// One record per wave (query_wave_kernel) for this launch?
+bool wave_walk(const uint8_t *d_in, uint64_t n, int64_t max_records) {
+  if ((reinterpret_cast<uintptr_t>(d_in) & 15) != 0) return false;
+  const int32_t w = g_walk.load();
+  if (w == GPJ_WALK_WAVE) return true;
+  if (w == GPJ_WALK_LANE) return false;
+  return max_records <= GPJ_WAVE_WALK_RECORDS ||
+         (max_records <= GPJ_WAVE_WALK_LONG_RECORDS &&
+          n >= (uint64_t)GPJ_WAVE_WALK_LONG_BYTES * (uint64_t)max_records);
+}
 
 // Deepest interpreter level of a query batch (0 = unknown counts as 15).
 int deepest_level(const gpj_query_desc *queries, int32_t num_queries) {
@@ -617,6 +630,7 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
   if (!d_in || !d_starts || !d_num_starts || !index->d_string || !index->d_leveled)
     return fail(GPJ_ERR_QUERY_EXECUTION,
                 "Query execution requires GPU-resident newline, string, and bitmap indices");
+  const bool wave = wave_walk(d_in, n, max_records);
   // one wave per block (the kernel keeps each wave's interpreter arrays in
   // its block's LDS).  Few records: fewer per wave (down to 8), so that more
   // waves share the SIMDs and a wave's step waits for fewer records.
@@ -625,10 +639,12 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
   mode.per_wave = per_wave;
   const uint64_t blocks =
       std::min<uint64_t>(div_up((uint64_t)max_records, (uint64_t)per_wave), 1u << 22);
+  // the wave walk: one block (one wave) per record, up to 2^20 blocks
+  const uint32_t wave_blocks = (uint32_t)std::min<uint64_t>((uint64_t)max_records, 1u << 20);
   // 32-bit positions; records the narrow kernel marks (only possible when
   // the input is longer than the span) are rerun by the wide kernel
   const int64_t span = g_narrow_span.load();
-  const bool fixup = (int64_t)n > span;
+  const bool fixup = !wave && (int64_t)n > span;
   hipStream_t st = as_stream(stream);
   if (dense) { // empty staging regions
     hipLaunchKernelGGL(gpj::fill_u64_kernel, dim3(16), dim3(256), 0, st, batch.ds.cursors,
@@ -652,6 +668,13 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
   } while (0)
 #define GPJ_QPAIR(LV)                                                                     \
   do {                                                                                    \
+    if (wave) {                                                                           \
+      hipLaunchKernelGGL((gpj::query_wave_kernel<LV>), dim3(wave_blocks), dim3(64), 0, st,  \
+                         d_in, (int64_t)n, d_starts, d_num_starts, max_records,            \
+                         index->d_string, index->d_leveled, index->level_size,             \
+                         index->num_levels, mode, batch);                                  \
+      break;                                                                              \
+    }                                                                                     \
     GPJ_QLAUNCH(LV, true, 0);                                                             \
     if (fixup) GPJ_QLAUNCH(LV, false, 1);                                                 \
   } while (0)
@@ -708,6 +731,13 @@ gpj_status gpj_debug_set_narrow_span(int64_t bytes) {
   if (bytes < 4096 || bytes > gpj::kNarrowSpan)
     return fail(GPJ_ERR_INVALID_ARGUMENT, "narrow span: 0 or 4096 .. 2^31 - 2^20 bytes");
   g_narrow_span.store(bytes);
+  return GPJ_OK;
+}
+
+gpj_status gpj_query_set_walk(int32_t walk) {
+  if (walk != GPJ_WALK_AUTO && walk != GPJ_WALK_LANE && walk != GPJ_WALK_WAVE)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "query walk: GPJ_WALK_AUTO, _LANE or _WAVE");
+  g_walk.store(walk);
   return GPJ_OK;
 }
 

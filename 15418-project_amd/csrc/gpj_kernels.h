@@ -168,6 +168,16 @@ __global__ void query_batch_kernel(const uint8_t *file, int64_t n, const int64_t
                                    int64_t level_size, int32_t num_levels, RecordMode mode,
                                    PosMode pm, QueryBatch batch);
 
+// The same queries with one wave per record (gpj_query_wave.hip): outputs as
+// query_batch_kernel<MAXLV, false, true> with fixup 0 (64-bit positions, so
+// no record is marked for a second pass); inputs 16-byte aligned.
+template <int MAXLV>
+__global__ void query_wave_kernel(const uint8_t *file, int64_t n, const int64_t *newline,
+                                  const int64_t *num_lines, int64_t max_lines,
+                                  const uint64_t *string, const uint64_t *leveled,
+                                  int64_t level_size, int32_t num_levels, RecordMode mode,
+                                  QueryBatch batch);
+
 // Multi-file mode of the query kernel (see BatchFile).
 struct FilesMode {
   const BatchFile *files;

@@ -105,6 +105,11 @@ WORKLOADS = {
     "ndjson_1GiB_4q": (2, 1 << 30, 0x15418002,
                        ["$.user.lang", "$.e[0].f", "$.h[*].t", '$.h[?(@.k == "v")].id']),
     "ndjson_64MiB_ab": (0, 64 << 20, 0x15418000, ["$.a.b"]),
+    # the throughput schema at smaller sizes (where the query walks cross over)
+    "ndjson_64MiB_4q": (2, 64 << 20, 0x15418002,
+                        ["$.user.lang", "$.e[0].f", "$.h[*].t", '$.h[?(@.k == "v")].id']),
+    "ndjson_256MiB_4q": (2, 256 << 20, 0x15418002,
+                         ["$.user.lang", "$.e[0].f", "$.h[*].t", '$.h[?(@.k == "v")].id']),
     # one 1 GiB document: a top-level array of 2M objects; its elements are
     # the records (queries start with $[*])
     "array_1GiB_items_sku": (1, 2_000_000, 0x15418001, ["$[*].items[*].sku"]),
@@ -181,6 +186,9 @@ def main():
                          "emitted on the device).  The other layout is timed afterwards on "
                          "the same index and compared in full")
     ap.add_argument("--no-layout-check", action="store_true")
+    ap.add_argument("--walk", choices=["auto", "lane", "wave"], default="auto",
+                    help="query walk: one record per lane, one record per wave (staged in "
+                         "LDS), or auto (the wave walk for few records, or few long ones)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel over MIB MiB (profiler counter scale)")
     args = ap.parse_args()
@@ -207,6 +215,8 @@ def main():
     if args.workload == "serving":
         print(json.dumps(serving_line(args)), flush=True)
         return
+    gpj.check(gpj.lib().gpj_query_set_walk({"auto": gpj.WALK_AUTO, "lane": gpj.WALK_LANE,
+                                            "wave": gpj.WALK_WAVE}[args.walk]), "query walk")
     schema, nbytes, seed, queries = WORKLOADS[args.workload]
     if schema == -1:
         print(json.dumps(run_file_batch(args, gpj, hip, nbytes, seed)), flush=True)
@@ -504,11 +514,15 @@ def run_single(args, gpj, hip, schema, nbytes, seed, queries):
     # per query: newline index read, one pass over the first leveled row,
     # 8 bytes per emitted span (index-walk and text reads are data dependent)
     algo_queries = sum(8 * (records + 1) + n / 8 + 8 * m for m in matches)
+    # the walk the query calls took (gpj_query_set_walk; the input buffer is
+    # 16-byte aligned)
+    walk_used = args.walk if args.walk != "auto" else (
+        "wave" if gpj.auto_walk_is_wave(n, grid_lines) else "lane")
+    qkernel = "gpj::query_wave_kernel" if walk_used == "wave" else "gpj::query_batch_kernel"
     kernels = [
         roofline_entry("gpj::index_kernel", algo_index, index_ms,
                        profiled_traffic(args.workload, "gpj::index_kernel")),
-        roofline_entry("gpj::query_batch_kernel", algo_queries, batch_ms,
-                       profiled_traffic(args.workload, "gpj::query_batch_kernel")),
+        roofline_entry(qkernel, algo_queries, batch_ms, profiled_traffic(args.workload, qkernel)),
     ]
     dominant = max(kernels, key=lambda k: k["kernel_ms"])
     out = {
@@ -527,7 +541,7 @@ def run_single(args, gpj, hip, schema, nbytes, seed, queries):
         "config": {"workload": args.workload, "bytes": n,
                    "records": num_lines if array_mode else num_lines - 1,
                    "queries": queries, "levels": levels, "matches": matches,
-                   "results": args.results, "other_results": other,
+                   "results": args.results, "other_results": other, "walk": walk_used,
                    "parallelism": "single-gpu"},
         "roofline": dominant,
         "roofline_kernels": kernels,

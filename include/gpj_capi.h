@@ -184,6 +184,24 @@ gpj_status gpj_debug_copy(void *d_dst, const void *d_src, uint64_t bytes, void *
  * never depend on it.  Process-wide; set it while no query runs. */
 gpj_status gpj_debug_set_narrow_span(int64_t bytes);
 
+/* Query walk (no counterpart in the reference, whose kernel runs one thread
+ * per line, query.cu:144-185): GPJ_WALK_LANE runs one record per lane (64
+ * records per wave), GPJ_WALK_WAVE one record per wave over the record staged
+ * in LDS.  GPJ_WALK_AUTO (the default) picks the wave walk when the call's
+ * record bound R (max_lines, or the record bound of the records calls) is at
+ * most GPJ_WAVE_WALK_RECORDS, or at most GPJ_WAVE_WALK_LONG_RECORDS with at
+ * least GPJ_WAVE_WALK_LONG_BYTES input bytes per record (n >= that * R), and
+ * the lane walk otherwise (measured crossovers on MI355X).  Inputs whose
+ * device address is not 16-byte aligned always take the lane walk.  Results
+ * never depend on the walk.  Process-wide; set it while no query runs. */
+#define GPJ_WALK_AUTO 0
+#define GPJ_WALK_LANE 1
+#define GPJ_WALK_WAVE 2
+#define GPJ_WAVE_WALK_RECORDS 4096
+#define GPJ_WAVE_WALK_LONG_RECORDS 24576
+#define GPJ_WAVE_WALK_LONG_BYTES 2048
+gpj_status gpj_query_set_walk(int32_t walk);
+
 /* ---- query: replaces the query kernel launch of QueryExecutor::execute_batch
  *      (src/gpjson/query/kernels/orig/query_executor_host.cu:179-237,
  *      src/gpjson/query/kernels/orig/query.cu:144-454) */

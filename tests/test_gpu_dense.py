@@ -69,7 +69,8 @@ def test_dense_records_beyond_the_stage(gpu, orc):
         vals = ",".join(str(int(v)) for v in r.integers(0, 1000, k))
         lines.append('{"id":%d,"a":[%s],"b":{"c":[%s]}}' % (i, vals, vals))
     data = ("\n".join(lines) + "\n").encode()
-    check_dense(gpu, orc, data, ["$.a[*]", "$.b.c[*]", "$.id", "$.a[3]"], label="stage")
+    with gpu.walk(gpu.WALK_LANE):  # (the lane walk's stage holds 8 spans per record)
+        check_dense(gpu, orc, data, ["$.a[*]", "$.b.c[*]", "$.id", "$.a[3]"], label="stage")
 
 
 def test_dense_output_smaller_than_total(gpu, orc):
@@ -140,7 +141,7 @@ def test_dense_wide_positions(gpu, orc):
     data = mixed_data(43)
     qs = ["$.a.b[*].x", "$.a.b[149].s", "$.h[*].t", "$.tags[*]", "$.user.lang", "$.a.b[-1].x"]
     want = check_dense(gpu, orc, data, qs, label="default span")
-    with gpu.narrow_span(SPAN):
+    with gpu.narrow_span(SPAN), gpu.walk(gpu.WALK_LANE):
         got = check_dense(gpu, orc, data, qs, label=f"span={SPAN}")
     for (a, b), (c, d) in zip(want, got):
         np.testing.assert_array_equal(a, c)

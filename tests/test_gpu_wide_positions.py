@@ -40,7 +40,7 @@ def test_long_records_fixed_queries(gpu, orc):
     assert max(len(x) for x in data.split(b"\n")) > 2 * SPAN
     for q in QUERIES:
         print(f"[wide] fixed q={q}", flush=True)
-        with gpu.narrow_span(SPAN):
+        with gpu.narrow_span(SPAN), gpu.walk(gpu.WALK_LANE):
             got, want = run_both(gpu, orc, data, q)
         np.testing.assert_array_equal(got, want)
 
@@ -49,7 +49,7 @@ def test_long_records_extension_queries(gpu, orc):
     data = mixed_data(42)
     for q in EXT_QUERIES:
         print(f"[wide] ext q={q}", flush=True)
-        with gpu.narrow_span(SPAN):
+        with gpu.narrow_span(SPAN), gpu.walk(gpu.WALK_LANE):
             spans, counts, ws, wc = run_both_ext(gpu, orc, data, q, capacity=4)
         np.testing.assert_array_equal(counts, wc)
         np.testing.assert_array_equal(spans, ws)
@@ -64,7 +64,7 @@ def test_backward_search_past_the_span(gpu, orc):
     data = b"\n".join(lines * 3) + b"\n"
     for q in ("$.a[-1]", "$.a[-2]", "$.a[-3:]", "$.a[0]", "$.b.c[-1]", "$.s"):
         print(f"[wide] backward q={q}", flush=True)
-        with gpu.narrow_span(SPAN):
+        with gpu.narrow_span(SPAN), gpu.walk(gpu.WALK_LANE):
             got, want = run_both(gpu, orc, data, q, levels=4)
         np.testing.assert_array_equal(got, want)
         base, _ = run_both(gpu, orc, data, q, levels=4)
@@ -80,7 +80,7 @@ def test_batch_marks_in_both_result_modes(gpu, orc):
         levels = max(c[1] for c in compiled)
         dev = gpu.DeviceBuffer.from_host(np.frombuffer(data, dtype=np.uint8))
         idx = gpu.build_index(dev, len(data), levels)
-        with gpu.narrow_span(SPAN):
+        with gpu.narrow_span(SPAN), gpu.walk(gpu.WALK_LANE):
             batch = gpu.run_query_batch(dev, len(data), idx, compiled, capacity=8)
         oidx = orc.build_index(data, levels)
         for (ir, _d, slots), (spans, counts), q in zip(compiled, batch, queries):
