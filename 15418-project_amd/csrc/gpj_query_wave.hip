@@ -251,6 +251,165 @@ template <int MAXLV> struct Win {
   }
 };
 
+// A key as the 16 bytes ending at a compact key's ':' show it: opening
+// quote, key, closing quote, ':' (keys of at most 13 bytes that do not end in
+// a backslash; `fast` false otherwise).  lo holds the first 8 of the 16 bytes.
+struct KeyPattern {
+  uint64_t plo, phi, mlo, mhi;
+  int32_t klen;
+  bool fast;
+  __device__ static __forceinline__ KeyPattern make(const IrReader &ir, uint32_t key_pos,
+                                                    int32_t klen) {
+    KeyPattern k;
+    k.klen = klen;
+    const uint64_t key0 = ir.u64(key_pos);
+    const uint64_t key1 = klen > 8 ? ir.u64(key_pos + 8) : 0ull;
+    const uint32_t last = klen == 0 ? 0u
+                          : (uint32_t)(((klen > 8 ? key1 : key0) >> (8 * ((klen - 1) & 7))) & 0xFFu);
+    k.fast = klen <= 13 && last != '\\';
+    typedef unsigned __int128 u128;
+    const int32_t kl = k.fast ? klen : 0;
+    const u128 key = kl == 0 ? (u128)0
+                     : (((u128)key1 << 64) | key0) & (((u128)1 << (8 * kl)) - 1);
+    const u128 e = ((u128)'"' << (8 * (13 - kl))) | (key << (8 * (14 - kl))) |
+                   ((u128)'"' << 112) | ((u128)':' << 120);
+    const u128 m = ~(u128)0 << (8 * (13 - kl));
+    k.plo = (uint64_t)e;
+    k.phi = (uint64_t)(e >> 64);
+    k.mlo = (uint64_t)m;
+    k.mhi = (uint64_t)(m >> 64);
+    return k;
+  }
+};
+
+// Whether the key at the ':' at pos matches (the sequential test of the
+// reference's search_key: the string mask's last set bit before pos ends the
+// key's string, whose opening quote must lie at or after line_start).
+template <int MAXLV>
+__device__ __forceinline__ bool key_matches(Win<MAXLV> &w, const IrReader &ir, uint32_t key_pos,
+                                            int32_t klen, const KeyPattern &kp, int64_t pos,
+                                            int64_t line_start) {
+  const uint64_t t1 = w.text8(pos - 7);
+  const bool compact = pos - 2 >= line_start + 1 && ((t1 >> 48) & 0xFFu) == '"' &&
+                       ((t1 >> 40) & 0xFFu) != '\\';
+  if (compact && kp.fast)
+    return pos - 2 - klen >= line_start && ((t1 ^ kp.phi) & kp.mhi) == 0 &&
+           ((w.text8(pos - 15) ^ kp.plo) & kp.mlo) == 0;
+  const int64_t se = compact ? pos - 2 : w.last_bit(-1, line_start + 1, pos - 1);
+  const int64_t ss = se - klen;
+  bool match = se >= 0 && ss >= line_start && w.byte_at(ss, true) == '"';
+  for (int32_t k = 0; match && k < klen; k += 8) {
+    const int32_t rest = klen - k;
+    const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
+    match = ((w.text8(ss + 1 + k) ^ ir.u64(key_pos + (uint32_t)k)) & m) == 0;
+  }
+  return match;
+}
+
+__device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v) {
+  const int l = lane_id();
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(x, d, 64);
+    if (l >= d) x += o;
+  }
+  return x - v;
+}
+
+enum { kScanFound = 1, kScanFail = 2, kScanUnknown = 3, kScanExhausted = 4 };
+struct KeyScan {
+  int kind;     // kScan*
+  int64_t pos;  // the deciding character, or the last one scanned
+  int phase;    // role of the character after pos (0: key position)
+  int64_t cand; // the last key position scanned
+};
+
+// The key search over the row characters of level `lvl` after `pos` up to
+// lend, whole windows at a time: each lane takes one word and runs its
+// characters in order (rank parity from the wave's prefix count gives each
+// its role); the first lane with a deciding character decides.  A key
+// position decides when its byte is not ':' (found), when its compact key
+// matches (found) or when the 16 bytes cannot decide it (unknown); a
+// separator position decides when it is '}' (fail).
+template <int MAXLV>
+__device__ __forceinline__ KeyScan coop_key_scan(Win<MAXLV> &w, int lvl, int64_t pos, int phase,
+                                                 int64_t cand, int64_t lend, int64_t line_start,
+                                                 const KeyPattern &kp) {
+  const int l = lane_id();
+  for (;;) {
+    const int64_t start = pos + 1;
+    if (start > lend) return KeyScan{kScanExhausted, pos, phase, cand};
+    w.need_word(start >> 6, false);
+    const int64_t wb = w.base >> 6;
+    const int64_t rel = (start >> 6) - wb, srel = (lend >> 6) - wb;
+    uint64_t m = w.lds->rows[lvl][l];
+    if (l < rel) m = 0;
+    else if (l == rel) m &= ~0ull << (start & 63);
+    if (l > srel) m = 0;
+    else if (l == srel) m &= through64((int)(lend & 63));
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    const uint32_t pre = wave_excl_sum(cnt);
+    const uint8_t *text = reinterpret_cast<const uint8_t *>(w.lds->text);
+    int ev = 0;
+    int64_t evpos = 0;
+    uint32_t r = (uint32_t)phase + pre;
+    uint64_t mm = m;
+    while (mm) {
+      const int b = (int)__builtin_ctzll(mm);
+      mm &= mm - 1;
+      const int64_t p = (wb + l) * 64 + b;
+      const int64_t a = p - w.base; // 0 .. 4095
+      if ((r & 1u) == 0) {
+        if (a < 15) { ev = kScanUnknown; evpos = p; break; }
+        const uint64_t *tw = w.lds->text;
+        const int64_t a0 = a - 15, q = a0 >> 3;
+        const uint32_t sh = 8 * (uint32_t)(a0 & 7);
+        const uint64_t w0 = tw[q], w1 = tw[q + 1], w2 = tw[q + 2];
+        const uint64_t t0 = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
+        const uint64_t t1 = sh ? (w1 >> sh) | (w2 << (64 - sh)) : w1;
+        if ((t1 >> 56) != ':') { ev = kScanFound; evpos = p; break; }
+        const bool compact = p - 2 >= line_start + 1 && ((t1 >> 48) & 0xFFu) == '"' &&
+                             ((t1 >> 40) & 0xFFu) != '\\';
+        if (!(compact && kp.fast)) { ev = kScanUnknown; evpos = p; break; }
+        if (p - 2 - kp.klen >= line_start && ((t0 ^ kp.plo) & kp.mlo) == 0 &&
+            ((t1 ^ kp.phi) & kp.mhi) == 0) {
+          ev = kScanFound;
+          evpos = p;
+          break;
+        }
+      } else if (text[a] == '}') {
+        ev = kScanFail;
+        evpos = p;
+        break;
+      }
+      ++r;
+    }
+    const uint64_t evb = __ballot(ev != 0);
+    if (evb) {
+      const int f = (int)__builtin_ctzll(evb);
+      const int kind = __builtin_amdgcn_readlane(ev, f);
+      const int64_t p = (int64_t)lane64((uint64_t)evpos, f);
+      // (a deciding key position leaves phase 0 and its own position as the
+      // last key position; only kScanUnknown continues, from there)
+      return KeyScan{kind, p, 0, p};
+    }
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)(pre + cnt), 63);
+    if (total) {
+      const uint64_t nz = __ballot(m != 0);
+      const int f = 63 - (int)__builtin_clzll(nz);
+      const int64_t last = (wb + f) * 64 + 63 - (int64_t)__builtin_clzll(lane64(m, f));
+      if ((((uint32_t)phase + total - 1) & 1u) == 0) cand = last; // last was a key position
+      phase = (int)(((uint32_t)phase + total) & 1u);
+    }
+    // nothing decided: on from the window's end (or the tail past lend)
+    if (srel <= 63) {
+      return KeyScan{kScanExhausted, lend, phase, cand};
+    }
+    pos = (wb + 64) * 64 - 1;
+  }
+}
+
 // Where a query's spans go: entries below `keep` of `res` (int64 pairs), or,
 // in dense mode, the lanes of `stage_s` / `stage_e` (the first 64 spans).
 struct WaveSink {
@@ -322,42 +481,50 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
       qp += (uint32_t)klen;
       if (klen < 0 || w.byte_at(idx) != '{') { fail = true; break; }
       const int64_t lend = level_end.get(lvl);
-      const uint64_t key0 = ir.u64(key_pos);
-      const uint64_t mask0 = klen >= 8 ? ~0ull : ((1ull << (8 * klen)) - 1);
-      // a key of at most 5 bytes not ending in a backslash: the 8 bytes
-      // ending at a compact key's ':' decide it (see the lane kernel)
-      const bool short_key =
-          klen <= 5 && (klen == 0 || ((key0 >> (8 * (klen - 1))) & 0xFFu) != '\\');
-      const uint32_t jss = short_key ? (uint32_t)(5 - klen) : 0u;
-      const uint64_t pattern = ((uint64_t)'"' << (8 * jss)) | ((key0 & mask0) << (8 * (jss + 1))) |
-                               (0x22ull << 48) | (0x3Aull << 56);
-      const uint64_t pmask = ~0ull << (8 * jss);
-      for (;;) {
-        idx = w.next_structural(lend, idx + 1, lvl);
-        const uint64_t t = w.text8(idx - 7); // bytes idx - 7 .. idx
-        if ((t >> 56) != ':') break;
-        const int64_t cand = idx;
-        const bool compact = idx - 2 >= line_start + 1 && ((t >> 48) & 0xFFu) == '"' &&
-                             ((t >> 40) & 0xFFu) != '\\';
-        bool match;
-        if (compact && short_key) {
-          match = idx - 2 - klen >= line_start && ((t ^ pattern) & pmask) == 0;
-        } else {
-          const int64_t se = compact ? idx - 2 : w.last_bit(-1, line_start + 1, idx - 1);
-          const int64_t ss = se - klen;
-          match = se >= 0 && ss >= line_start && w.byte_at(ss, true) == '"';
-          for (int32_t k = 0; match && k < klen; k += 8) {
-            const int32_t rest = klen - k;
-            const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
-            match = ((w.text8(ss + 1 + k) ^ ir.u64(key_pos + (uint32_t)k)) & m) == 0;
+      const KeyPattern kp = KeyPattern::make(ir, key_pos, klen);
+      // the reference's search_key loop (query.cu:254-302) visits the row's
+      // structural characters after the '{' in order, alternating a key
+      // position (':' expected) and a separator position, and after the last
+      // one the level end forever.  The cooperative scan decides the real
+      // characters window by window; a key it cannot decide from the 16 bytes
+      // ending at its ':', and the level-end tail, go through the sequential
+      // steps below.
+      int64_t pos = idx, cand = -1;
+      int phase = 0; // 0: the next character is a key position, 1: a separator
+      int state = 0; // 0: searching, 1: found at pos, 2: failed
+      while (state == 0) {
+        const KeyScan ks = w.row_built(lvl) ? coop_key_scan(w, lvl, pos, phase, cand, lend,
+                                                             line_start, kp)
+                                            : KeyScan{kScanExhausted, pos, phase, cand};
+        pos = ks.pos;
+        phase = ks.phase;
+        cand = ks.cand;
+        if (ks.kind == kScanFound) { state = 1; break; }
+        if (ks.kind == kScanFail) { state = 2; break; }
+        if (ks.kind == kScanUnknown) { // a key position the scan could not decide
+          if ((w.text8(pos - 7) >> 56) != ':') { state = 1; break; }
+          if (key_matches(w, ir, key_pos, klen, kp, pos, line_start)) { state = 1; break; }
+          phase = 1;
+          cand = pos;
+          continue;
+        }
+        // past the row's last character: sequential steps (each reaches the
+        // level end at once, so this ends within a few)
+        for (;;) {
+          pos = w.next_structural(lend, pos + 1, lvl);
+          if (phase == 0) {
+            if ((w.text8(pos - 7) >> 56) != ':') { state = 1; break; }
+            cand = pos;
+            if (key_matches(w, ir, key_pos, klen, kp, pos, line_start)) { state = 1; break; }
+            phase = 1;
+          } else {
+            if (w.byte_at(pos) == '}' || pos == cand) { state = 2; break; }
+            phase = 0;
           }
         }
-        if (match) break;
-        idx = w.next_structural(lend, idx + 1, lvl);
-        if (w.byte_at(idx) == '}') { fail = true; break; }
-        if (idx == cand) { fail = true; break; } // a ':' at the level end (malformed)
       }
-      ++idx;
+      if (state == 2) fail = true;
+      idx = pos + 1;
       break;
     }
     case kMoveToIndex: {
