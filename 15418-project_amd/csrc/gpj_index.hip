@@ -99,16 +99,21 @@ __device__ __forceinline__ void load_word(const uint8_t *in, uint64_t n,
       x[4 * i + 3] = v.w;
     }
   } else {
+    // the input's tail, or an input not 16-byte aligned: 4-byte aligned
+    // dwords (each holding at least one input byte, so none crosses past the
+    // input's last page) shifted into place, bytes from n on cleared
+    const uintptr_t a = reinterpret_cast<uintptr_t>(in + pos);
+    const uint32_t k = (uint32_t)(a & 3);
+    const uint32_t *d = reinterpret_cast<const uint32_t *>(a - k);
+    const int64_t first = (int64_t)pos - (int64_t)k; // file offset of d[0]'s byte 0
+    uint32_t w[17];
+#pragma unroll
+    for (int j = 0; j < 17; ++j) w[j] = first + 4 * j < (int64_t)n ? d[j] : 0u;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-      uint32_t v = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        uint64_t q = pos + 4 * i + b;
-        uint32_t c = q < n ? in[q] : 0u;
-        v |= c << (8 * b);
-      }
-      x[i] = v;
+      const uint32_t v = __builtin_amdgcn_alignbyte(w[i + 1], w[i], k);
+      const int64_t valid = (int64_t)n - (int64_t)(pos + 4 * i);
+      x[i] = valid >= 4 ? v : (valid <= 0 ? 0u : v & ((1u << (8 * valid)) - 1u));
     }
   }
 }
