@@ -484,32 +484,52 @@ def run_single(args, gpj, hip, schema, nbytes, seed, queries):
                 sample = data[: min(data.size, sample_bytes >> 3)]
             else:
                 sample = cut_at_newline(data, sample_bytes)
+            outs = []
             t0 = time.time()
             ix = orc.build_index(sample, levels, threads)
             if array_mode:
                 b = orc.row_bounds(ix)
                 for c, v in zip(compiled, variable):
-                    orc.query_records(sample, ix, b, c[0], ext_cap if v else c[2], v)
+                    outs.append(orc.query_records(sample, ix, b, c[0], ext_cap if v else c[2], v))
             else:
                 for c, v in zip(compiled, variable):
                     if v:
-                        orc.query_ext(sample, ix, c[0], ext_cap, threads)
+                        outs.append(orc.query_ext(sample, ix, c[0], ext_cap, threads))
                     else:
-                        orc.query(sample, ix, c[0], c[2], threads)
+                        outs.append((orc.query(sample, ix, c[0], c[2], threads), None))
             secs = time.time() - t0
             return {"value": round(sample.size / secs / 1e9, 4), "threads": threads,
                     "sample": f"first {sample.size} bytes of the same data"
                               f"{'' if array_mode else ' (cut at a newline)'}, index + all "
-                              f"{len(compiled)} queries, {secs:.2f}s wall"}
+                              f"{len(compiled)} queries, {secs:.2f}s wall"}, outs
+
+        def sample_agrees(outs) -> int:
+            # the timed results of the sample's records equal the oracle's
+            # (all but the sample's last record, which the cut changes)
+            checked = 0
+            for (sp, ct), v, (g_off, g_sp) in zip(outs, variable, got):
+                o_off, o_sp = gpj.dense_from_slots(sp, ct if v else None)
+                r = o_off.size - 2
+                if r <= 0:
+                    continue
+                if not (np.array_equal(g_off[: r + 1], o_off[: r + 1]) and
+                        np.array_equal(g_sp[: int(g_off[r])], o_sp[: int(o_off[r])])):
+                    raise SystemExit("bench.py: results differ from the CPU oracle's on the "
+                                     "CPU baseline sample")
+                checked = r
+            return checked
 
         all_threads = args.cpu_threads or baseline_threads()
-        t_all = cpu_run(args.cpu_sample_mib << 20, all_threads)
-        t_one = cpu_run(args.cpu_sample_mib << 17, 1)  # 1/8 of the sample on one thread
+        t_all, outs = cpu_run(args.cpu_sample_mib << 20, all_threads)
+        checked = sample_agrees(outs)
+        del outs
+        t_one, _ = cpu_run(args.cpu_sample_mib << 17, 1)  # 1/8 of the sample on one thread
         cpu = {"value": t_all["value"], "unit": "GB/s", "cores": all_threads, "kind": "port",
                "sample": t_all["sample"] + "; CPU oracle (port of the reference algorithm, "
                          "index and line queries split over contiguous line ranges"
                          f"{', element queries on one thread' if array_mode else ''})",
-               "threads_all": t_all, "threads_1": t_one, **cpu_description()}
+               "threads_all": t_all, "threads_1": t_one,
+               "records_equal_to_gpu": checked, **cpu_description()}
 
     # per query: newline index read, one pass over the first leveled row,
     # 8 bytes per emitted span (index-walk and text reads are data dependent)
