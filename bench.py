@@ -40,7 +40,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "kernel_traffic.json")
 
 
-KERNEL_SOURCES = ["csrc/gpj_index.hip", "csrc/gpj_query.hip", "csrc/gpj_kernels.h"]
+KERNEL_SOURCES = ["csrc/gpj_index.hip", "csrc/gpj_query.hip", "csrc/gpj_query_wave.hip",
+                  "csrc/gpj_bytecode.h", "csrc/gpj_kernels.h"]
 
 
 def kernel_sources_hash() -> str:
