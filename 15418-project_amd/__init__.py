@@ -99,6 +99,18 @@ class DenseResultT(ctypes.Structure):
     ]
 
 
+class CarryT(ctypes.Structure):
+    """gpj_carry: the index scan's state before a shard."""
+    _fields_ = [("escape", ctypes.c_int32), ("in_string", ctypes.c_int32),
+                ("depth", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("newlines", ctypes.c_int64)]
+
+
+class ShardSummaryT(ctypes.Structure):
+    """gpj_shard_summary: a shard's transfer table and newline count."""
+    _fields_ = [("table", ctypes.c_uint64), ("newlines", ctypes.c_uint64)]
+
+
 class BatchFileT(ctypes.Structure):
     _fields_ = [
         ("offset", ctypes.c_uint64),
@@ -158,6 +170,11 @@ def lib():
         L.gpj_index_scratch_bytes.argtypes = [u64]
         L.gpj_index_scratch_bytes.restype = u64
         L.gpj_index_build_into.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp, vp]
+        L.gpj_index_build_shard.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp,
+                                            ctypes.POINTER(CarryT), vp]
+        L.gpj_index_shard_summary.argtypes = [vp, u64, vp, vp]
+        L.gpj_shard_carries.argtypes = [ctypes.POINTER(ShardSummaryT), i32,
+                                        ctypes.POINTER(CarryT)]
         L.gpj_index_status.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int32), vp]
         L.gpj_index_build_static.argtypes = [vp, u64, i32, vp, vp, vp, i64, vp, vp]
         L.gpj_debug_set_wait_limit.argtypes = [i64]
@@ -219,7 +236,8 @@ def lib():
                      "gpj_batch_query_shaped",
                      "gpj_batch_per_file", "gpj_graph_begin", "gpj_graph_end",
                      "gpj_graph_launch", "gpj_graph_destroy", "gpj_stream_create",
-                     "gpj_stream_destroy", "gpj_debug_set_narrow_span", "gpj_query_set_walk"):
+                     "gpj_stream_destroy", "gpj_debug_set_narrow_span", "gpj_query_set_walk",
+                     "gpj_index_build_shard", "gpj_index_shard_summary", "gpj_shard_carries"):
             getattr(L, name).restype = i32
         _lib = L
     return _lib
@@ -436,6 +454,61 @@ def build_index(dev: DeviceBuffer | int, n: int, levels: int, newline_capacity: 
     check(lib().gpj_index_build(ptr, int(n), int(levels), ctypes.byref(opts), ctypes.byref(raw),
                                 stream), "index build")
     return BuiltIndex(raw, n)
+
+
+class ShardIndex:
+    """A stream-ordered index of one shard (gpj_index_build_into, or
+    gpj_index_build_shard from a carry-in) with its own buffers."""
+
+    def __init__(self, dev: DeviceBuffer | int, n: int, levels: int, carry: CarryT | None = None,
+                 newline_capacity: int | None = None, stream=None):
+        L = lib()
+        ptr = dev.ptr if isinstance(dev, DeviceBuffer) else int(dev)
+        w = (n + 63) // 64
+        cap = n + 2 if newline_capacity is None else int(newline_capacity)
+        self.n, self.levels, self.words, self.cap = n, levels, w, cap
+        self.string = DeviceBuffer(max(w * 8, 8))
+        self.leveled = DeviceBuffer(max(w * 8 * levels, 8))
+        self.newline = DeviceBuffer(max(cap * 8, 8))
+        self.lines = DeviceBuffer(8)
+        self.scratch = DeviceBuffer(L.gpj_index_scratch_bytes(n))
+        if carry is None:
+            check(L.gpj_index_build_into(ptr, n, levels, self.string.ptr, self.leveled.ptr,
+                                         self.newline.ptr, cap, self.lines.ptr, self.scratch.ptr,
+                                         stream), "shard index")
+        else:
+            check(L.gpj_index_build_shard(ptr, n, levels, self.string.ptr, self.leveled.ptr,
+                                          self.newline.ptr, cap, self.lines.ptr,
+                                          self.scratch.ptr, ctypes.byref(carry), stream),
+                  "shard index from a carry-in")
+        check(L.gpj_stream_sync(stream))
+
+    def summary(self, stream=None) -> ShardSummaryT:
+        d = DeviceBuffer(16)
+        check(lib().gpj_index_shard_summary(self.scratch.ptr, self.n, d.ptr, stream), "summary")
+        check(lib().gpj_stream_sync(stream))
+        v = d.to_host(np.uint64, 2, stream)
+        return ShardSummaryT(int(v[0]), int(v[1]))
+
+    def string_words(self) -> np.ndarray:
+        return self.string.to_host(np.uint64, self.words)
+
+    def leveled_words(self) -> np.ndarray:
+        return self.leveled.to_host(np.uint64, self.words * self.levels).reshape(self.levels, -1)
+
+    def newlines(self) -> np.ndarray:
+        k = int(self.lines.to_host(np.int64, 1)[0])
+        if k < 0:
+            raise DeviceError("shard index build timed out")
+        return self.newline.to_host(np.int64, min(k, self.cap))
+
+
+def shard_carries(summaries) -> list:
+    """The state before each shard and after the last (gpj_shard_carries)."""
+    arr = (ShardSummaryT * max(len(summaries), 1))(*summaries)
+    out = (CarryT * (len(summaries) + 1))()
+    check(lib().gpj_shard_carries(arr, len(summaries), out), "shard carries")
+    return list(out)
 
 
 def run_query(dev: DeviceBuffer | int, n: int, index: BuiltIndex, ir: bytes, num_results: int,

@@ -65,13 +65,14 @@ gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int lev
                         uint64_t *d_string, uint64_t *d_leveled, int64_t *d_newline,
                         int64_t cap, int64_t *d_num_lines, uint64_t *desc,
                         uint32_t *counter, uint64_t tiles, hipStream_t st,
-                        gpj::BatchSeg seg = gpj::BatchSeg{nullptr, nullptr, nullptr}) {
+                        gpj::BatchSeg seg = gpj::BatchSeg{nullptr, nullptr, nullptr},
+                        uint64_t init = gpj::index_start_state(0, 0, -1)) {
   const int aligned = (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
   // single input: one extra workgroup (block 0) runs the prefix scan
   hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG>), dim3((uint32_t)(SEG ? tiles : tiles + 1)),
                      dim3(gpj::kTileWords), 0, st, d_in, n, words, levels, aligned, d_string,
                      d_leveled, d_newline, cap, d_num_lines, desc, counter, tiles, seg,
-                     g_wait_limit.load());
+                     g_wait_limit.load(), init);
   GPJ_HIP(hipGetLastError(), "index_kernel launch");
   if (!SEG) { // a failed build reports -1 lines
     hipLaunchKernelGGL(gpj::index_status_kernel, dim3(1), dim3(64), 0, st, counter + 1,
@@ -302,11 +303,13 @@ gpj_status gpj_debug_copy(void *d_dst, const void *d_src, uint64_t bytes, void *
   return GPJ_OK;
 }
 
-gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
-                                uint64_t *d_string, uint64_t *d_leveled,
-                                int64_t *d_newline, int64_t newline_capacity,
-                                int64_t *d_num_lines, void *d_scratch,
-                                void *stream) {
+} // extern "C"
+
+namespace {
+
+gpj_status build_into(const uint8_t *d_in, uint64_t n, int32_t levels, uint64_t *d_string,
+                      uint64_t *d_leveled, int64_t *d_newline, int64_t newline_capacity,
+                      int64_t *d_num_lines, void *d_scratch, void *stream, uint64_t init) {
   if (levels < 0 || levels > GPJ_MAX_INDEX_LEVELS)
     return fail(GPJ_ERR_INVALID_ARGUMENT,
                 "index levels must be in [0, " + std::to_string(GPJ_MAX_INDEX_LEVELS) + "]");
@@ -333,13 +336,75 @@ gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
   GPJ_HIP(hipGetLastError(), "index_clear_kernel launch");
 #define GPJ_DISPATCH(L)                                                            \
   return launch_index<L>(d_in, n, words, levels, d_string, d_leveled, d_newline,    \
-                         newline_capacity, d_num_lines, desc, counter, tiles, st)
+                         newline_capacity, d_num_lines, desc, counter, tiles, st,  \
+                         gpj::BatchSeg{nullptr, nullptr, nullptr}, init)
   if (levels <= 2) GPJ_DISPATCH(2);
   if (levels <= 4) GPJ_DISPATCH(4);
   if (levels <= 8) GPJ_DISPATCH(8);
   if (levels <= 16) GPJ_DISPATCH(16);
   GPJ_DISPATCH(22);
 #undef GPJ_DISPATCH
+}
+
+} // namespace
+
+extern "C" {
+
+gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
+                                uint64_t *d_string, uint64_t *d_leveled,
+                                int64_t *d_newline, int64_t newline_capacity,
+                                int64_t *d_num_lines, void *d_scratch,
+                                void *stream) {
+  return build_into(d_in, n, levels, d_string, d_leveled, d_newline, newline_capacity,
+                    d_num_lines, d_scratch, stream, gpj::index_start_state(0, 0, -1));
+}
+
+gpj_status gpj_index_build_shard(const uint8_t *d_in, uint64_t n, int32_t levels,
+                                 uint64_t *d_string, uint64_t *d_leveled,
+                                 int64_t *d_newline, int64_t newline_capacity,
+                                 int64_t *d_num_lines, void *d_scratch,
+                                 const gpj_carry *carry_in, void *stream) {
+  if (!carry_in) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_build_shard: null carry");
+  if ((carry_in->escape != 0 && carry_in->escape != 1) ||
+      (carry_in->in_string != 0 && carry_in->in_string != 1) || carry_in->depth < -128 ||
+      carry_in->depth > 127)
+    return fail(GPJ_ERR_INVALID_ARGUMENT,
+                "gpj_index_build_shard: escape and in_string are 0 or 1, depth in [-128, 127]");
+  return build_into(d_in, n, levels, d_string, d_leveled, d_newline, newline_capacity,
+                    d_num_lines, d_scratch, stream,
+                    gpj::index_start_state((uint32_t)carry_in->escape,
+                                           (uint32_t)carry_in->in_string, carry_in->depth));
+}
+
+gpj_status gpj_index_shard_summary(const void *d_scratch, uint64_t n,
+                                   gpj_shard_summary *d_summary, void *stream) {
+  if (!d_scratch || !d_summary)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_shard_summary: null scratch or output");
+  if (n == 0) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_shard_summary: empty shard");
+  hipLaunchKernelGGL(gpj::shard_summary_kernel, dim3(1), dim3(256), 0, as_stream(stream),
+                     static_cast<const uint64_t *>(d_scratch), tiles_for(n),
+                     reinterpret_cast<uint64_t *>(d_summary));
+  GPJ_HIP(hipGetLastError(), "shard_summary_kernel launch");
+  return GPJ_OK;
+}
+
+gpj_status gpj_shard_carries(const gpj_shard_summary *summaries, int32_t count,
+                             gpj_carry *carries) {
+  if (count < 0 || (count > 0 && !summaries) || !carries)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_shard_carries: null summaries or carries");
+  gpj_carry c{0, 0, -1, 0, 0};
+  carries[0] = c;
+  for (int32_t k = 0; k < count; ++k) {
+    const uint32_t s = 8u * ((uint32_t)c.escape | ((uint32_t)c.in_string << 1));
+    const uint32_t st = (uint32_t)(summaries[k].table >> s) & 3u;
+    const uint32_t dd = (uint32_t)(summaries[k].table >> (32 + s)) & 0xFFu;
+    c.escape = (int32_t)(st & 1u);
+    c.in_string = (int32_t)(st >> 1);
+    c.depth = (int32_t)(int8_t)(uint8_t)((uint32_t)c.depth + dd);
+    c.newlines += (int64_t)summaries[k].newlines;
+    carries[k + 1] = c;
+  }
+  return GPJ_OK;
 }
 
 } // extern "C"

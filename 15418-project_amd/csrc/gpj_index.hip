@@ -321,15 +321,11 @@ __device__ __forceinline__ void scan_step(uint64_t &tab, uint32_t &nl) {
 // through LDS), so the barriers always match.
 __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tiles,
                            uint32_t *error_flag, uint32_t max_spins, uint32_t lane,
-                           uint32_t wv) {
+                           uint32_t wv, uint64_t init) {
   __shared__ uint32_t s_ready[kScanWaves];
   __shared__ uint64_t s_tab[kScanWaves];
   __shared__ uint32_t s_nl[kScanWaves];
-  Carry carry;
-  carry.esc = 0;
-  carry.str = 0;
-  carry.depth = -1;
-  carry.nl = 0;
+  Carry carry = decode_inclusive(init);
   uint64_t next = 0;
   uint32_t spins = 0;
   const uint64_t lo = (uint64_t)(wv * 64 + lane) * kScanPer; // this lane's first slot
@@ -441,8 +437,9 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
     uint64_t *__restrict__ leveled_out, int64_t *__restrict__ newline_out,
     int64_t newline_cap, int64_t *__restrict__ num_lines_out,
     uint64_t *__restrict__ desc, uint32_t *__restrict__ tile_counter,
-    uint64_t num_tiles, BatchSeg seg, uint32_t max_spins) {
+    uint64_t num_tiles, BatchSeg seg, uint32_t max_spins, uint64_t init) {
   uint32_t *const error_flag = tile_counter + 1;
+  if (SEG) init = kInitialInclusive; // every file of a batch starts fresh
   __shared__ uint32_t s_tile;
   __shared__ uint8_t s_bsinfo[kTileWords];  // bit0 trailing parity, bit1 all-backslash
   __shared__ uint8_t s_lead[kTileWords];    // leading backslash run, bit7 = quote at its end
@@ -454,7 +451,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
   __shared__ uint32_t s_wscan[kTileWords / 64];
   __shared__ uint32_t s_first;              // k | lead_k << 16 | g << 24
   __shared__ uint32_t s_rows[LMAX + 1][2][kTileWords]; // per-lane leveled half-words (+ sink)
-  __shared__ uint32_t s_tail_esc;           // tile escape out (X=0) | all-bs << 1
+  __shared__ uint32_t s_tail_esc;           // tile escape transfer: 0, 1 constant, 2 identity, 3 flip
   __shared__ Carry s_carry;
 
   const uint32_t t = threadIdx.x;
@@ -468,7 +465,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
   //    costs ~11 ns per tile in series on MI355X.)
   if (!SEG) {
     if (blockIdx.x == 0) {
-      scan_tiles(desc, desc + num_tiles, num_tiles, error_flag, max_spins, lane, wv);
+      scan_tiles(desc, desc + num_tiles, num_tiles, error_flag, max_spins, lane, wv, init);
       return;
     }
     if (t == 0) s_tile = blockIdx.x - 1;
@@ -505,13 +502,27 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
     m = classify(x);
   }
 
-  // 2. escapes, assuming the tile starts with escape carry 0
+  // 2. escapes, assuming the tile starts with escape carry 0.  Each word's
+  //    escape transfer: the parity of the backslash run ending at its last
+  //    byte (a constant), or, for a word of only backslashes, its carry-in
+  //    passed through (64 of them) or flipped (an odd number at the input's
+  //    end).  Words past a single input's end pass the carry through, so the
+  //    last tile's escape-out is the input's (a shard's successor needs it).
   const uint64_t B = m.backslash;
-  const bool all_bs = (B == ~0ull);
+  const bool real = pos < file_end;
+  const uint32_t valid = real ? (uint32_t)min((uint64_t)64, file_end - pos) : 0u;
   const uint64_t e0 = escaped_from_zero(B);
-  const int lead = (B == ~0ull) ? 64 : __builtin_ctzll(~B);
-  const uint32_t trail = (uint32_t)((~B == 0ull) ? 0 : (__builtin_clzll(~B) & 1));
-  s_bsinfo[t] = (uint8_t)(trail | (all_bs ? 2u : 0u));
+  const int lead = (!SEG && !real) ? 64 : ((B == ~0ull) ? 64 : __builtin_ctzll(~B));
+  uint32_t mode; // 0, 1: escape-out constant; 2: carry passes; 3: carry flips
+  if (!real) {
+    mode = SEG ? 0u : 2u;
+  } else {
+    const uint64_t bv = valid == 64 ? B : (B << (64 - valid)); // last byte at bit 63
+    const uint32_t run = (~bv == 0ull) ? 64u : (uint32_t)__builtin_clzll(~bv);
+    mode = run >= valid ? (2u | (valid & 1u)) : (run & 1u);
+  }
+  const bool all_bs = mode >= 2u;
+  s_bsinfo[t] = (uint8_t)mode;
   s_lead[t] = (uint8_t)(lead | ((lead < 64 && ((m.quote >> lead) & 1ull)) ? 0x80 : 0));
   const uint64_t any_ballot = __ballot(all_bs);
   if (lane == 0) s_wany[wv] = any_ballot != 0;
@@ -521,20 +532,22 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
   for (uint32_t k = 0; k < kTileWords / 64; ++k) any_all_bs |= s_wany[k];
   uint32_t cin, tail_esc, first;
   if (any_all_bs) {
-    // rare: a word of only backslashes passes its carry through; one serial
-    // pass over the tile's words
+    // rare: a word of only backslashes (or past the input's end) passes its
+    // carry through or flips it; one serial pass over the tile's words, from
+    // carry-in 0 and 1, gives every word's carry-in for 0 and the tile's
+    // transfer (constant, 2 = identity, 3 = flip)
     if (t == 0) {
-      uint32_t c = 0;
+      uint32_t c = 0, c1 = 1;
       for (uint32_t k = 0; k < kTileWords; ++k) {
         s_cin[k] = (uint8_t)c;
-        const uint8_t info = s_bsinfo[k];
-        c = (info & 2u) ? c : (info & 1u);
+        const uint32_t md = s_bsinfo[k];
+        c = md == 2u ? c : (md == 3u ? c ^ 1u : md);
+        c1 = md == 2u ? c1 : (md == 3u ? c1 ^ 1u : md);
       }
-      s_tail_esc = c;
+      s_tail_esc = c == c1 ? c : (2u | c);
       uint32_t k = 0;
       while (k < kTileWords && (s_lead[k] & 0x7F) >= 64) ++k;
       if (k == kTileWords) {
-        s_tail_esc = 2u; // every word is backslashes: identity transfer
         s_first = (uint32_t)kTileWords;
       } else {
         s_first = k | ((uint32_t)(s_lead[k] & 0x7F) << 16) |
@@ -611,7 +624,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
 #pragma unroll
       for (uint32_t Y = 0; Y < 2; ++Y) {
         const uint32_t eff = Y ^ (X & g);
-        const uint32_t esc_out = (tail & 2u) ? X : (tail & 1u);
+        const uint32_t esc_out = (tail & 2u) ? (X ^ (tail & 1u)) : (tail & 1u);
         const uint32_t str_out = Y ^ tile_par ^ (X & g);
         const uint32_t dd = eff ? ((TOTb - D0b) & 0xFFu) : D0b;
         const uint32_t in = X | (Y << 1); // input state
@@ -619,11 +632,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
         tbl |= (uint64_t)dd << (8 + 8 * in);
       }
     }
-    Carry excl;
-    excl.esc = 0;
-    excl.str = 0;
-    excl.depth = -1;
-    excl.nl = 0;
+    Carry excl = decode_inclusive(init); // the first tile's carry-in
     if (!SEG && lane == 0) // the aggregate, for the scanner
       desc_store(&desc[tile], tbl | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
     if (!SEG && tile != first_tile) {
@@ -646,7 +655,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
         }
         uint64_t v = 0;
         if (lane < need) v = desc_load(&desc[g0 + lane]);
-        else if (lane == 63) v = g0 > 0 ? desc_load(&inc[g0 - 1]) : kInitialInclusive;
+        else if (lane == 63) v = g0 > 0 ? desc_load(&inc[g0 - 1]) : init;
         const uint64_t missing = __ballot(lane < need && (v >> kFlagShift) == 0);
         d = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63) << 32) |
             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
@@ -841,16 +850,47 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
   }
 }
 
-template __global__ void index_kernel<2, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
-template __global__ void index_kernel<2, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
-template __global__ void index_kernel<4, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
-template __global__ void index_kernel<4, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
-template __global__ void index_kernel<8, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
-template __global__ void index_kernel<8, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
-template __global__ void index_kernel<16, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
-template __global__ void index_kernel<16, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
-template __global__ void index_kernel<22, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
-template __global__ void index_kernel<22, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t);
+template __global__ void index_kernel<2, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<2, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<4, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<4, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<8, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<8, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<16, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<16, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<22, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<22, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+
+// One workgroup: each thread composes a contiguous run of tile tables, then
+// the runs are composed in order.
+__global__ __launch_bounds__(256) void shard_summary_kernel(const uint64_t *desc,
+                                                            uint64_t num_tiles,
+                                                            uint64_t *out) {
+  __shared__ uint64_t s_tab[256];
+  __shared__ uint64_t s_nl[256];
+  const uint32_t t = threadIdx.x;
+  const uint64_t per = (num_tiles + 255) / 256;
+  const uint64_t lo = (uint64_t)t * per;
+  const uint64_t hi = lo + per < num_tiles ? lo + per : num_tiles;
+  uint64_t tab = kIdentityTable, nl = 0;
+  for (uint64_t k = lo; k < hi; ++k) {
+    const uint64_t d = desc[k];
+    tab = tbl_compose(tbl_unpack(d), tab);
+    nl += (d >> 40) & ((1ull << 22) - 1);
+  }
+  s_tab[t] = tab;
+  s_nl[t] = nl;
+  __syncthreads();
+  if (t == 0) {
+    uint64_t all = kIdentityTable, total = 0;
+    for (uint32_t k = 0; k < 256; ++k) {
+      all = tbl_compose(s_tab[k], all);
+      total += s_nl[k];
+    }
+    out[0] = all;
+    out[1] = total;
+  }
+}
 
 __global__ void index_status_kernel(const uint32_t *error_flag, int64_t *lines_out) {
   if (threadIdx.x == 0 && __hip_atomic_load(error_flag, __ATOMIC_RELAXED,

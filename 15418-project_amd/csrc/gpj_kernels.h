@@ -47,7 +47,26 @@ __global__ void index_kernel(const uint8_t *in, uint64_t n, uint64_t words,
                              uint64_t *leveled_out, int64_t *newline_out,
                              int64_t newline_cap, int64_t *num_lines_out,
                              uint64_t *desc, uint32_t *tile_counter,
-                             uint64_t num_tiles, BatchSeg seg, uint32_t max_spins);
+                             uint64_t num_tiles, BatchSeg seg, uint32_t max_spins,
+                             uint64_t init);
+
+// The state the index scan carries from byte to byte (escape parity,
+// in-string parity, nesting depth as int8), encoded as the `init` argument of
+// index_kernel: the state before the input's first byte.  A whole input
+// starts at (0, 0, -1); a shard of a larger document starts at the state its
+// predecessors leave (a multi-file batch always starts its files at
+// (0, 0, -1)).
+__host__ __device__ inline uint64_t index_start_state(uint32_t esc, uint32_t str, int32_t depth) {
+  return (uint64_t)(esc & 1u) | ((uint64_t)(str & 1u) << 1) |
+         ((uint64_t)((uint32_t)depth & 0xFFu) << 2) | (2ull << 62);
+}
+
+// After a single-input index launch: composes its tiles' transfer tables
+// (still in the scratch's descriptors) into the input's own: out[0] the
+// unpacked table (byte s of bits 0..31: the state after the input from
+// start state s = esc | str << 1, as esc | str << 1; byte s of bits
+// 32..63: the depth change from s, mod 256), out[1] the newline count.
+__global__ void shard_summary_kernel(const uint64_t *desc, uint64_t num_tiles, uint64_t *out);
 
 // After an index launch: when its failure flag is set, *lines_out = -1 (the
 // single-input line count, or the total of a batch's line prefix), so every

@@ -132,6 +132,53 @@ gpj_status gpj_index_build_into(const uint8_t *d_in, uint64_t n, int32_t levels,
                                 int64_t *d_num_lines, void *d_scratch,
                                 void *stream);
 
+/* ---- shards of one document (no counterpart in the reference, which
+ *      indexes each partition from the start state,
+ *      src/gpjson/index/orig_index_builder.cu:296-395).  A document cut at
+ *      64-byte multiples into shards (one per GPU) is indexed shard by shard:
+ *      (1) each shard is built alone (gpj_index_build_into) and
+ *      gpj_index_shard_summary condenses it into its transfer table, 16 bytes;
+ *      (2) the summaries, gathered in shard order, give every shard's carry-in
+ *      (gpj_shard_carries, on the host); (3) gpj_index_build_shard rebuilds
+ *      each shard from its carry-in.  Its string mask and leveled rows are
+ *      then the document's words from the shard's first word on, and its
+ *      newline index holds the shard's newlines as offsets from the shard's
+ *      start (entry 0 = 0 as for any input; document entry
+ *      carry.newlines + i is the shard's entry i, i >= 1, plus the shard's
+ *      offset). */
+typedef struct {
+  int32_t escape;    /* the shard's first byte is escaped (0 or 1) */
+  int32_t in_string; /* the shard starts inside a string (0 or 1) */
+  int32_t depth;     /* nesting depth before it, int8 (the document starts at -1) */
+  int32_t reserved;
+  int64_t newlines;  /* newlines before the shard */
+} gpj_carry;
+/* The state change across a shard: byte s (s = escape | in_string << 1) of
+ * bits 0..31 of `table` is the state after the shard from start state s
+ * (escape | in_string << 1), byte s of bits 32..63 its depth change mod 256;
+ * `newlines` counts the shard's newlines. */
+typedef struct {
+  uint64_t table;
+  uint64_t newlines;
+} gpj_shard_summary;
+/* After a gpj_index_build_into or gpj_index_build_shard of n bytes with
+ * d_scratch: writes that input's summary to device memory.  Stream-ordered. */
+gpj_status gpj_index_shard_summary(const void *d_scratch, uint64_t n,
+                                   gpj_shard_summary *d_summary, void *stream);
+/* Host: carries[k] = the state before shard k (carries[0] = the document's
+ * start, escape 0, in_string 0, depth -1, newlines 0; carries[count] = the
+ * state after the last shard). */
+gpj_status gpj_shard_carries(const gpj_shard_summary *summaries, int32_t count,
+                             gpj_carry *carries);
+/* gpj_index_build_into from the carry-in state instead of the document's
+ * start (the carry's newline count is not used: the newline index stays
+ * shard-relative). */
+gpj_status gpj_index_build_shard(const uint8_t *d_in, uint64_t n, int32_t levels,
+                                 uint64_t *d_string, uint64_t *d_leveled,
+                                 int64_t *d_newline, int64_t newline_capacity,
+                                 int64_t *d_num_lines, void *d_scratch,
+                                 const gpj_carry *carry_in, void *stream);
+
 /* gpj_index_build_into without a scratch argument, for callers that cannot
  * allocate device memory (code under HIP graph capture, serving loops and
  * harnesses that forbid hipMalloc).  It has no counterpart in the reference.
