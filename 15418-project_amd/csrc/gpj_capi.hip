@@ -583,7 +583,8 @@ std::atomic<int32_t> g_walk{GPJ_WALK_AUTO};
 
 // One record per wave (query_wave_kernel) for this launch?
 bool wave_walk(const uint8_t *d_in, uint64_t n, int64_t max_records) {
-  if ((reinterpret_cast<uintptr_t>(d_in) & 15) != 0) return false;
+  // (its positions are 32-bit: inputs up to kNarrowSpan bytes)
+  if ((reinterpret_cast<uintptr_t>(d_in) & 15) != 0 || n > (uint64_t)gpj::kNarrowSpan) return false;
   const int32_t w = g_walk.load();
   if (w == GPJ_WALK_WAVE) return true;
   if (w == GPJ_WALK_LANE) return false;

@@ -15,7 +15,8 @@
 // lane takes one of the window's 64 words, and a ballot picks the first (or,
 // searching backwards, the last) word holding a set bit.  An access outside
 // the window restages it around the position (records longer than the
-// window, searches that leave it).  Positions are 64-bit file offsets, so no
+// window, searches that leave it).  Positions are 32-bit file offsets: the
+// host takes this walk for inputs of at most kNarrowSpan bytes only, so no
 // record needs a second pass.
 //
 // One record costs the whole wave, so this kernel is meant for launches with
@@ -33,9 +34,10 @@ namespace gpj {
 namespace {
 
 typedef uint64_t u64x2w __attribute__((ext_vector_type(2)));
+typedef int32_t Pos; // a byte position (file offset), or a word index
 
-constexpr int64_t kWin = 4096;                 // staged text bytes (one word per lane)
-constexpr int64_t kWinPast = 80;               // bytes staged past the window
+constexpr Pos kWin = 4096;                     // staged text bytes (one word per lane)
+constexpr Pos kWinPast = 80;                   // bytes staged past the window
 constexpr int kTextWords = (int)((kWin + kWinPast) / 8) + 2;
 
 template <int MAXLV> struct WaveLds {
@@ -62,46 +64,40 @@ __device__ __forceinline__ uint64_t lane64(uint64_t v, int l) {
 }
 
 // A per-record array spread over the lanes: entry k lives in lane k.
-struct LaneI64 {
-  int64_t v;
-  __device__ __forceinline__ int64_t get(int k) const { return (int64_t)lane64((uint64_t)v, k); }
-  __device__ __forceinline__ void set(int k, int64_t x) { v = lane_id() == k ? x : v; }
-  __device__ __forceinline__ void fill(int64_t x) { v = x; }
+struct LaneI32 {
+  int32_t v;
+  __device__ __forceinline__ int32_t get(int k) const { return __builtin_amdgcn_readlane(v, k); }
+  __device__ __forceinline__ void set(int k, int32_t x) { v = lane_id() == k ? x : v; }
   // entry k := x for every k > i
-  __device__ __forceinline__ void fill_above(int i, int64_t x) { v = lane_id() > i ? x : v; }
+  __device__ __forceinline__ void fill_above(int i, int32_t x) { v = lane_id() > i ? x : v; }
   // walking up from entry i + 1: := x until the first entry already x (at or
   // above `lim`, the array's end, nothing changes)
-  __device__ __forceinline__ void fill_above_until(int i, int64_t x, int lim) {
+  __device__ __forceinline__ void fill_above_until(int i, int32_t x, int lim) {
     const uint64_t stop = __ballot(lane_id() > i && v == x);
     const int first = stop ? (int)__builtin_ctzll(stop) : lim;
     v = (lane_id() > i && lane_id() < first && lane_id() < lim) ? x : v;
   }
 };
-struct LaneI32 {
-  int32_t v;
-  __device__ __forceinline__ int32_t get(int k) const { return __builtin_amdgcn_readlane(v, k); }
-  __device__ __forceinline__ void set(int k, int32_t x) { v = lane_id() == k ? x : v; }
-};
 
 // The input and the staged window of one wave.
 template <int MAXLV> struct Win {
   const uint8_t *file; // 16-byte aligned
-  int64_t n;
-  int64_t words;       // string-mask words
+  Pos n;
+  Pos words;           // string-mask words
   const uint64_t *str;
   const uint64_t *lev;
-  int64_t level_size;
+  Pos level_size;
   int32_t num_levels;
   int32_t lbase;
   int32_t rows;        // staged rows (levels lbase .. lbase + rows - 1)
-  int64_t base;        // window start (a multiple of 64), -1 before the first stage
+  Pos base;            // window start (a multiple of 64), -1 before the first stage
   WaveLds<MAXLV> *lds;
 
   // Granule at file offset off (a multiple of 16, >= 0); bytes from n on read 0.
-  __device__ __forceinline__ u64x2w granule(int64_t off) const {
+  __device__ __forceinline__ u64x2w granule(Pos off) const {
     if (off >= n) return u64x2w{0, 0};
     u64x2w v = *reinterpret_cast<const u64x2w *>(file + off);
-    const int64_t valid = n - off;
+    const Pos valid = n - off;
     if (valid < 16) {
       if (valid <= 8) {
         v.y = 0;
@@ -113,16 +109,16 @@ template <int MAXLV> struct Win {
     return v;
   }
 
-  __device__ __forceinline__ void stage(int64_t b) {
+  __device__ __forceinline__ void stage(Pos b) {
     base = b;
     const int l = lane_id();
-    const int64_t off = b + 64 * (int64_t)l;
+    const Pos off = b + 64 * l;
     u64x2w g[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) g[k] = granule(off + 16 * k);
     u64x2w e = u64x2w{0, 0};
-    if (l < 5) e = granule(b + kWin + 16 * (int64_t)l);
-    const int64_t wi = (b >> 6) + l;
+    if (l < 5) e = granule(b + kWin + 16 * l);
+    const Pos wi = (b >> 6) + l;
     uint64_t rv[MAXLV];
 #pragma unroll
     for (int k = 0; k < MAXLV; ++k)
@@ -149,34 +145,34 @@ template <int MAXLV> struct Win {
 
   // Text bytes [p, p + len) (len <= 16) inside the window; a forward walk
   // keeps 256 bytes behind p, a backward one 256 bytes past it.
-  __device__ __forceinline__ void need_text(int64_t p, int64_t len, bool backward) {
+  __device__ __forceinline__ void need_text(Pos p, Pos len, bool backward) {
     if (p >= base && p + len <= base + kWin + kWinPast - 8) return;
-    int64_t nb = backward ? ((p + len + 63) & ~63ll) + 256 - kWin : (p & ~63ll) - 256;
+    Pos nb = backward ? ((p + len + 63) & ~63) + 256 - kWin : (p & ~63) - 256;
     if (nb < 0) nb = 0;
     stage(nb);
   }
   // Bitmap word wi inside the window.
-  __device__ __forceinline__ void need_word(int64_t wi, bool backward) {
-    const int64_t wb = base >> 6;
+  __device__ __forceinline__ void need_word(Pos wi, bool backward) {
+    const Pos wb = base >> 6;
     if (base >= 0 && wi >= wb && wi < wb + 64) return;
-    int64_t nb = backward ? ((wi + 5) << 6) - kWin : (wi - 4) << 6;
+    Pos nb = backward ? ((wi + 5) << 6) - kWin : (wi - 4) << 6;
     if (nb < 0) nb = 0;
     stage(nb);
   }
 
   // Byte at p; bytes outside [0, n) read as 0.
-  __device__ __forceinline__ uint32_t byte_at(int64_t p, bool backward = false) {
+  __device__ __forceinline__ uint32_t byte_at(Pos p, bool backward = false) {
     if (p < 0 || p >= n) return 0;
     need_text(p, 1, backward);
     const uint8_t *t = reinterpret_cast<const uint8_t *>(lds->text);
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)t[p - base]);
   }
   // Little-endian 8 bytes from p (bytes outside [0, n) read as 0).
-  __device__ __forceinline__ uint64_t text8(int64_t p) {
+  __device__ __forceinline__ uint64_t text8(Pos p) {
     if (p >= n || p <= -8) return 0;
-    const int64_t q = p < 0 ? 0 : p;
+    const Pos q = p < 0 ? 0 : p;
     need_text(q, 8, false);
-    const int64_t a = q - base;
+    const Pos a = q - base;
     const uint64_t lo = uni64((int64_t)lds->text[a >> 3]);
     const uint32_t sh = 8 * (uint32_t)(a & 7);
     uint64_t v = lo;
@@ -187,12 +183,11 @@ template <int MAXLV> struct Win {
 
   // First set bit of staged row `which` (-1: the string mask) in [idx,
   // stop], or -1 (idx >= 0); the source has `limit` words.
-  __device__ __forceinline__ int64_t next_bit(int which, int64_t idx, int64_t stop,
-                                              int64_t limit) {
+  __device__ __forceinline__ Pos next_bit(int which, Pos idx, Pos stop, Pos limit) {
     for (;;) {
       need_word(idx >> 6, false);
-      const int64_t wb = base >> 6;
-      const int64_t rel = (idx >> 6) - wb, srel = (stop >> 6) - wb;
+      const Pos wb = base >> 6;
+      const Pos rel = (idx >> 6) - wb, srel = (stop >> 6) - wb;
       const int l = lane_id();
       uint64_t m = which < 0 ? lds->str[l] : lds->rows[which][l];
       if (l < rel) m = 0;
@@ -202,19 +197,19 @@ template <int MAXLV> struct Win {
       const uint64_t b = __ballot(m != 0);
       if (b) {
         const int f = (int)__builtin_ctzll(b);
-        return (wb + f) * 64 + (int64_t)__builtin_ctzll(lane64(m, f));
+        return (wb + f) * 64 + (Pos)__builtin_ctzll(lane64(m, f));
       }
       if (srel <= 63 || wb + 64 >= limit) return -1;
       idx = (wb + 64) * 64;
     }
   }
   // Last set bit of `which` in [lo, idx], or -1 (lo >= 0).
-  __device__ __forceinline__ int64_t last_bit(int which, int64_t lo, int64_t idx) {
+  __device__ __forceinline__ Pos last_bit(int which, Pos lo, Pos idx) {
     if (idx < lo || idx < 0) return -1;
     for (;;) {
       need_word(idx >> 6, true);
-      const int64_t wb = base >> 6;
-      const int64_t rel = (idx >> 6) - wb, lrel = (lo >> 6) - wb;
+      const Pos wb = base >> 6;
+      const Pos rel = (idx >> 6) - wb, lrel = (lo >> 6) - wb;
       const int l = lane_id();
       uint64_t m = which < 0 ? lds->str[l] : lds->rows[which][l];
       if (l > rel) m = 0;
@@ -224,7 +219,7 @@ template <int MAXLV> struct Win {
       const uint64_t b = __ballot(m != 0);
       if (b) {
         const int f = 63 - (int)__builtin_clzll(b);
-        return (wb + f) * 64 + 63 - (int64_t)__builtin_clzll(lane64(m, f));
+        return (wb + f) * 64 + 63 - (Pos)__builtin_clzll(lane64(m, f));
       }
       if (lrel >= 0 || wb == 0) return -1;
       idx = wb * 64 - 1;
@@ -236,17 +231,17 @@ template <int MAXLV> struct Win {
   }
   // First structural bit of the level's row in [idx, level_end], else
   // level_end (find_next_structural_char, query.cu:79-115; idx >= 0).
-  __device__ __forceinline__ int64_t next_structural(int64_t level_end, int64_t idx, int level) {
+  __device__ __forceinline__ Pos next_structural(Pos level_end, Pos idx, int level) {
     if (idx >= level_end || !row_built(level)) return level_end;
     if (idx < 0) idx = 0;
-    const int64_t s = next_bit(level, idx, level_end, level_size);
+    const Pos s = next_bit(level, idx, level_end, level_size);
     return s < 0 ? level_end : s;
   }
   // Last structural bit of the level's row in [0, idx], else 0
   // (find_previous_structural_char, query.cu:117-128, level start 0).
-  __device__ __forceinline__ int64_t prev_structural(int64_t idx, int level) {
+  __device__ __forceinline__ Pos prev_structural(Pos idx, int level) {
     if (idx <= 0 || !row_built(level)) return 0;
-    const int64_t s = last_bit(level, 0, idx);
+    const Pos s = last_bit(level, 0, idx);
     return s < 0 ? 0 : s;
   }
 };
@@ -287,16 +282,16 @@ struct KeyPattern {
 // key's string, whose opening quote must lie at or after line_start).
 template <int MAXLV>
 __device__ __forceinline__ bool key_matches(Win<MAXLV> &w, const IrReader &ir, uint32_t key_pos,
-                                            int32_t klen, const KeyPattern &kp, int64_t pos,
-                                            int64_t line_start) {
+                                            int32_t klen, const KeyPattern &kp, Pos pos,
+                                            Pos line_start) {
   const uint64_t t1 = w.text8(pos - 7);
   const bool compact = pos - 2 >= line_start + 1 && ((t1 >> 48) & 0xFFu) == '"' &&
                        ((t1 >> 40) & 0xFFu) != '\\';
   if (compact && kp.fast)
     return pos - 2 - klen >= line_start && ((t1 ^ kp.phi) & kp.mhi) == 0 &&
            ((w.text8(pos - 15) ^ kp.plo) & kp.mlo) == 0;
-  const int64_t se = compact ? pos - 2 : w.last_bit(-1, line_start + 1, pos - 1);
-  const int64_t ss = se - klen;
+  const Pos se = compact ? pos - 2 : w.last_bit(-1, line_start + 1, pos - 1);
+  const Pos ss = se - klen;
   bool match = se >= 0 && ss >= line_start && w.byte_at(ss, true) == '"';
   for (int32_t k = 0; match && k < klen; k += 8) {
     const int32_t rest = klen - k;
@@ -320,9 +315,9 @@ __device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v) {
 enum { kScanFound = 1, kScanFail = 2, kScanUnknown = 3, kScanExhausted = 4 };
 struct KeyScan {
   int kind;     // kScan*
-  int64_t pos;  // the deciding character, or the last one scanned
+  Pos pos;      // the deciding character, or the last one scanned
   int phase;    // role of the character after pos (0: key position)
-  int64_t cand; // the last key position scanned
+  Pos cand;     // the last key position scanned
 };
 
 // The key search over the row characters of level `lvl` after `pos` up to
@@ -333,16 +328,16 @@ struct KeyScan {
 // matches (found) or when the 16 bytes cannot decide it (unknown); a
 // separator position decides when it is '}' (fail).
 template <int MAXLV>
-__device__ __forceinline__ KeyScan coop_key_scan(Win<MAXLV> &w, int lvl, int64_t pos, int phase,
-                                                 int64_t cand, int64_t lend, int64_t line_start,
+__device__ __forceinline__ KeyScan coop_key_scan(Win<MAXLV> &w, int lvl, Pos pos, int phase,
+                                                 Pos cand, Pos lend, Pos line_start,
                                                  const KeyPattern &kp) {
   const int l = lane_id();
   for (;;) {
-    const int64_t start = pos + 1;
+    const Pos start = pos + 1;
     if (start > lend) return KeyScan{kScanExhausted, pos, phase, cand};
     w.need_word(start >> 6, false);
-    const int64_t wb = w.base >> 6;
-    const int64_t rel = (start >> 6) - wb, srel = (lend >> 6) - wb;
+    const Pos wb = w.base >> 6;
+    const Pos rel = (start >> 6) - wb, srel = (lend >> 6) - wb;
     uint64_t m = w.lds->rows[lvl][l];
     if (l < rel) m = 0;
     else if (l == rel) m &= ~0ull << (start & 63);
@@ -352,18 +347,18 @@ __device__ __forceinline__ KeyScan coop_key_scan(Win<MAXLV> &w, int lvl, int64_t
     const uint32_t pre = wave_excl_sum(cnt);
     const uint8_t *text = reinterpret_cast<const uint8_t *>(w.lds->text);
     int ev = 0;
-    int64_t evpos = 0;
+    Pos evpos = 0;
     uint32_t r = (uint32_t)phase + pre;
     uint64_t mm = m;
     while (mm) {
       const int b = (int)__builtin_ctzll(mm);
       mm &= mm - 1;
-      const int64_t p = (wb + l) * 64 + b;
-      const int64_t a = p - w.base; // 0 .. 4095
+      const Pos p = (wb + l) * 64 + b;
+      const Pos a = p - w.base; // 0 .. 4095
       if ((r & 1u) == 0) {
         if (a < 15) { ev = kScanUnknown; evpos = p; break; }
         const uint64_t *tw = w.lds->text;
-        const int64_t a0 = a - 15, q = a0 >> 3;
+        const Pos a0 = a - 15, q = a0 >> 3;
         const uint32_t sh = 8 * (uint32_t)(a0 & 7);
         const uint64_t w0 = tw[q], w1 = tw[q + 1], w2 = tw[q + 2];
         const uint64_t t0 = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
@@ -389,7 +384,7 @@ __device__ __forceinline__ KeyScan coop_key_scan(Win<MAXLV> &w, int lvl, int64_t
     if (evb) {
       const int f = (int)__builtin_ctzll(evb);
       const int kind = __builtin_amdgcn_readlane(ev, f);
-      const int64_t p = (int64_t)lane64((uint64_t)evpos, f);
+      const Pos p = __builtin_amdgcn_readlane(evpos, f);
       // (a deciding key position leaves phase 0 and its own position as the
       // last key position; only kScanUnknown continues, from there)
       return KeyScan{kind, p, 0, p};
@@ -398,7 +393,7 @@ __device__ __forceinline__ KeyScan coop_key_scan(Win<MAXLV> &w, int lvl, int64_t
     if (total) {
       const uint64_t nz = __ballot(m != 0);
       const int f = 63 - (int)__builtin_clzll(nz);
-      const int64_t last = (wb + f) * 64 + 63 - (int64_t)__builtin_clzll(lane64(m, f));
+      const Pos last = (wb + f) * 64 + 63 - (Pos)__builtin_clzll(lane64(m, f));
       if ((((uint32_t)phase + total - 1) & 1u) == 0) cand = last; // last was a key position
       phase = (int)(((uint32_t)phase + total) & 1u);
     }
@@ -414,7 +409,7 @@ __device__ __forceinline__ KeyScan coop_key_scan(Win<MAXLV> &w, int lvl, int64_t
 // in dense mode, the lanes of `stage_s` / `stage_e` (the first 64 spans).
 struct WaveSink {
   int64_t *res;
-  int64_t keep;
+  int32_t keep;
   bool dense;
 };
 
@@ -422,22 +417,22 @@ struct WaveSink {
 // run_query): levels and marks below maxlv, element iterations below
 // min(maxlv, kMaxFrames).  Returns the number of results (stored or counted).
 template <int MAXLV>
-__device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start, int64_t line_end,
+__device__ __forceinline__ int32_t wave_query(Win<MAXLV> &w, Pos line_start, Pos line_end,
                               const IrReader ir, int32_t cap, bool variable, int32_t maxlv,
-                              const WaveSink &out, LaneI64 &stage_s, LaneI64 &stage_e) {
+                              const WaveSink &out, LaneI32 &stage_s, LaneI32 &stage_e) {
   const int kf = maxlv < kMaxFrames ? maxlv : kMaxFrames;
-  int64_t idx = line_start;
+  Pos idx = line_start;
   int32_t lvl = 0, mi = -1, nfr = 0;
   uint32_t qp = 0;
-  int64_t count = 0;
-  LaneI64 level_end{-1};
+  int32_t count = 0;
+  LaneI32 level_end{-1};
   level_end.set(0, line_end);
-  LaneI64 marked_pos{0};
+  LaneI32 marked_pos{0};
   LaneI32 marked_lvl{0};
   LaneI32 curr_index{-1};
   LaneI32 fr_lvl{0}, fr_mi{0};
   LaneI32 fr_qp{0};
-  LaneI64 fr_sep{0};
+  LaneI32 fr_sep{0};
 
   for (;;) {
     bool fail = false;
@@ -445,14 +440,14 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
     switch (op) {
     case kStoreResult: {
       if (!variable && count >= cap) { fail = true; break; }
-      const int64_t lend = level_end.get(lvl);
-      int64_t end_str = lend;
+      const Pos lend = level_end.get(lvl);
+      Pos end_str = lend;
       while (idx < lend && w.byte_at(idx) == ' ') ++idx;
       while (end_str > idx && w.byte_at(end_str - 1, true) == ' ') --end_str;
       if (out.dense) {
         if (count < 64) {
-          stage_s.set((int)count, idx);
-          stage_e.set((int)count, end_str);
+          stage_s.set(count, idx);
+          stage_e.set(count, end_str);
         }
       } else if (count < out.keep && lane_id() == 0) {
         out.res[2 * count] = idx;
@@ -480,7 +475,7 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
       const uint32_t key_pos = qp;
       qp += (uint32_t)klen;
       if (klen < 0 || w.byte_at(idx) != '{') { fail = true; break; }
-      const int64_t lend = level_end.get(lvl);
+      const Pos lend = level_end.get(lvl);
       const KeyPattern kp = KeyPattern::make(ir, key_pos, klen);
       // the reference's search_key loop (query.cu:254-302) visits the row's
       // structural characters after the '{' in order, alternating a key
@@ -489,7 +484,7 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
       // characters window by window; a key it cannot decide from the 16 bytes
       // ending at its ':', and the level-end tail, go through the sequential
       // steps below.
-      int64_t pos = idx, cand = -1;
+      Pos pos = idx, cand = -1;
       int phase = 0; // 0: the next character is a key position, 1: a separator
       int state = 0; // 0: searching, 1: found at pos, 2: failed
       while (state == 0) {
@@ -531,7 +526,7 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
       const int32_t target = ir.varint(qp);
       const uint32_t ch = w.byte_at(idx);
       if (ch != '[' && ch != ',' && ch != ']') { fail = true; break; }
-      const int64_t lend = level_end.get(lvl);
+      const Pos lend = level_end.get(lvl);
       int32_t ci = ch == '[' ? 0 : curr_index.get(lvl) + 1;
       for (;;) {
         if (ci < target) {
@@ -554,7 +549,7 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
     }
     case kMoveToIndexReverse: {
       const int32_t target = ir.varint(qp);
-      const int64_t lend = level_end.get(lvl);
+      const Pos lend = level_end.get(lvl);
       idx = lend - 1;
       while (idx >= 0 && w.byte_at(idx, true) == ' ') --idx;
       const uint32_t ch = w.byte_at(idx);
@@ -596,11 +591,11 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
       const uint32_t key_pos = qp;
       qp += (uint32_t)klen;
       if (klen < 0) { fail = true; break; }
-      const int64_t lend = level_end.get(lvl);
+      const Pos lend = level_end.get(lvl);
       while (idx < lend && w.byte_at(idx) == ' ') ++idx;
-      int64_t string_end = lend - 1;
+      Pos string_end = lend - 1;
       while (idx < string_end && w.byte_at(string_end, true) == ' ') --string_end;
-      if (string_end - idx + 1 != (int64_t)klen) { fail = true; break; }
+      if (string_end - idx + 1 != klen) { fail = true; break; }
       for (int32_t k = 0; k < klen; k += 8) {
         const int32_t rest = klen - k;
         const uint64_t m = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
@@ -613,8 +608,8 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
     }
     case kMoveToEach: {
       if (!variable || nfr >= kf || w.byte_at(idx) != '[') { fail = true; break; }
-      const int64_t first = w.next_structural(level_end.get(lvl), idx + 1, lvl);
-      int64_t p = idx + 1;
+      const Pos first = w.next_structural(level_end.get(lvl), idx + 1, lvl);
+      Pos p = idx + 1;
       while (p < first && w.byte_at(p) == ' ') ++p;
       if (p == first && w.byte_at(first) == ']') { fail = true; break; } // empty array
       fr_lvl.set(nfr, lvl);
@@ -641,8 +636,8 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
     while (nfr > 0) {
       const int f = nfr - 1;
       const int flvl = fr_lvl.get(f);
-      const int64_t lend = level_end.get(flvl);
-      const int64_t sep = w.next_structural(lend, fr_sep.get(f) + 1, flvl);
+      const Pos lend = level_end.get(flvl);
+      const Pos sep = w.next_structural(lend, fr_sep.get(f) + 1, flvl);
       if (sep < lend && w.byte_at(sep) == ',') {
         fr_sep.set(f, sep);
         lvl = flvl;
@@ -661,7 +656,7 @@ __device__ __forceinline__ int64_t wave_query(Win<MAXLV> &w, int64_t line_start,
 
 // The record's bounds trimmed to its first '{' and last '}' (query.cu:160-172).
 template <int MAXLV>
-__device__ __forceinline__ bool wave_trim(Win<MAXLV> &w, int64_t &start, int64_t &end) {
+__device__ __forceinline__ bool wave_trim(Win<MAXLV> &w, Pos &start, Pos &end) {
   while (end > start && w.byte_at(end, true) != '}') --end;
   while (start < end && w.byte_at(start) != '{') ++start;
   return start != end;
@@ -687,11 +682,11 @@ __global__ __launch_bounds__(64) void query_wave_kernel(
   const int lane = lane_id();
   Win<MAXLV> w;
   w.file = file;
-  w.n = n;
-  w.words = (n + 63) >> 6;
+  w.n = (Pos)n; // (at most kNarrowSpan: the host checks)
+  w.words = (Pos)((n + 63) >> 6);
   w.str = string;
   w.lev = leveled;
-  w.level_size = level_size;
+  w.level_size = level_size < (int64_t)INT32_MAX ? (Pos)level_size : INT32_MAX;
   w.num_levels = num_levels;
   w.lbase = mode.level_base;
   {
@@ -701,12 +696,13 @@ __global__ __launch_bounds__(64) void query_wave_kernel(
   w.base = -1;
   w.lds = &lds;
   for (int64_t line = blockIdx.x; line < lines; line += gridDim.x) {
-    int64_t ls = newline[line];
-    int64_t le = (line + 1 < all) ? newline[line + 1] : n - 1;
-    bool record = ls >= 0 && ls < n && le >= 0;
-    if (record && le >= n) le = n - 1;
+    const int64_t ls64 = newline[line];
+    int64_t le64 = (line + 1 < all) ? newline[line + 1] : n - 1;
+    bool record = ls64 >= 0 && ls64 < n && le64 >= 0;
+    if (record && le64 >= n) le64 = n - 1;
+    Pos ls = record ? (Pos)ls64 : 0, le = record ? (Pos)le64 : 0;
     if (record) {
-      int64_t b = (ls & ~63ll) - 64;
+      const Pos b = (ls & ~63) - 64;
       w.stage(b < 0 ? 0 : b);
       record = wave_trim(w, ls, le);
     }
@@ -721,7 +717,7 @@ __global__ __launch_bounds__(64) void query_wave_kernel(
         if (!qs.variable && lane == 0)
           for (int i = 0; i < 2 * qs.cap; ++i) out.res[i] = -1;
       }
-      LaneI64 stage_s{0}, stage_e{0};
+      LaneI32 stage_s{0}, stage_e{0};
       int64_t found = 0;
       if (record)
         found = wave_query<MAXLV>(w, ls, le, IrReader::make(qs.ir, qs.ir_len), qs.cap,
@@ -741,8 +737,8 @@ __global__ __launch_bounds__(64) void query_wave_kernel(
             at = (int64_t)region * batch.ds.region + (int64_t)got;
             staged = true;
             if (lane < found) {
-              batch.ds.tmp[2 * (at + lane)] = stage_s.v;
-              batch.ds.tmp[2 * (at + lane) + 1] = stage_e.v;
+              batch.ds.tmp[2 * (at + lane)] = (int64_t)stage_s.v;
+              batch.ds.tmp[2 * (at + lane) + 1] = (int64_t)stage_e.v;
             }
           }
         }

@@ -239,7 +239,8 @@ gpj_status gpj_debug_set_narrow_span(int64_t bytes);
  * most GPJ_WAVE_WALK_RECORDS, or at most GPJ_WAVE_WALK_LONG_RECORDS with at
  * least GPJ_WAVE_WALK_LONG_BYTES input bytes per record (n >= that * R), and
  * the lane walk otherwise (measured crossovers on MI355X).  Inputs whose
- * device address is not 16-byte aligned always take the lane walk.  Results
+ * device address is not 16-byte aligned, and inputs above 2^31 - 2^20 bytes,
+ * always take the lane walk.  Results
  * never depend on the walk.  Process-wide; set it while no query runs. */
 #define GPJ_WALK_AUTO 0
 #define GPJ_WALK_LANE 1
