@@ -162,7 +162,9 @@ typedef struct {
   uint64_t newlines;
 } gpj_shard_summary;
 /* After a gpj_index_build_into or gpj_index_build_shard of n bytes with
- * d_scratch: writes that input's summary to device memory.  Stream-ordered. */
+ * d_scratch (n must be that build's size; check the build's status first,
+ * gpj_index_status): writes that input's summary to device memory.
+ * Stream-ordered. */
 gpj_status gpj_index_shard_summary(const void *d_scratch, uint64_t n,
                                    gpj_shard_summary *d_summary, void *stream);
 /* Host: carries[k] = the state before shard k (carries[0] = the document's
