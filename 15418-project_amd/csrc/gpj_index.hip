@@ -118,6 +118,11 @@ __device__ __forceinline__ void load_word(const uint8_t *in, uint64_t n,
   }
 }
 
+// ASCII: every byte of the word is below 0x80, so the compares need neither
+// the high-bit mask on their inputs nor the per-class clear of bytes >= 0x80
+// (the classify() of a wave whose words are all ASCII; bytes past the input
+// are 0).
+template <bool ASCII>
 __device__ __forceinline__ WordClasses classify(const uint32_t (&x)[16]) {
   // class constants 0x7F ^ c in every byte ('[' and ']' fold onto '{' and '}'
   // once bit 5 is set)
@@ -137,8 +142,8 @@ __device__ __forceinline__ WordClasses classify(const uint32_t (&x)[16]) {
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const uint32_t v = x[2 * j + e];
-      const uint32_t nh = ~v & 0x80808080u;
-      const uint32_t lo = v & 0x7F7F7F7Fu;
+      const uint32_t nh = ASCII ? 0x80808080u : (~v & 0x80808080u);
+      const uint32_t lo = ASCII ? v : (v & 0x7F7F7F7Fu);
       const uint32_t lo20 = lo | 0x20202020u;
       f[0][e] = xad(lo, kq, ones) & nh;
       f[1][e] = xad(lo, kb, ones) & nh;
@@ -232,6 +237,13 @@ __device__ __forceinline__ uint64_t tbl_unpack(uint64_t d) {
   const uint32_t a2 = (uint32_t)d & 0xFFu;
   const uint32_t a = (a2 | (a2 << 6) | (a2 << 12) | (a2 << 18)) & 0x03030303u;
   return (uint64_t)a | (((d >> 8) & 0xFFFFFFFFull) << 32);
+}
+// Packs an unpacked table into the 40-bit table field of an aggregate
+// descriptor (the inverse of tbl_unpack).
+__device__ __forceinline__ uint64_t tbl_pack(uint64_t u) {
+  const uint32_t a = (uint32_t)u;
+  const uint32_t st = (a & 3u) | ((a >> 6) & 0xCu) | ((a >> 12) & 0x30u) | ((a >> 18) & 0xC0u);
+  return (uint64_t)st | ((u >> 32) << 8);
 }
 // (f o g)(s) = f(g(s)): apply g first.  v_perm_b32 picks, for each byte s,
 // byte g(s) of f's states and of f's deltas; the deltas add bytewise mod 256.
@@ -421,38 +433,46 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
   }
 }
 
-// The fused single-pass index kernel.  One workgroup per 16 KiB tile, plus
-// the scanner workgroup (block 0) for a single input.  SEG: multi-file batch
-// (see BatchSeg); a tile resolves its carry with a decoupled look-back that
-// never reads before its file's first tile, whose carry-in is the initial
-// state.
-// Single input, up to 4 levels: 8 waves per SIMD (64 VGPRs; only the
-// byte-wise load path of a partial word spills).  The carry takes ~9 us per
-// tile to arrive, so the tiles in flight, not the work per tile, set the
-// throughput: 8 waves run the 8 GiB input in 4.49 ms against 5.14 ms at 6.
+// The fused single-pass index kernel, plus the scanner workgroup (block 0)
+// for a single input.  A workgroup of 256 lanes takes kTileWords words (16
+// KiB) per half-tile, one word per lane.  Single input: a tile is kTileHalves
+// half-tiles (32 KiB; lane t holds word t of each half); the halves'
+// carry-free work (classification, escapes, in-half string parity, depth and
+// newline sums) runs one half after the other, the tile publishes the
+// composition of the halves' transfer tables and waits once for its
+// carry-in, and the second half starts from the first half's carry-out.
+// The carry takes ~10 us to arrive (a tile's publish, the scanner's read and
+// write and the tile's poll are device-scope memory round trips under load),
+// so the bytes a wave slot holds across that wait, not the work per byte,
+// set the throughput: 16 KiB tiles at 8 waves per SIMD ran the 8 GiB input
+// in 4.3 ms.  SEG: multi-file batch (see BatchSeg): one half per tile (files
+// start on 16 KiB boundaries), whose carry comes from a decoupled look-back
+// that never reads before its file's first tile; that tile starts from the
+// initial state.
 template <int LMAX, bool SEG>
-__global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SEG && LMAX <= 4) ? 8 : 6))) void index_kernel(
+__global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG ? 6 : ((LMAX <= 4) ? 6 : 4)))) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
     int aligned, uint64_t *__restrict__ string_out,
     uint64_t *__restrict__ leveled_out, int64_t *__restrict__ newline_out,
     int64_t newline_cap, int64_t *__restrict__ num_lines_out,
     uint64_t *__restrict__ desc, uint32_t *__restrict__ tile_counter,
     uint64_t num_tiles, BatchSeg seg, uint32_t max_spins, uint64_t init) {
+  constexpr int NH = SEG ? 1 : kTileHalves; // half-tiles per tile
   uint32_t *const error_flag = tile_counter + 1;
   if (SEG) init = kInitialInclusive; // every file of a batch starts fresh
   __shared__ uint32_t s_tile;
-  __shared__ uint8_t s_bsinfo[kTileWords];  // bit0 trailing parity, bit1 all-backslash
-  __shared__ uint8_t s_lead[kTileWords];    // leading backslash run, bit7 = quote at its end
-  __shared__ uint8_t s_cin[kTileWords];     // escape carry-in per word (tile carry-in 0)
-  __shared__ uint32_t s_wany[kTileWords / 64]; // per wave: a word of only backslashes
-  __shared__ uint32_t s_wpar[kTileWords / 64]; // per wave: quote parity
-  __shared__ uint32_t s_wsum[kTileWords / 64]; // per wave: sum of (d0a+64) | (tot+64) << 16
-  __shared__ uint32_t s_wnl[kTileWords / 64];  // per wave: newlines
-  __shared__ uint32_t s_wscan[kTileWords / 64];
-  __shared__ uint32_t s_first;              // k | lead_k << 16 | g << 24
+  __shared__ uint8_t s_bsinfo[NH][kTileWords];  // bit0 trailing parity, bit1 all-backslash
+  __shared__ uint8_t s_lead[NH][kTileWords];    // leading backslash run, bit7 = quote at its end
+  __shared__ uint8_t s_cin[NH][kTileWords];     // escape carry-in per word (half carry-in 0)
+  __shared__ uint32_t s_wany[NH][kTileWords / 64]; // per wave: a word of only backslashes
+  __shared__ uint32_t s_wpar[NH][kTileWords / 64]; // per wave: quote parity
+  __shared__ uint32_t s_wsum[NH][kTileWords / 64]; // per wave: sum of (d0a+64) | (tot+64) << 16
+  __shared__ uint32_t s_wnl[NH][kTileWords / 64];  // per wave: newlines
+  __shared__ uint32_t s_wscan[NH][kTileWords / 64];
+  __shared__ uint32_t s_first[NH];              // k | lead_k << 16 | g << 24
   __shared__ uint32_t s_rows[LMAX + 1][2][kTileWords]; // per-lane leveled half-words (+ sink)
-  __shared__ uint32_t s_tail_esc;           // tile escape transfer: 0, 1 constant, 2 identity, 3 flip
-  __shared__ Carry s_carry;
+  __shared__ uint32_t s_tail_esc[NH];           // half escape transfer: 0, 1 constant, 2 identity, 3 flip
+  __shared__ Carry s_carry[NH];                 // carry-in of each half
 
   const uint32_t t = threadIdx.x;
   const uint32_t lane = t & 63u;
@@ -476,8 +496,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
   // tile and word base are wave-uniform (scalar registers); per-lane offsets
   // are formed from them where needed rather than kept alive in VGPRs
   const uint64_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_tile);
-  const uint64_t wbase = tile * kTileWords;
-  const uint64_t pos = (wbase + t) * 64;
+  const uint64_t wbase = tile * (uint64_t)(kTileWords * NH);
   // the file this tile belongs to (the whole input when !SEG)
   uint64_t first_tile = 0, last_tile = num_tiles - 1, file_base = 0, file_end = n;
   int64_t *nl_out = newline_out;
@@ -494,143 +513,200 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
     lines_out = seg.line_counts + seg.tile_file[tile];
   }
 
-  // 1. classify this lane's 64 bytes
-  WordClasses m = {0, 0, 0, 0, 0, 0};
-  if (pos < file_end) {
-    uint32_t x[16];
-    load_word(in, file_end, pos, aligned != 0, x);
-    m = classify(x);
-  }
+  // What each half keeps from its carry-free work until the carry arrives:
+  // per lane the word's openers, closers, separators, newlines and string
+  // mask for string carry-in 0 at the half's start, and its in-wave scans;
+  // per wave (scalar registers) the half's transfer-table inputs and the
+  // sums over the waves before this one.
+  uint64_t kO[NH], kC[NH], kP[NH], kN[NH], kS0[NH];
+  uint32_t kDincl[NH], kDv[NH], kNincl[NH], kNlc[NH];
+  uint32_t uFirst[NH], uTail[NH], uPar[NH], uNL[NH], uNlBefore[NH], uMyPar[NH];
+  int32_t uD0[NH], uTOT[NH], uDBefore[NH], uTotBefore[NH];
 
-  // 2. escapes, assuming the tile starts with escape carry 0.  Each word's
-  //    escape transfer: the parity of the backslash run ending at its last
-  //    byte (a constant), or, for a word of only backslashes, its carry-in
-  //    passed through (64 of them) or flipped (an odd number at the input's
-  //    end).  Words past a single input's end pass the carry through, so the
-  //    last tile's escape-out is the input's (a shard's successor needs it).
-  const uint64_t B = m.backslash;
-  const bool real = pos < file_end;
-  const uint32_t valid = real ? (uint32_t)min((uint64_t)64, file_end - pos) : 0u;
-  const uint64_t e0 = escaped_from_zero(B);
-  const int lead = (!SEG && !real) ? 64 : ((B == ~0ull) ? 64 : __builtin_ctzll(~B));
-  uint32_t mode; // 0, 1: escape-out constant; 2: carry passes; 3: carry flips
-  if (!real) {
-    mode = SEG ? 0u : 2u;
-  } else {
-    const uint64_t bv = valid == 64 ? B : (B << (64 - valid)); // last byte at bit 63
-    const uint32_t run = (~bv == 0ull) ? 64u : (uint32_t)__builtin_clzll(~bv);
-    mode = run >= valid ? (2u | (valid & 1u)) : (run & 1u);
-  }
-  const bool all_bs = mode >= 2u;
-  s_bsinfo[t] = (uint8_t)mode;
-  s_lead[t] = (uint8_t)(lead | ((lead < 64 && ((m.quote >> lead) & 1ull)) ? 0x80 : 0));
-  const uint64_t any_ballot = __ballot(all_bs);
-  if (lane == 0) s_wany[wv] = any_ballot != 0;
-  lds_barrier();
-  uint32_t any_all_bs = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < kTileWords / 64; ++k) any_all_bs |= s_wany[k];
-  uint32_t cin, tail_esc, first;
-  if (any_all_bs) {
-    // rare: a word of only backslashes (or past the input's end) passes its
-    // carry through or flips it; one serial pass over the tile's words, from
-    // carry-in 0 and 1, gives every word's carry-in for 0 and the tile's
-    // transfer (constant, 2 = identity, 3 = flip)
-    if (t == 0) {
-      uint32_t c = 0, c1 = 1;
-      for (uint32_t k = 0; k < kTileWords; ++k) {
-        s_cin[k] = (uint8_t)c;
-        const uint32_t md = s_bsinfo[k];
-        c = md == 2u ? c : (md == 3u ? c ^ 1u : md);
-        c1 = md == 2u ? c1 : (md == 3u ? c1 ^ 1u : md);
-      }
-      s_tail_esc = c == c1 ? c : (2u | c);
-      uint32_t k = 0;
-      while (k < kTileWords && (s_lead[k] & 0x7F) >= 64) ++k;
-      if (k == kTileWords) {
-        s_first = (uint32_t)kTileWords;
-      } else {
-        s_first = k | ((uint32_t)(s_lead[k] & 0x7F) << 16) |
-                  ((uint32_t)(s_lead[k] >> 7) << 24);
-      }
+  for (int h = 0; h < NH; ++h) {
+    const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the half's first word
+    const uint64_t pos = (hb + t) * 64;
+
+    // 1. classify this lane's 64 bytes
+    WordClasses m = {0, 0, 0, 0, 0, 0};
+    if (pos < file_end) {
+      uint32_t x[16];
+      load_word(in, file_end, pos, aligned != 0, x);
+      // a wave whose words hold no byte >= 0x80 (wave-uniform) classifies
+      // without the high-bit masks
+      uint32_t high = 0;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) high |= x[i];
+      if (__ballot((high & 0x80808080u) != 0) == 0)
+        m = classify<true>(x);
+      else
+        m = classify<false>(x);
     }
+
+    // 2. escapes, assuming the half starts with escape carry 0.  Each word's
+    //    escape transfer: the parity of the backslash run ending at its last
+    //    byte (a constant), or, for a word of only backslashes, its carry-in
+    //    passed through (64 of them) or flipped (an odd number at the
+    //    input's end).  Words past a single input's end pass the carry
+    //    through, so the last tile's escape-out is the input's (a shard's
+    //    successor needs it).
+    const uint64_t B = m.backslash;
+    const bool real = pos < file_end;
+    const uint32_t valid = real ? (uint32_t)min((uint64_t)64, file_end - pos) : 0u;
+    const uint64_t e0 = escaped_from_zero(B);
+    const int lead = (!SEG && !real) ? 64 : ((B == ~0ull) ? 64 : __builtin_ctzll(~B));
+    uint32_t mode; // 0, 1: escape-out constant; 2: carry passes; 3: carry flips
+    if (!real) {
+      mode = SEG ? 0u : 2u;
+    } else {
+      const uint64_t bv = valid == 64 ? B : (B << (64 - valid)); // last byte at bit 63
+      const uint32_t run = (~bv == 0ull) ? 64u : (uint32_t)__builtin_clzll(~bv);
+      mode = run >= valid ? (2u | (valid & 1u)) : (run & 1u);
+    }
+    const bool all_bs = mode >= 2u;
+    s_bsinfo[h][t] = (uint8_t)mode;
+    s_lead[h][t] = (uint8_t)(lead | ((lead < 64 && ((m.quote >> lead) & 1ull)) ? 0x80 : 0));
+    const uint64_t any_ballot = __ballot(all_bs);
+    if (lane == 0) s_wany[h][wv] = any_ballot != 0;
     lds_barrier();
-    cin = s_cin[t];
-    tail_esc = s_tail_esc;
-    first = s_first;
-  } else {
-    cin = t ? (s_bsinfo[t - 1] & 1u) : 0u;
-    tail_esc = s_bsinfo[kTileWords - 1] & 1u;
-    first = ((uint32_t)(s_lead[0] & 0x7F) << 16) | ((uint32_t)(s_lead[0] >> 7) << 24);
-  }
-  const uint64_t escaped = cin ? (e0 ^ mask_upto(lead)) : e0;
-
-  // 3. string mask for string carry-in 0 at this wave's first word (the
-  //    parity the tile's earlier waves leave is applied below)
-  const uint64_t q = m.quote & ~escaped;
-  const uint64_t px = prefix_xor64(q);
-  const uint64_t par_ballot = __ballot((uint32_t)(px >> 63));
-  const uint64_t s0a = px ^ ((count_below(par_ballot) & 1u) ? ~0ull : 0ull);
-
-  // 4. depth delta of this word under s0a and its total, newline count; the
-  //    in-wave inclusive scans serve both the tile aggregate and, once the
-  //    carry-in is known, this word's depth and newline offsets
-  const int d0a = __popcll(m.open & ~s0a) - __popcll(m.close & ~s0a);
-  const int tot = __popcll(m.open) - __popcll(m.close);
-  const uint32_t nlc = (uint32_t)__popcll(m.newline);
-  // 16-bit fields: in-wave sums stay below 64 * 129
-  const uint32_t dv = (uint32_t)(d0a + 64) | ((uint32_t)(tot + 64) << 16);
-  const uint32_t dincl = wave_incl_scan(dv);
-  const uint32_t nincl = wave_incl_scan(nlc);
-  if (lane == 63) {
-    s_wsum[wv] = dincl;
-    s_wnl[wv] = nincl;
-  }
-  if (lane == 0) s_wpar[wv] = (uint32_t)__popcll(par_ballot) & 1u;
-  lds_barrier();
-
-  // tile sums for string carry-in 0 (wave k's words see the parity of the
-  // waves before it), and the same sums over the waves before this one
-  uint32_t run_par = 0, my_par = 0, NL = 0, nl_before = 0;
-  int D0 = 0, TOT = 0, d_before = 0, tot_before = 0;
+    uint32_t any_all_bs = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < kTileWords / 64; ++k) {
-    const uint32_t ws = s_wsum[k];
-    const int d0k = (int)(ws & 0xFFFFu) - 64 * 64;
-    const int totk = (int)(ws >> 16) - 64 * 64;
-    const int dk = run_par ? totk - d0k : d0k;
-    if (k < wv) {
-      d_before += dk;
-      tot_before += totk;
-      nl_before += s_wnl[k];
+    for (uint32_t k = 0; k < kTileWords / 64; ++k) any_all_bs |= s_wany[h][k];
+    uint32_t cin;
+    if (any_all_bs) {
+      // rare: a word of only backslashes (or past the input's end) passes
+      // its carry through or flips it; one serial pass over the half's
+      // words, from carry-in 0 and 1, gives every word's carry-in for 0 and
+      // the half's transfer (constant, 2 = identity, 3 = flip)
+      if (t == 0) {
+        uint32_t c = 0, c1 = 1;
+        for (uint32_t k = 0; k < kTileWords; ++k) {
+          s_cin[h][k] = (uint8_t)c;
+          const uint32_t md = s_bsinfo[h][k];
+          c = md == 2u ? c : (md == 3u ? c ^ 1u : md);
+          c1 = md == 2u ? c1 : (md == 3u ? c1 ^ 1u : md);
+        }
+        s_tail_esc[h] = c == c1 ? c : (2u | c);
+        uint32_t k = 0;
+        while (k < kTileWords && (s_lead[h][k] & 0x7F) >= 64) ++k;
+        if (k == kTileWords) {
+          s_first[h] = (uint32_t)kTileWords;
+        } else {
+          s_first[h] = k | ((uint32_t)(s_lead[h][k] & 0x7F) << 16) |
+                       ((uint32_t)(s_lead[h][k] >> 7) << 24);
+        }
+      }
+      lds_barrier();
+      cin = s_cin[h][t];
+      uTail[h] = __builtin_amdgcn_readfirstlane(s_tail_esc[h]);
+      uFirst[h] = __builtin_amdgcn_readfirstlane(s_first[h]);
+    } else {
+      cin = t ? (s_bsinfo[h][t - 1] & 1u) : 0u;
+      uTail[h] = __builtin_amdgcn_readfirstlane(s_bsinfo[h][kTileWords - 1] & 1u);
+      const uint32_t l0 = s_lead[h][0];
+      uFirst[h] = __builtin_amdgcn_readfirstlane(((l0 & 0x7Fu) << 16) | ((l0 >> 7) << 24));
     }
-    if (k == wv) my_par = run_par;
-    D0 += dk;
-    TOT += totk;
-    NL += s_wnl[k];
-    run_par ^= s_wpar[k];
+    const uint64_t escaped = cin ? (e0 ^ mask_upto(lead)) : e0;
+
+    // 3. string mask for string carry-in 0 at this wave's first word (the
+    //    parity the half's earlier waves leave is applied below)
+    const uint64_t q = m.quote & ~escaped;
+    const uint64_t px = prefix_xor64(q);
+    const uint64_t par_ballot = __ballot((uint32_t)(px >> 63));
+    const uint64_t s0a = px ^ ((count_below(par_ballot) & 1u) ? ~0ull : 0ull);
+
+    // 4. depth delta of this word under s0a and its total, newline count;
+    //    the in-wave inclusive scans serve both the half's aggregate and,
+    //    once the carry-in is known, this word's depth and newline offsets
+    const int d0a = __popcll(m.open & ~s0a) - __popcll(m.close & ~s0a);
+    const int tot = __popcll(m.open) - __popcll(m.close);
+    const uint32_t nlc = (uint32_t)__popcll(m.newline);
+    // 16-bit fields: in-wave sums stay below 64 * 129
+    const uint32_t dv = (uint32_t)(d0a + 64) | ((uint32_t)(tot + 64) << 16);
+    const uint32_t dincl = wave_incl_scan(dv);
+    const uint32_t nincl = wave_incl_scan(nlc);
+    if (lane == 63) {
+      s_wsum[h][wv] = dincl;
+      s_wnl[h][wv] = nincl;
+    }
+    if (lane == 0) s_wpar[h][wv] = (uint32_t)__popcll(par_ballot) & 1u;
+    lds_barrier();
+
+    // half sums for string carry-in 0 (wave k's words see the parity of the
+    // waves before it), and the same sums over the waves before this one
+    // (wave-uniform: scalar registers)
+    uint32_t run_par = 0, my_par = 0, NL = 0, nl_before = 0;
+    int D0 = 0, TOT = 0, d_before = 0, tot_before = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kTileWords / 64; ++k) {
+      const uint32_t ws = __builtin_amdgcn_readfirstlane(s_wsum[h][k]);
+      const uint32_t wn = __builtin_amdgcn_readfirstlane(s_wnl[h][k]);
+      const int d0k = (int)(ws & 0xFFFFu) - 64 * 64;
+      const int totk = (int)(ws >> 16) - 64 * 64;
+      const int dk = run_par ? totk - d0k : d0k;
+      if (k < wv) {
+        d_before += dk;
+        tot_before += totk;
+        nl_before += wn;
+      }
+      if (k == wv) my_par = run_par;
+      D0 += dk;
+      TOT += totk;
+      NL += wn;
+      run_par ^= __builtin_amdgcn_readfirstlane(s_wpar[h][k]);
+    }
+    kO[h] = m.open;
+    kC[h] = m.close;
+    kP[h] = m.sep;
+    kN[h] = m.newline;
+    kS0[h] = s0a ^ (my_par ? ~0ull : 0ull);
+    kDincl[h] = dincl;
+    kDv[h] = dv;
+    kNincl[h] = nincl;
+    kNlc[h] = nlc;
+    uPar[h] = run_par;
+    uNL[h] = NL;
+    uNlBefore[h] = nl_before;
+    uMyPar[h] = my_par;
+    uD0[h] = D0;
+    uTOT[h] = TOT;
+    uDBefore[h] = d_before;
+    uTotBefore[h] = tot_before;
   }
-  const uint32_t tile_par = run_par;
-  const uint64_t s0 = s0a ^ (my_par ? ~0ull : 0ull);
 
   // 5. publish the tile aggregate and resolve the exclusive prefix (wave 0)
   if (wv == 0) {
-    const uint32_t g = (first >> 24) & 1u;
-    const uint32_t tail = tail_esc;
-    const uint32_t D0b = (uint32_t)D0 & 0xFFu, TOTb = (uint32_t)TOT & 0xFFu;
-    uint64_t tbl = 0;
+    // each half's transfer table (packed: output states in bits 0..7,
+    // depth deltas from bit 8), the tile's as their composition
+    uint64_t htbl[NH];
 #pragma unroll
-    for (uint32_t X = 0; X < 2; ++X) {
+    for (int h = 0; h < NH; ++h) {
+      const uint32_t g = (uFirst[h] >> 24) & 1u;
+      const uint32_t tail = uTail[h];
+      const uint32_t D0b = (uint32_t)uD0[h] & 0xFFu, TOTb = (uint32_t)uTOT[h] & 0xFFu;
+      uint64_t tb = 0;
 #pragma unroll
-      for (uint32_t Y = 0; Y < 2; ++Y) {
-        const uint32_t eff = Y ^ (X & g);
-        const uint32_t esc_out = (tail & 2u) ? (X ^ (tail & 1u)) : (tail & 1u);
-        const uint32_t str_out = Y ^ tile_par ^ (X & g);
-        const uint32_t dd = eff ? ((TOTb - D0b) & 0xFFu) : D0b;
-        const uint32_t in = X | (Y << 1); // input state
-        tbl |= (uint64_t)(esc_out | (str_out << 1)) << (2 * in);
-        tbl |= (uint64_t)dd << (8 + 8 * in);
+      for (uint32_t X = 0; X < 2; ++X) {
+#pragma unroll
+        for (uint32_t Y = 0; Y < 2; ++Y) {
+          const uint32_t eff = Y ^ (X & g);
+          const uint32_t esc_out = (tail & 2u) ? (X ^ (tail & 1u)) : (tail & 1u);
+          const uint32_t str_out = Y ^ uPar[h] ^ (X & g);
+          const uint32_t dd = eff ? ((TOTb - D0b) & 0xFFu) : D0b;
+          const uint32_t in = X | (Y << 1); // input state
+          tb |= (uint64_t)(esc_out | (str_out << 1)) << (2 * in);
+          tb |= (uint64_t)dd << (8 + 8 * in);
+        }
       }
+      htbl[h] = tb;
+    }
+    uint64_t tbl = htbl[0];
+    uint64_t NL = uNL[0];
+#pragma unroll
+    for (int h = 1; h < NH; ++h) {
+      tbl = tbl_pack(tbl_compose(tbl_unpack(htbl[h]), tbl_unpack(tbl)));
+      NL += uNL[h];
     }
     Carry excl = decode_inclusive(init); // the first tile's carry-in
     if (!SEG && lane == 0) // the aggregate, for the scanner
@@ -747,105 +823,117 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((!SE
       const Carry incl = tbl_apply(tbl_unpack(tbl), NL, excl);
       if (SEG) desc_store(&desc[tile], encode_inclusive(incl));
       if (tile == last_tile) *lines_out = (int64_t)(1 + incl.nl);
-      s_carry = excl;
+      Carry ch = excl;
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        s_carry[h] = ch;
+        if (h + 1 < NH) ch = tbl_apply(tbl_unpack(htbl[h]), uNL[h], ch);
+      }
     }
   }
   lds_barrier();
-  const Carry c = s_carry;
 
-  // 6. final string mask: string carry-in, plus the flip of everything from
-  //    the tile's first non-backslash byte on when an escape carry-in of 1
-  //    un-counts a quote there
-  const uint32_t k_first = first & 0xFFFFu;
-  const uint32_t lead_k = (first >> 16) & 0xFFu;
-  const uint32_t g = (first >> 24) & 1u;
-  const bool esc_flip = (c.esc & g) != 0; // uniform
-  uint64_t flip = c.str ? ~0ull : 0ull;
-  if (esc_flip) {
-    if (t > k_first) flip ^= ~0ull;
-    else if (t == k_first) flip ^= ~0ull << lead_k;
-  }
-  const uint64_t S = s0 ^ flip;
-  const uint64_t O = m.open & ~S;
-  const uint64_t C = m.close & ~S;
-  const uint64_t P = m.sep & ~S;
-
-  // 7. depth before this word and its newline offset
-  const uint64_t nl_base = 1 + c.nl + nl_before + (nincl - nlc);
-  int depth_sum;
-  if (!esc_flip) {
-    // every word keeps its polarity relative to s0a: the words of this wave
-    // flip iff my_par ^ c.str, the earlier waves' sums flip iff c.str
-    const uint32_t dex = dincl - dv;
-    const int d0_ex = (int)(dex & 0xFFFFu) - 64 * (int)lane;
-    const int tot_ex = (int)(dex >> 16) - 64 * (int)lane;
-    depth_sum = (c.str ? tot_before - d_before : d_before) +
-                ((my_par ^ c.str) ? tot_ex - d0_ex : d0_ex);
-  } else {
-    // rare: scan the final per-word deltas
-    const int d = __popcll(O) - __popcll(C);
-    const uint32_t v = (uint32_t)(d + 64);
-    const uint32_t incl_v = wave_incl_scan(v);
-    if (lane == 63) s_wscan[wv] = incl_v;
-    lds_barrier();
-    uint32_t excl_v = incl_v - v;
-    for (uint32_t k = 0; k < wv; ++k) excl_v += s_wscan[k];
-    depth_sum = (int)excl_v - 64 * (int)t;
-  }
-  const int depth_in = (int)(int8_t)(uint8_t)((uint32_t)c.depth + (uint32_t)depth_sum);
-
-  // 8. leveled rows: walk this word's depth changes in order, one 32-bit half
-  //    at a time; every structural bit between two changes belongs to the
-  //    current depth (an opener's own bit to the depth it opens, a closer's
-  //    to the depth it closes).  Each segment is ORed into this lane's LDS
-  //    slot of its row; row LMAX is a sink for depths outside [0, LMAX).
-  //    Depth needs no 8-bit wrap inside the word: from an entry depth in
-  //    [-128, 127], 64 changes cannot wrap into [0, LMAX).
-  const bool live = wbase + t < words;
-  if (live) {
 #pragma unroll
-    for (int k = 0; k < LMAX; ++k) {
-      s_rows[k][0][t] = 0;
-      s_rows[k][1][t] = 0;
+  for (int h = 0; h < NH; ++h) {
+    const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the half's first word
+    const Carry c = s_carry[h];
+
+    // 6. final string mask: string carry-in, plus the flip of everything from
+    //    the half's first non-backslash byte on when an escape carry-in of 1
+    //    un-counts a quote there
+    const uint32_t first = uFirst[h];
+    const uint32_t k_first = first & 0xFFFFu;
+    const uint32_t lead_k = (first >> 16) & 0xFFu;
+    const uint32_t g = (first >> 24) & 1u;
+    const bool esc_flip = (c.esc & g) != 0; // uniform
+    uint64_t flip = c.str ? ~0ull : 0ull;
+    if (esc_flip) {
+      if (t > k_first) flip ^= ~0ull;
+      else if (t == k_first) flip ^= ~0ull << lead_k;
     }
-    const uint64_t E = O | C | P;
-    int depth = depth_in;
+    const uint64_t S = kS0[h] ^ flip;
+    const uint64_t O = kO[h] & ~S;
+    const uint64_t C = kC[h] & ~S;
+    const uint64_t P = kP[h] & ~S;
+
+    // 7. depth before this word and its newline offset
+    const uint64_t nl_base = 1 + c.nl + uNlBefore[h] + (kNincl[h] - kNlc[h]);
+    int depth_sum;
+    if (!esc_flip) {
+      // every word keeps its polarity relative to s0a: the words of this
+      // wave flip iff my_par ^ c.str, the earlier waves' sums flip iff c.str
+      const uint32_t dex = kDincl[h] - kDv[h];
+      const int d0_ex = (int)(dex & 0xFFFFu) - 64 * (int)lane;
+      const int tot_ex = (int)(dex >> 16) - 64 * (int)lane;
+      depth_sum = (c.str ? uTotBefore[h] - uDBefore[h] : uDBefore[h]) +
+                  ((uMyPar[h] ^ c.str) ? tot_ex - d0_ex : d0_ex);
+    } else {
+      // rare: scan the final per-word deltas
+      const int d = __popcll(O) - __popcll(C);
+      const uint32_t v = (uint32_t)(d + 64);
+      const uint32_t incl_v = wave_incl_scan(v);
+      if (lane == 63) s_wscan[h][wv] = incl_v;
+      lds_barrier();
+      uint32_t excl_v = incl_v - v;
+      for (uint32_t k = 0; k < wv; ++k) excl_v += s_wscan[h][k];
+      depth_sum = (int)excl_v - 64 * (int)t;
+    }
+    const int depth_in = (int)(int8_t)(uint8_t)((uint32_t)c.depth + (uint32_t)depth_sum);
+
+    // 8. leveled rows: walk this word's depth changes in order, one 32-bit
+    //    half at a time; every structural bit between two changes belongs to
+    //    the current depth (an opener's own bit to the depth it opens, a
+    //    closer's to the depth it closes).  Each segment is ORed into this
+    //    lane's LDS slot of its row; row LMAX is a sink for depths outside
+    //    [0, LMAX).  Depth needs no 8-bit wrap inside the word: from an entry
+    //    depth in [-128, 127], 64 changes cannot wrap into [0, LMAX).  (A
+    //    lane's slots are its own, so the halves reuse them in turn.)
+    const bool live = hb + t < words;
+    if (live) {
 #pragma unroll
-    for (uint32_t h = 0; h < 2; ++h) {
-      const uint32_t Eh = (uint32_t)(E >> (32 * h));
-      const uint32_t Oh = (uint32_t)(O >> (32 * h));
-      uint32_t x = Oh | (uint32_t)(C >> (32 * h));
-      uint32_t done = 0;
-      uint32_t *const slot = &s_rows[0][h][t];
-      while (x) {
-        const uint32_t low = x & (0u - x);
-        x ^= low;
-        const uint32_t upto = (low - 1u) | (low & ~Oh);
-        const uint32_t seg = upto & ~done & Eh;
-        done = upto;
-        const uint32_t row = min((uint32_t)depth, (uint32_t)LMAX);
-        atomicOr(slot + row * (2 * kTileWords), seg);
-        depth += (low & Oh) ? 1 : -1;
+      for (int k = 0; k < LMAX; ++k) {
+        s_rows[k][0][t] = 0;
+        s_rows[k][1][t] = 0;
       }
-      const uint32_t row = min((uint32_t)depth, (uint32_t)LMAX);
-      atomicOr(slot + row * (2 * kTileWords), Eh & ~done);
-    }
-    (string_out + wbase)[t] = S;
+      const uint64_t E = O | C | P;
+      int depth = depth_in;
 #pragma unroll
-    for (int k = 0; k < LMAX; ++k)
-      if (k < levels)
-        (leveled_out + (uint64_t)k * words + wbase)[t] =
-            (uint64_t)s_rows[k][0][t] | ((uint64_t)s_rows[k][1][t] << 32);
+      for (uint32_t hw = 0; hw < 2; ++hw) {
+        const uint32_t Eh = (uint32_t)(E >> (32 * hw));
+        const uint32_t Oh = (uint32_t)(O >> (32 * hw));
+        uint32_t x = Oh | (uint32_t)(C >> (32 * hw));
+        uint32_t done = 0;
+        uint32_t *const slot = &s_rows[0][hw][t];
+        while (x) {
+          const uint32_t low = x & (0u - x);
+          x ^= low;
+          const uint32_t upto = (low - 1u) | (low & ~Oh);
+          const uint32_t seg = upto & ~done & Eh;
+          done = upto;
+          const uint32_t row = min((uint32_t)depth, (uint32_t)LMAX);
+          atomicOr(slot + row * (2 * kTileWords), seg);
+          depth += (low & Oh) ? 1 : -1;
+        }
+        const uint32_t row = min((uint32_t)depth, (uint32_t)LMAX);
+        atomicOr(slot + row * (2 * kTileWords), Eh & ~done);
+      }
+      (string_out + hb)[t] = S;
+#pragma unroll
+      for (int k = 0; k < LMAX; ++k)
+        if (k < levels)
+          (leveled_out + (uint64_t)k * words + hb)[t] =
+              (uint64_t)s_rows[k][0][t] | ((uint64_t)s_rows[k][1][t] << 32);
 
-    // 9. newline positions
-    if (tile == first_tile && t == 0 && nl_cap > 0) nl_out[0] = 0;
-    uint64_t nlm = m.newline;
-    int64_t o = (int64_t)nl_base;
-    while (nlm) {
-      const int b = __builtin_ctzll(nlm);
-      nlm &= nlm - 1;
-      if (o < nl_cap) nl_out[o] = (int64_t)(wbase * 64 - file_base + t * 64 + (uint64_t)b);
-      ++o;
+      // 9. newline positions
+      if (h == 0 && tile == first_tile && t == 0 && nl_cap > 0) nl_out[0] = 0;
+      uint64_t nlm = kN[h];
+      int64_t o = (int64_t)nl_base;
+      while (nlm) {
+        const int b = __builtin_ctzll(nlm);
+        nlm &= nlm - 1;
+        if (o < nl_cap) nl_out[o] = (int64_t)(hb * 64 - file_base + t * 64 + (uint64_t)b);
+        ++o;
+      }
     }
   }
 }
@@ -1077,7 +1165,7 @@ __global__ __launch_bounds__(256) void escape_quote_kernel(
   const uint64_t pos = w * 64;
   uint32_t x[16];
   load_word(in, n, pos, false, x);
-  const WordClasses m = classify(x);
+  const WordClasses m = classify<false>(x);
   // carry-in: parity of the backslash run ending right before this word
   uint32_t cin = 0;
   for (uint64_t p = pos; p > 0 && in[p - 1] == '\\'; --p) cin ^= 1u;
