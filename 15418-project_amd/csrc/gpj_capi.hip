@@ -41,12 +41,6 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 uint64_t tiles_for(uint64_t n) { return div_up(div_up(n, 64), gpj::kTileWords); }
-// Tiles of a single-input index launch (kTileHalves 16 KiB halves each): its
-// tile descriptors fill the first 2 * single_tiles_for(n) of the
-// 2 * tiles_for(n) scratch words.
-uint64_t single_tiles_for(uint64_t n) {
-  return div_up(div_up(n, 64), (uint64_t)gpj::kTileWords * gpj::kTileHalves);
-}
 
 // Scratch layout of the index build: tile aggregates (multi-file: tile
 // descriptors), tile inclusive states (one uint64 per tile each), then a
@@ -74,8 +68,10 @@ gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int lev
                         gpj::BatchSeg seg = gpj::BatchSeg{nullptr, nullptr, nullptr},
                         uint64_t init = gpj::index_start_state(0, 0, -1)) {
   const int aligned = (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
-  // single input: one extra workgroup (block 0) runs the prefix scan
-  hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG>), dim3((uint32_t)(SEG ? tiles : tiles + 1)),
+  // single input: a workgroup per kTileHalves tiles, plus one (block 0) that
+  // runs the prefix scan
+  hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG>),
+                     dim3((uint32_t)(SEG ? tiles : div_up(tiles, gpj::kTileHalves) + 1)),
                      dim3(gpj::kTileWords), 0, st, d_in, n, words, levels, aligned, d_string,
                      d_leveled, d_newline, cap, d_num_lines, desc, counter, tiles, seg,
                      g_wait_limit.load(), init);
@@ -342,8 +338,7 @@ gpj_status build_into(const uint8_t *d_in, uint64_t n, int32_t levels, uint64_t 
   GPJ_HIP(hipGetLastError(), "index_clear_kernel launch");
 #define GPJ_DISPATCH(L)                                                            \
   return launch_index<L>(d_in, n, words, levels, d_string, d_leveled, d_newline,    \
-                         newline_capacity, d_num_lines, desc, counter,             \
-                         single_tiles_for(n), st,                                  \
+                         newline_capacity, d_num_lines, desc, counter, tiles, st,  \
                          gpj::BatchSeg{nullptr, nullptr, nullptr}, init)
   if (levels <= 2) GPJ_DISPATCH(2);
   if (levels <= 4) GPJ_DISPATCH(4);
@@ -389,7 +384,7 @@ gpj_status gpj_index_shard_summary(const void *d_scratch, uint64_t n,
     return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_shard_summary: null scratch or output");
   if (n == 0) return fail(GPJ_ERR_INVALID_ARGUMENT, "gpj_index_shard_summary: empty shard");
   hipLaunchKernelGGL(gpj::shard_summary_kernel, dim3(1), dim3(256), 0, as_stream(stream),
-                     static_cast<const uint64_t *>(d_scratch), single_tiles_for(n),
+                     static_cast<const uint64_t *>(d_scratch), tiles_for(n),
                      reinterpret_cast<uint64_t *>(d_summary));
   GPJ_HIP(hipGetLastError(), "shard_summary_kernel launch");
   return GPJ_OK;

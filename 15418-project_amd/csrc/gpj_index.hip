@@ -238,13 +238,6 @@ __device__ __forceinline__ uint64_t tbl_unpack(uint64_t d) {
   const uint32_t a = (a2 | (a2 << 6) | (a2 << 12) | (a2 << 18)) & 0x03030303u;
   return (uint64_t)a | (((d >> 8) & 0xFFFFFFFFull) << 32);
 }
-// Packs an unpacked table into the 40-bit table field of an aggregate
-// descriptor (the inverse of tbl_unpack).
-__device__ __forceinline__ uint64_t tbl_pack(uint64_t u) {
-  const uint32_t a = (uint32_t)u;
-  const uint32_t st = (a & 3u) | ((a >> 6) & 0xCu) | ((a >> 12) & 0x30u) | ((a >> 18) & 0xC0u);
-  return (uint64_t)st | ((u >> 32) << 8);
-}
 // (f o g)(s) = f(g(s)): apply g first.  v_perm_b32 picks, for each byte s,
 // byte g(s) of f's states and of f's deltas; the deltas add bytewise mod 256.
 __device__ __forceinline__ uint64_t tbl_compose(uint64_t f, uint64_t g) {
@@ -320,6 +313,31 @@ __device__ __forceinline__ void scan_step(uint64_t &tab, uint32_t &nl) {
   dpp_tbl<CTRL, ROWMASK>((uint32_t)tab, (uint32_t)(tab >> 32), nl, oa, ob, onl);
   tab = tbl_compose(tab, (uint64_t)oa | ((uint64_t)ob << 32));
   nl += onl;
+}
+
+// A half-tile's transfer table (packed: output states in bits 0..7, depth
+// deltas from bit 8) from its escape transfer (tail: 0, 1 constant, 2
+// identity, 3 flip), whether an escape carry-in un-counts a quote at its
+// first non-backslash byte (g), its quote parity and its depth sums for
+// string carry-in 0 (D0) and over all brackets (TOT).
+__device__ __forceinline__ uint64_t half_table(uint32_t g, uint32_t tail, uint32_t par, int D0,
+                                               int TOT) {
+  const uint32_t D0b = (uint32_t)D0 & 0xFFu, TOTb = (uint32_t)TOT & 0xFFu;
+  uint64_t tb = 0;
+#pragma unroll
+  for (uint32_t X = 0; X < 2; ++X) {
+#pragma unroll
+    for (uint32_t Y = 0; Y < 2; ++Y) {
+      const uint32_t eff = Y ^ (X & g);
+      const uint32_t esc_out = (tail & 2u) ? (X ^ (tail & 1u)) : (tail & 1u);
+      const uint32_t str_out = Y ^ par ^ (X & g);
+      const uint32_t dd = eff ? ((TOTb - D0b) & 0xFFu) : D0b;
+      const uint32_t in = X | (Y << 1); // input state
+      tb |= (uint64_t)(esc_out | (str_out << 1)) << (2 * in);
+      tb |= (uint64_t)dd << (8 + 8 * in);
+    }
+  }
+  return tb;
 }
 
 // The prefix scanner of a single-input build (one workgroup): resolves tiles
@@ -434,21 +452,21 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
 }
 
 // The fused single-pass index kernel, plus the scanner workgroup (block 0)
-// for a single input.  A workgroup of 256 lanes takes kTileWords words (16
-// KiB) per half-tile, one word per lane.  Single input: a tile is kTileHalves
-// half-tiles (32 KiB; lane t holds word t of each half); the halves'
-// carry-free work (classification, escapes, in-half string parity, depth and
-// newline sums) runs one half after the other, the tile publishes the
-// composition of the halves' transfer tables and waits once for its
-// carry-in, and the second half starts from the first half's carry-out.
-// The carry takes ~10 us to arrive (a tile's publish, the scanner's read and
-// write and the tile's poll are device-scope memory round trips under load),
-// so the bytes a wave slot holds across that wait, not the work per byte,
-// set the throughput: 16 KiB tiles at 8 waves per SIMD ran the 8 GiB input
-// in 4.3 ms.  SEG: multi-file batch (see BatchSeg): one half per tile (files
-// start on 16 KiB boundaries), whose carry comes from a decoupled look-back
-// that never reads before its file's first tile; that tile starts from the
-// initial state.
+// for a single input.  A workgroup of 256 lanes takes whole 16 KiB tiles of
+// kTileWords words, one word per lane.  Single input: kTileHalves
+// consecutive tiles per workgroup (lane t holds word t of each); their
+// carry-free work (classification, escapes, in-tile string parity, depth and
+// newline sums) runs one tile after the other, each tile publishing its
+// aggregate as soon as that is done, then the workgroup waits once, for its
+// first tile's carry-in, and the next tile starts from the previous one's
+// carry-out.  The carry takes ~10 us to arrive (a tile's publish, the
+// scanner's read and write and the tile's poll are device-scope memory round
+// trips under load), so the bytes a wave slot holds across that wait, not
+// the work per byte, set the throughput: one tile per workgroup at 8 waves
+// per SIMD ran the 8 GiB input in 4.3 ms.  SEG: multi-file batch (see
+// BatchSeg): one tile per workgroup (files start on tiles), whose carry
+// comes from a decoupled look-back that never reads before its file's first
+// tile; that tile starts from the initial state.
 template <int LMAX, bool SEG>
 __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG ? 6 : ((LMAX <= 4) ? 6 : 4)))) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
@@ -457,7 +475,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
     int64_t newline_cap, int64_t *__restrict__ num_lines_out,
     uint64_t *__restrict__ desc, uint32_t *__restrict__ tile_counter,
     uint64_t num_tiles, BatchSeg seg, uint32_t max_spins, uint64_t init) {
-  constexpr int NH = SEG ? 1 : kTileHalves; // half-tiles per tile
+  constexpr int NH = SEG ? 1 : kTileHalves; // tiles per workgroup
   uint32_t *const error_flag = tile_counter + 1;
   if (SEG) init = kInitialInclusive; // every file of a batch starts fresh
   __shared__ uint32_t s_tile;
@@ -488,7 +506,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
       scan_tiles(desc, desc + num_tiles, num_tiles, error_flag, max_spins, lane, wv, init);
       return;
     }
-    if (t == 0) s_tile = blockIdx.x - 1;
+    if (t == 0) s_tile = NH * (blockIdx.x - 1);
   } else if (t == 0) {
     s_tile = blockIdx.x;
   }
@@ -496,7 +514,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
   // tile and word base are wave-uniform (scalar registers); per-lane offsets
   // are formed from them where needed rather than kept alive in VGPRs
   const uint64_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_tile);
-  const uint64_t wbase = tile * (uint64_t)(kTileWords * NH);
+  const uint64_t wbase = tile * (uint64_t)kTileWords;
   // the file this tile belongs to (the whole input when !SEG)
   uint64_t first_tile = 0, last_tile = num_tiles - 1, file_base = 0, file_end = n;
   int64_t *nl_out = newline_out;
@@ -520,8 +538,9 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
   // sums over the waves before this one.
   uint64_t kO[NH], kC[NH], kP[NH], kN[NH], kS0[NH];
   uint32_t kDincl[NH], kDv[NH], kNincl[NH], kNlc[NH];
-  uint32_t uFirst[NH], uTail[NH], uPar[NH], uNL[NH], uNlBefore[NH], uMyPar[NH];
-  int32_t uD0[NH], uTOT[NH], uDBefore[NH], uTotBefore[NH];
+  uint32_t uFirst[NH], uTail[NH], uNL[NH], uNlBefore[NH], uMyPar[NH];
+  int32_t uDBefore[NH], uTotBefore[NH];
+  uint64_t htbl[NH]; // each half's transfer table (wave 0)
 
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
@@ -665,52 +684,28 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
     kDv[h] = dv;
     kNincl[h] = nincl;
     kNlc[h] = nlc;
-    uPar[h] = run_par;
     uNL[h] = NL;
     uNlBefore[h] = nl_before;
     uMyPar[h] = my_par;
-    uD0[h] = D0;
-    uTOT[h] = TOT;
     uDBefore[h] = d_before;
     uTotBefore[h] = tot_before;
+    if (wv == 0) {
+      htbl[h] = half_table((uFirst[h] >> 24) & 1u, uTail[h], run_par, D0, TOT);
+      // single input: each half-tile is a tile of the scan, published as
+      // soon as its aggregate is known (the second half's work overlaps
+      // the scanner's latency)
+      if (!SEG && lane == 0 && tile + h < num_tiles)
+        desc_store(&desc[tile + h],
+                   htbl[h] | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
+    }
   }
 
-  // 5. publish the tile aggregate and resolve the exclusive prefix (wave 0)
+  // 5. resolve the exclusive prefix of the workgroup's first half-tile
+  //    (wave 0; a multi-file tile publishes its aggregate here)
   if (wv == 0) {
-    // each half's transfer table (packed: output states in bits 0..7,
-    // depth deltas from bit 8), the tile's as their composition
-    uint64_t htbl[NH];
-#pragma unroll
-    for (int h = 0; h < NH; ++h) {
-      const uint32_t g = (uFirst[h] >> 24) & 1u;
-      const uint32_t tail = uTail[h];
-      const uint32_t D0b = (uint32_t)uD0[h] & 0xFFu, TOTb = (uint32_t)uTOT[h] & 0xFFu;
-      uint64_t tb = 0;
-#pragma unroll
-      for (uint32_t X = 0; X < 2; ++X) {
-#pragma unroll
-        for (uint32_t Y = 0; Y < 2; ++Y) {
-          const uint32_t eff = Y ^ (X & g);
-          const uint32_t esc_out = (tail & 2u) ? (X ^ (tail & 1u)) : (tail & 1u);
-          const uint32_t str_out = Y ^ uPar[h] ^ (X & g);
-          const uint32_t dd = eff ? ((TOTb - D0b) & 0xFFu) : D0b;
-          const uint32_t in = X | (Y << 1); // input state
-          tb |= (uint64_t)(esc_out | (str_out << 1)) << (2 * in);
-          tb |= (uint64_t)dd << (8 + 8 * in);
-        }
-      }
-      htbl[h] = tb;
-    }
-    uint64_t tbl = htbl[0];
-    uint64_t NL = uNL[0];
-#pragma unroll
-    for (int h = 1; h < NH; ++h) {
-      tbl = tbl_pack(tbl_compose(tbl_unpack(htbl[h]), tbl_unpack(tbl)));
-      NL += uNL[h];
-    }
+    const uint64_t tbl = htbl[0];
+    const uint64_t NL = uNL[0];
     Carry excl = decode_inclusive(init); // the first tile's carry-in
-    if (!SEG && lane == 0) // the aggregate, for the scanner
-      desc_store(&desc[tile], tbl | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
     if (!SEG && tile != first_tile) {
       // wait for the inclusive state before this tile's group (the initial
       // state for group 0) and for the aggregates of the group's earlier
@@ -820,14 +815,13 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
       }
     }
     if (lane == 0) {
-      const Carry incl = tbl_apply(tbl_unpack(tbl), NL, excl);
-      if (SEG) desc_store(&desc[tile], encode_inclusive(incl));
-      if (tile == last_tile) *lines_out = (int64_t)(1 + incl.nl);
       Carry ch = excl;
 #pragma unroll
       for (int h = 0; h < NH; ++h) {
         s_carry[h] = ch;
-        if (h + 1 < NH) ch = tbl_apply(tbl_unpack(htbl[h]), uNL[h], ch);
+        ch = tbl_apply(tbl_unpack(htbl[h]), uNL[h], ch);
+        if (SEG) desc_store(&desc[tile], encode_inclusive(ch));
+        if (tile + h == last_tile) *lines_out = (int64_t)(1 + ch.nl);
       }
     }
   }
