@@ -468,7 +468,7 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
 // comes from a decoupled look-back that never reads before its file's first
 // tile; that tile starts from the initial state.
 template <int LMAX, bool SEG>
-__global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG ? 6 : ((LMAX <= 4) ? 6 : 4)))) void index_kernel(
+__global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG ? 6 : ((LMAX <= 4) ? 8 : 4)))) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
     int aligned, uint64_t *__restrict__ string_out,
     uint64_t *__restrict__ leveled_out, int64_t *__restrict__ newline_out,
@@ -476,6 +476,14 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
     uint64_t *__restrict__ desc, uint32_t *__restrict__ tile_counter,
     uint64_t num_tiles, BatchSeg seg, uint32_t max_spins, uint64_t init) {
   constexpr int NH = SEG ? 1 : kTileHalves; // tiles per workgroup
+  // With up to 4 levels a single input runs 8 waves per SIMD (64 VGPRs): the
+  // first tile's kept values wait in LDS (in the leveled-row slots, which
+  // only step 8 uses) while the second tile's steps 1-4 run.
+  constexpr bool STASH = !SEG && NH == 2 && LMAX <= 4;
+  constexpr int kKeptWords = 12; // dwords a tile keeps per lane (see below)
+  constexpr int kRowWords = (LMAX + 1) * 2 * kTileWords;
+  constexpr int kWorkWords =
+      (STASH && kKeptWords * kTileWords > kRowWords) ? kKeptWords * kTileWords : kRowWords;
   uint32_t *const error_flag = tile_counter + 1;
   if (SEG) init = kInitialInclusive; // every file of a batch starts fresh
   __shared__ uint32_t s_tile;
@@ -488,7 +496,10 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
   __shared__ uint32_t s_wnl[NH][kTileWords / 64];  // per wave: newlines
   __shared__ uint32_t s_wscan[NH][kTileWords / 64];
   __shared__ uint32_t s_first[NH];              // k | lead_k << 16 | g << 24
-  __shared__ uint32_t s_rows[LMAX + 1][2][kTileWords]; // per-lane leveled half-words (+ sink)
+  // per-lane leveled half-words (+ sink) of step 8: s_rows[LMAX + 1][2][256];
+  // before that, the first tile's kept values ([kKeptWords][256])
+  __shared__ uint32_t s_work[kWorkWords];
+  uint32_t(*const s_rows)[2][kTileWords] = reinterpret_cast<uint32_t(*)[2][kTileWords]>(s_work);
   __shared__ uint32_t s_tail_esc[NH];           // half escape transfer: 0, 1 constant, 2 identity, 3 flip
   __shared__ Carry s_carry[NH];                 // carry-in of each half
 
@@ -537,14 +548,14 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
   // per wave (scalar registers) the half's transfer-table inputs and the
   // sums over the waves before this one.
   uint64_t kO[NH], kC[NH], kP[NH], kN[NH], kS0[NH];
-  uint32_t kDincl[NH], kDv[NH], kNincl[NH], kNlc[NH];
+  uint32_t kDex[NH], kNlEx[NH]; // exclusive in-wave depth and newline sums
   uint32_t uFirst[NH], uTail[NH], uNL[NH], uNlBefore[NH], uMyPar[NH];
   int32_t uDBefore[NH], uTotBefore[NH];
   uint64_t htbl[NH]; // each half's transfer table (wave 0)
 
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
-    const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the half's first word
+    const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the tile's first word
     const uint64_t pos = (hb + t) * 64;
 
     // 1. classify this lane's 64 bytes
@@ -680,10 +691,23 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
     kP[h] = m.sep;
     kN[h] = m.newline;
     kS0[h] = s0a ^ (my_par ? ~0ull : 0ull);
-    kDincl[h] = dincl;
-    kDv[h] = dv;
-    kNincl[h] = nincl;
-    kNlc[h] = nlc;
+    kDex[h] = dincl - dv;
+    kNlEx[h] = nincl - nlc;
+    if (STASH && h == 0) {
+      uint32_t *const st = &s_work[t];
+      st[0 * kTileWords] = (uint32_t)kO[0];
+      st[1 * kTileWords] = (uint32_t)(kO[0] >> 32);
+      st[2 * kTileWords] = (uint32_t)kC[0];
+      st[3 * kTileWords] = (uint32_t)(kC[0] >> 32);
+      st[4 * kTileWords] = (uint32_t)kP[0];
+      st[5 * kTileWords] = (uint32_t)(kP[0] >> 32);
+      st[6 * kTileWords] = (uint32_t)kN[0];
+      st[7 * kTileWords] = (uint32_t)(kN[0] >> 32);
+      st[8 * kTileWords] = (uint32_t)kS0[0];
+      st[9 * kTileWords] = (uint32_t)(kS0[0] >> 32);
+      st[10 * kTileWords] = kDex[0];
+      st[11 * kTileWords] = kNlEx[0];
+    }
     uNL[h] = NL;
     uNlBefore[h] = nl_before;
     uMyPar[h] = my_par;
@@ -829,8 +853,18 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
 
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
-    const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the half's first word
+    const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the tile's first word
     const Carry c = s_carry[h];
+    if (STASH && h == 0) { // (each lane reads back only its own slots)
+      const uint32_t *const st = &s_work[t];
+      kO[0] = (uint64_t)st[0 * kTileWords] | ((uint64_t)st[1 * kTileWords] << 32);
+      kC[0] = (uint64_t)st[2 * kTileWords] | ((uint64_t)st[3 * kTileWords] << 32);
+      kP[0] = (uint64_t)st[4 * kTileWords] | ((uint64_t)st[5 * kTileWords] << 32);
+      kN[0] = (uint64_t)st[6 * kTileWords] | ((uint64_t)st[7 * kTileWords] << 32);
+      kS0[0] = (uint64_t)st[8 * kTileWords] | ((uint64_t)st[9 * kTileWords] << 32);
+      kDex[0] = st[10 * kTileWords];
+      kNlEx[0] = st[11 * kTileWords];
+    }
 
     // 6. final string mask: string carry-in, plus the flip of everything from
     //    the half's first non-backslash byte on when an escape carry-in of 1
@@ -851,12 +885,12 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
     const uint64_t P = kP[h] & ~S;
 
     // 7. depth before this word and its newline offset
-    const uint64_t nl_base = 1 + c.nl + uNlBefore[h] + (kNincl[h] - kNlc[h]);
+    const uint64_t nl_base = 1 + c.nl + uNlBefore[h] + kNlEx[h];
     int depth_sum;
     if (!esc_flip) {
       // every word keeps its polarity relative to s0a: the words of this
       // wave flip iff my_par ^ c.str, the earlier waves' sums flip iff c.str
-      const uint32_t dex = kDincl[h] - kDv[h];
+      const uint32_t dex = kDex[h];
       const int d0_ex = (int)(dex & 0xFFFFu) - 64 * (int)lane;
       const int tot_ex = (int)(dex >> 16) - 64 * (int)lane;
       depth_sum = (c.str ? uTotBefore[h] - uDBefore[h] : uDBefore[h]) +
