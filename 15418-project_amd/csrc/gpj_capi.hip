@@ -68,10 +68,10 @@ gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int lev
                         gpj::BatchSeg seg = gpj::BatchSeg{nullptr, nullptr, nullptr},
                         uint64_t init = gpj::index_start_state(0, 0, -1)) {
   const int aligned = (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
-  // single input: a workgroup per kTileHalves tiles, plus one (block 0) that
-  // runs the prefix scan
+  // a workgroup per kTileHalves tiles; a single input adds one (block 0)
+  // that runs the prefix scan
   hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG>),
-                     dim3((uint32_t)(SEG ? tiles : div_up(tiles, gpj::kTileHalves) + 1)),
+                     dim3((uint32_t)(div_up(tiles, gpj::kTileHalves) + (SEG ? 0 : 1))),
                      dim3(gpj::kTileWords), 0, st, d_in, n, words, levels, aligned, d_string,
                      d_leveled, d_newline, cap, d_num_lines, desc, counter, tiles, seg,
                      g_wait_limit.load(), init);

@@ -468,18 +468,18 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
 // comes from a decoupled look-back that never reads before its file's first
 // tile; that tile starts from the initial state.
 template <int LMAX, bool SEG>
-__global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG ? 6 : ((LMAX <= 4) ? 8 : 4)))) void index_kernel(
+__global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMAX <= 4) ? 8 : 4))) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
     int aligned, uint64_t *__restrict__ string_out,
     uint64_t *__restrict__ leveled_out, int64_t *__restrict__ newline_out,
     int64_t newline_cap, int64_t *__restrict__ num_lines_out,
     uint64_t *__restrict__ desc, uint32_t *__restrict__ tile_counter,
     uint64_t num_tiles, BatchSeg seg, uint32_t max_spins, uint64_t init) {
-  constexpr int NH = SEG ? 1 : kTileHalves; // tiles per workgroup
-  // With up to 4 levels a single input runs 8 waves per SIMD (64 VGPRs): the
+  constexpr int NH = kTileHalves; // tiles per workgroup
+  // With up to 4 levels the kernel runs 8 waves per SIMD (64 VGPRs): the
   // first tile's kept values wait in LDS (in the leveled-row slots, which
   // only step 8 uses) while the second tile's steps 1-4 run.
-  constexpr bool STASH = !SEG && NH == 2 && LMAX <= 4;
+  constexpr bool STASH = NH == 2 && LMAX <= 4;
   constexpr int kKeptWords = 12; // dwords a tile keeps per lane (see below)
   constexpr int kRowWords = (LMAX + 1) * 2 * kTileWords;
   constexpr int kWorkWords =
@@ -509,9 +509,9 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
 
   // 0. tile id from the block: workgroups are dispatched in block order, so
   //    the tile a wait depends on has been dispatched or is next in line.
-  //    Single input: block 0 is the prefix scanner, block b indexes tile
-  //    b - 1.  Multi-file: block b indexes tile b.  (An atomic tile counter
-  //    costs ~11 ns per tile in series on MI355X.)
+  //    Single input: block 0 is the prefix scanner, block b takes tiles
+  //    NH (b - 1) ..  Multi-file: block b takes tiles NH b ..  (An atomic
+  //    tile counter costs ~11 ns per tile in series on MI355X.)
   if (!SEG) {
     if (blockIdx.x == 0) {
       scan_tiles(desc, desc + num_tiles, num_tiles, error_flag, max_spins, lane, wv, init);
@@ -519,28 +519,30 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
     }
     if (t == 0) s_tile = NH * (blockIdx.x - 1);
   } else if (t == 0) {
-    s_tile = blockIdx.x;
+    s_tile = NH * blockIdx.x;
   }
   lds_barrier();
   // tile and word base are wave-uniform (scalar registers); per-lane offsets
   // are formed from them where needed rather than kept alive in VGPRs
   const uint64_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_tile);
   const uint64_t wbase = tile * (uint64_t)kTileWords;
-  // the file this tile belongs to (the whole input when !SEG)
-  uint64_t first_tile = 0, last_tile = num_tiles - 1, file_base = 0, file_end = n;
-  int64_t *nl_out = newline_out;
-  int64_t nl_cap = newline_cap;
-  int64_t *lines_out = num_lines_out;
-  if (SEG) {
-    const BatchFile bf = seg.files[seg.tile_file[tile]];
-    first_tile = bf.offset / kTileBytes;
-    last_tile = (bf.offset + bf.bytes - 1) / kTileBytes;
-    file_base = bf.offset;
-    file_end = bf.offset + bf.bytes; // bytes past it (padding) read as 0
-    nl_out = newline_out + bf.nl_offset;
-    nl_cap = (int64_t)bf.nl_capacity;
-    lines_out = seg.line_counts + seg.tile_file[tile];
-  }
+  // The file tile tt belongs to (the whole input when !SEG; wave-uniform):
+  // its first and last tiles, byte range (bytes past its end read as 0; a
+  // tile past the last one reads none), newline slice and line count.
+  struct TileFile {
+    uint64_t first, last, base, end;
+    int64_t *nl_out;
+    int64_t nl_cap;
+    int64_t *lines_out;
+  };
+  auto file_of = [&](uint64_t tt) -> TileFile {
+    if (!SEG) return TileFile{0, num_tiles - 1, 0, n, newline_out, newline_cap, num_lines_out};
+    const uint32_t fi = seg.tile_file[tt < num_tiles ? tt : num_tiles - 1];
+    const BatchFile bf = seg.files[fi];
+    return TileFile{bf.offset / kTileBytes, (bf.offset + bf.bytes - 1) / kTileBytes, bf.offset,
+                    tt < num_tiles ? bf.offset + bf.bytes : 0, newline_out + bf.nl_offset,
+                    (int64_t)bf.nl_capacity, seg.line_counts + fi};
+  };
 
   // What each half keeps from its carry-free work until the carry arrives:
   // per lane the word's openers, closers, separators, newlines and string
@@ -557,6 +559,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
   for (int h = 0; h < NH; ++h) {
     const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the tile's first word
     const uint64_t pos = (hb + t) * 64;
+    const uint64_t file_end = file_of(tile + h).end;
 
     // 1. classify this lane's 64 bytes
     WordClasses m = {0, 0, 0, 0, 0, 0};
@@ -715,20 +718,23 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
     uTotBefore[h] = tot_before;
     if (wv == 0) {
       htbl[h] = half_table((uFirst[h] >> 24) & 1u, uTail[h], run_par, D0, TOT);
-      // single input: each half-tile is a tile of the scan, published as
-      // soon as its aggregate is known (the second half's work overlaps
-      // the scanner's latency)
-      if (!SEG && lane == 0 && tile + h < num_tiles)
-        desc_store(&desc[tile + h],
-                   htbl[h] | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
+      // each tile publishes as soon as its aggregate is known (the second
+      // tile's work overlaps the wait for the first's carry); a file's
+      // first tile of a batch knows its inclusive state already
+      if (lane == 0 && tile + h < num_tiles) {
+        if (SEG && tile + h == file_of(tile + h).first)
+          desc_store(&desc[tile + h],
+                     encode_inclusive(tbl_apply(tbl_unpack(htbl[h]), NL, decode_inclusive(init))));
+        else
+          desc_store(&desc[tile + h],
+                     htbl[h] | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
+      }
     }
   }
 
-  // 5. resolve the exclusive prefix of the workgroup's first half-tile
-  //    (wave 0; a multi-file tile publishes its aggregate here)
+  // 5. resolve the exclusive prefix of the workgroup's first tile (wave 0)
   if (wv == 0) {
-    const uint64_t tbl = htbl[0];
-    const uint64_t NL = uNL[0];
+    const uint64_t first_tile = file_of(tile).first;
     Carry excl = decode_inclusive(init); // the first tile's carry-in
     if (!SEG && tile != first_tile) {
       // wait for the inclusive state before this tile's group (the initial
@@ -772,8 +778,6 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
       excl = tbl_apply(acc, acc_nl, decode_inclusive(d));
 
     } else if (SEG && tile != first_tile) {
-      if (lane == 0)
-        desc_store(&desc[tile], tbl | ((uint64_t)NL << 40) | (kFlagAggregate << kFlagShift));
       // Look back 64 predecessors at a time: lane i holds tile-1-i.  Compose
       // the transfer tables newer-first up to the nearest inclusive prefix.
       uint64_t acc = kIdentityTable;
@@ -842,10 +846,12 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
       Carry ch = excl;
 #pragma unroll
       for (int h = 0; h < NH; ++h) {
+        const TileFile f = file_of(tile + h);
+        if (SEG && h > 0 && tile + h == f.first) ch = decode_inclusive(init); // a new file
         s_carry[h] = ch;
         ch = tbl_apply(tbl_unpack(htbl[h]), uNL[h], ch);
-        if (SEG) desc_store(&desc[tile], encode_inclusive(ch));
-        if (tile + h == last_tile) *lines_out = (int64_t)(1 + ch.nl);
+        if (SEG && tile + h < num_tiles) desc_store(&desc[tile + h], encode_inclusive(ch));
+        if (tile + h == f.last && tile + h < num_tiles) *f.lines_out = (int64_t)(1 + ch.nl);
       }
     }
   }
@@ -855,6 +861,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
   for (int h = 0; h < NH; ++h) {
     const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the tile's first word
     const Carry c = s_carry[h];
+    const TileFile f = file_of(tile + h);
     if (STASH && h == 0) { // (each lane reads back only its own slots)
       const uint32_t *const st = &s_work[t];
       kO[0] = (uint64_t)st[0 * kTileWords] | ((uint64_t)st[1 * kTileWords] << 32);
@@ -953,13 +960,13 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu(SEG 
               (uint64_t)s_rows[k][0][t] | ((uint64_t)s_rows[k][1][t] << 32);
 
       // 9. newline positions
-      if (h == 0 && tile == first_tile && t == 0 && nl_cap > 0) nl_out[0] = 0;
+      if (tile + h == f.first && t == 0 && f.nl_cap > 0) f.nl_out[0] = 0;
       uint64_t nlm = kN[h];
       int64_t o = (int64_t)nl_base;
       while (nlm) {
         const int b = __builtin_ctzll(nlm);
         nlm &= nlm - 1;
-        if (o < nl_cap) nl_out[o] = (int64_t)(hb * 64 - file_base + t * 64 + (uint64_t)b);
+        if (o < f.nl_cap) f.nl_out[o] = (int64_t)(hb * 64 - f.base + t * 64 + (uint64_t)b);
         ++o;
       }
     }

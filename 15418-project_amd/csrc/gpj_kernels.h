@@ -7,8 +7,8 @@ namespace gpj {
 
 constexpr int kTileWords = 256;              // words (64 B each) per index tile
 constexpr uint64_t kTileBytes = 64ull * kTileWords;
-// A single-input index workgroup takes kTileHalves consecutive tiles (one
-// carry wait for all of them); a multi-file batch workgroup takes one.
+// An index workgroup takes kTileHalves consecutive tiles (one carry wait for
+// all of them; in a multi-file batch they may belong to different files).
 constexpr int kTileHalves = 2;
 
 __global__ void index_clear_kernel(uint64_t *desc, uint64_t count, uint32_t *counter);
