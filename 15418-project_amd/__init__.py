@@ -181,6 +181,7 @@ def lib():
         L.gpj_debug_escape_quote.argtypes = [vp, u64, vp, vp, vp]
         L.gpj_debug_copy.argtypes = [vp, vp, u64, vp]
         L.gpj_debug_set_narrow_span.argtypes = [i64]
+        L.gpj_debug_set_index_pairing.argtypes = [i64]
         L.gpj_query_set_walk.argtypes = [i32]
         L.gpj_query_run.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp, vp]
         L.gpj_query_run_ext.argtypes = [vp, u64, ctypes.POINTER(IndexT), vp, u32, i32, i64, vp,
@@ -237,6 +238,7 @@ def lib():
                      "gpj_batch_per_file", "gpj_graph_begin", "gpj_graph_end",
                      "gpj_graph_launch", "gpj_graph_destroy", "gpj_stream_create",
                      "gpj_stream_destroy", "gpj_debug_set_narrow_span", "gpj_query_set_walk",
+                     "gpj_debug_set_index_pairing",
                      "gpj_index_build_shard", "gpj_index_shard_summary", "gpj_shard_carries"):
             getattr(L, name).restype = i32
         _lib = L
@@ -725,6 +727,26 @@ class narrow_span:
         check(lib().gpj_debug_set_narrow_span(0), "narrow span")
         return False
 
+
+class index_pairing:
+    """Context manager: single-input index builds of at least `min_tiles`
+    16 KiB tiles run two tiles per workgroup (default 8192; 1 = every input,
+    PAIRING_NEVER = none).  Results never depend on it; tests drive both
+    forms on small inputs."""
+
+    def __init__(self, min_tiles: int):
+        self.min_tiles = min_tiles
+
+    def __enter__(self):
+        check(lib().gpj_debug_set_index_pairing(self.min_tiles), "index pairing")
+        return self
+
+    def __exit__(self, *exc):
+        check(lib().gpj_debug_set_index_pairing(0), "index pairing")
+        return False
+
+
+PAIRING_NEVER = 1 << 62
 
 WALK_AUTO, WALK_LANE, WALK_WAVE = 0, 1, 2
 WAVE_WALK_RECORDS, WAVE_WALK_LONG_RECORDS, WAVE_WALK_LONG_BYTES = 4096, 24576, 2048

@@ -54,13 +54,16 @@ uint32_t *scratch_counter(void *scratch, uint64_t n) {
 
 // Polls a waiting index tile makes before it gives up (tests lower it).
 std::atomic<uint32_t> g_wait_limit{gpj::kDefaultWaitLimit};
+// Single inputs of at least this many tiles take two per workgroup (tests
+// set it to drive either form on any input).
+std::atomic<uint64_t> g_pair_min_tiles{gpj::kPairMinTiles};
 
 __global__ void empty_index_kernel(int64_t *newline, int64_t cap, int64_t *num_lines) {
   if (cap > 0) newline[0] = 0;
   *num_lines = 1;
 }
 
-template <int LMAX, bool SEG = false>
+template <int LMAX, bool SEG, int NH>
 gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int levels,
                         uint64_t *d_string, uint64_t *d_leveled, int64_t *d_newline,
                         int64_t cap, int64_t *d_num_lines, uint64_t *desc,
@@ -70,8 +73,8 @@ gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int lev
   const int aligned = (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
   // a workgroup per kTileHalves tiles; a single input adds one (block 0)
   // that runs the prefix scan
-  hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG>),
-                     dim3((uint32_t)(div_up(tiles, gpj::kTileHalves) + (SEG ? 0 : 1))),
+  hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG, NH>),
+                     dim3((uint32_t)(div_up(tiles, NH) + (SEG ? 0 : 1))),
                      dim3(gpj::kTileWords), 0, st, d_in, n, words, levels, aligned, d_string,
                      d_leveled, d_newline, cap, d_num_lines, desc, counter, tiles, seg,
                      g_wait_limit.load(), init);
@@ -336,10 +339,16 @@ gpj_status build_into(const uint8_t *d_in, uint64_t n, int32_t levels, uint64_t 
   hipLaunchKernelGGL(gpj::index_clear_kernel, dim3((uint32_t)clear_blocks), dim3(256), 0, st,
                      desc, 2 * tiles, counter);
   GPJ_HIP(hipGetLastError(), "index_clear_kernel launch");
-#define GPJ_DISPATCH(L)                                                            \
-  return launch_index<L>(d_in, n, words, levels, d_string, d_leveled, d_newline,    \
-                         newline_capacity, d_num_lines, desc, counter, tiles, st,  \
-                         gpj::BatchSeg{nullptr, nullptr, nullptr}, init)
+#define GPJ_DISPATCH(L)                                                                   \
+  return tiles >= g_pair_min_tiles.load()                                                  \
+             ? launch_index<L, false, gpj::kTileHalves>(                                   \
+                   d_in, n, words, levels, d_string, d_leveled, d_newline, newline_capacity, \
+                   d_num_lines, desc, counter, tiles, st,                                  \
+                   gpj::BatchSeg{nullptr, nullptr, nullptr}, init)                         \
+             : launch_index<L, false, 1>(d_in, n, words, levels, d_string, d_leveled,      \
+                                         d_newline, newline_capacity, d_num_lines, desc,   \
+                                         counter, tiles, st,                               \
+                                         gpj::BatchSeg{nullptr, nullptr, nullptr}, init)
   if (levels <= 2) GPJ_DISPATCH(2);
   if (levels <= 4) GPJ_DISPATCH(4);
   if (levels <= 8) GPJ_DISPATCH(8);
@@ -794,6 +803,13 @@ gpj_status run_batch(const uint8_t *d_in, uint64_t n, const gpj_index *index,
 
 } // namespace
 
+gpj_status gpj_debug_set_index_pairing(int64_t min_tiles) {
+  if (min_tiles < 0)
+    return fail(GPJ_ERR_INVALID_ARGUMENT, "index pairing: 0 (default) or a tile count >= 1");
+  g_pair_min_tiles.store(min_tiles == 0 ? gpj::kPairMinTiles : (uint64_t)min_tiles);
+  return GPJ_OK;
+}
+
 gpj_status gpj_debug_set_narrow_span(int64_t bytes) {
   if (bytes == 0) bytes = gpj::kNarrowSpan;
   if (bytes < 4096 || bytes > gpj::kNarrowSpan)
@@ -1004,7 +1020,7 @@ gpj_status gpj_batch_index(const uint8_t *d_in, uint64_t total_bytes,
                           d_line_prefix};
   gpj_status r;
 #define GPJ_SEG(L)                                                                         \
-  r = launch_index<L, true>(d_in, total_bytes, words, levels, d_string, d_leveled, d_newline, \
+  r = launch_index<L, true, gpj::kTileHalves>(d_in, total_bytes, words, levels, d_string, d_leveled, d_newline, \
                             0, nullptr, desc, counter, tiles, st, seg)
   if (levels <= 2) GPJ_SEG(2);
   else if (levels <= 4) GPJ_SEG(4);

@@ -467,7 +467,7 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
 // BatchSeg): one tile per workgroup (files start on tiles), whose carry
 // comes from a decoupled look-back that never reads before its file's first
 // tile; that tile starts from the initial state.
-template <int LMAX, bool SEG>
+template <int LMAX, bool SEG, int NH>
 __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMAX <= 4) ? 8 : 4))) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
     int aligned, uint64_t *__restrict__ string_out,
@@ -475,10 +475,11 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMA
     int64_t newline_cap, int64_t *__restrict__ num_lines_out,
     uint64_t *__restrict__ desc, uint32_t *__restrict__ tile_counter,
     uint64_t num_tiles, BatchSeg seg, uint32_t max_spins, uint64_t init) {
-  constexpr int NH = kTileHalves; // tiles per workgroup
-  // With up to 4 levels the kernel runs 8 waves per SIMD (64 VGPRs): the
-  // first tile's kept values wait in LDS (in the leveled-row slots, which
-  // only step 8 uses) while the second tile's steps 1-4 run.
+  static_assert(NH == 1 || NH == 2, "one or two tiles per workgroup");
+  // With up to 4 levels the kernel runs 8 waves per SIMD (64 VGPRs): with
+  // two tiles per workgroup the first tile's kept values wait in LDS (in the
+  // leveled-row slots, which only step 8 uses) while the second tile's steps
+  // 1-4 run.
   constexpr bool STASH = NH == 2 && LMAX <= 4;
   constexpr int kKeptWords = 12; // dwords a tile keeps per lane (see below)
   constexpr int kRowWords = (LMAX + 1) * 2 * kTileWords;
@@ -973,16 +974,21 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMA
   }
 }
 
-template __global__ void index_kernel<2, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
-template __global__ void index_kernel<2, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
-template __global__ void index_kernel<4, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
-template __global__ void index_kernel<4, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
-template __global__ void index_kernel<8, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
-template __global__ void index_kernel<8, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
-template __global__ void index_kernel<16, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
-template __global__ void index_kernel<16, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
-template __global__ void index_kernel<22, false>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
-template __global__ void index_kernel<22, true>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<2, false, 1>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<2, false, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<2, true, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<4, false, 1>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<4, false, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<4, true, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<8, false, 1>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<8, false, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<8, true, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<16, false, 1>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<16, false, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<16, true, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<22, false, 1>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<22, false, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
+template __global__ void index_kernel<22, true, 2>(const uint8_t *, uint64_t, uint64_t, int, int, uint64_t *, uint64_t *, int64_t *, int64_t, int64_t *, uint64_t *, uint32_t *, uint64_t, BatchSeg, uint32_t, uint64_t);
 
 // One workgroup: each thread composes a contiguous run of tile tables, then
 // the runs are composed in order.

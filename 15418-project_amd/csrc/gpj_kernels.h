@@ -7,9 +7,13 @@ namespace gpj {
 
 constexpr int kTileWords = 256;              // words (64 B each) per index tile
 constexpr uint64_t kTileBytes = 64ull * kTileWords;
-// An index workgroup takes kTileHalves consecutive tiles (one carry wait for
-// all of them; in a multi-file batch they may belong to different files).
+// An index workgroup takes NH (1 or kTileHalves) consecutive tiles, with one
+// carry wait for all of them (in a multi-file batch they may belong to
+// different files).  Single inputs of at least kPairMinTiles tiles take two
+// per workgroup: more input bytes wait in each wave slot for their carry;
+// smaller ones, whose tiles are all in flight at once anyway, take one.
 constexpr int kTileHalves = 2;
+constexpr uint64_t kPairMinTiles = 8192; // 128 MiB (measured crossover: 64-128 MiB)
 
 __global__ void index_clear_kernel(uint64_t *desc, uint64_t count, uint32_t *counter);
 
@@ -44,7 +48,7 @@ struct BatchSeg {
 // of -1.  No wait is unbounded, so every launch ends.
 constexpr uint32_t kDefaultWaitLimit = 1u << 24;
 
-template <int LMAX, bool SEG>
+template <int LMAX, bool SEG, int NH>
 __global__ void index_kernel(const uint8_t *in, uint64_t n, uint64_t words,
                              int levels, int aligned, uint64_t *string_out,
                              uint64_t *leveled_out, int64_t *newline_out,

@@ -233,6 +233,14 @@ gpj_status gpj_debug_copy(void *d_dst, const void *d_src, uint64_t bytes, void *
  * never depend on it.  Process-wide; set it while no query runs. */
 gpj_status gpj_debug_set_narrow_span(int64_t bytes);
 
+/* Debug (no counterpart in the reference): a single-input index build of at
+ * least `min_tiles` 16 KiB tiles runs two tiles per workgroup (one carry wait
+ * for both); smaller inputs run one.  The default, 8192 tiles (128 MiB), is
+ * restored with 0; tests set 1 (every input in pairs) or a huge count (none)
+ * to drive either form on any input.  Results never depend on it.
+ * Process-wide; set it while no index build runs. */
+gpj_status gpj_debug_set_index_pairing(int64_t min_tiles);
+
 /* Query walk (no counterpart in the reference, whose kernel runs one thread
  * per line, query.cu:144-185): GPJ_WALK_LANE runs one record per lane (64
  * records per wave), GPJ_WALK_WAVE one record per wave over the record staged

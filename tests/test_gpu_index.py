@@ -7,6 +7,14 @@ import jsongen
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, params=["single", "pairs"])
+def pairing(request, gpu):
+    # every test under both index forms: one tile per workgroup and two
+    # (the default picks two from 128 MiB on)
+    with gpu.index_pairing(1 if request.param == "pairs" else gpu.PAIRING_NEVER):
+        yield request.param
+
 FIXTURE = (b'{"id":1,"name":"Ada","tags":["gpu","json"],"active":true}\n'
            b'{"id":2,"name":"Bob \\"The Builder\\"","note":"literal { braces } and [brackets] stay strings"}\n'
            b'{"id":3,"nested":{"arr":[1,{"x":"y,z"}],"empty":{}},"path":"C:\\\\tmp\\\\file"}')

@@ -14,6 +14,14 @@ from shardstate import scan
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=["single", "pairs"])
+def pairing(request, gpu):
+    # every test under both index forms: one tile per workgroup and two
+    # (the default picks two from 128 MiB on)
+    with gpu.index_pairing(1 if request.param == "pairs" else gpu.PAIRING_NEVER):
+        yield request.param
+
+
 def check_shards(gpj, data: bytes, cuts, levels: int, label: str):
     n = len(data)
     dev = gpj.DeviceBuffer.from_host(np.frombuffer(data, dtype=np.uint8))
