@@ -518,21 +518,184 @@ __device__ __forceinline__ bool store_result(Ctx<P> &c, LaneState<MAXLV, P> &s, 
   return true;
 }
 
+// ---- shared first step of a batch -----------------------------------------
+// Every query whose bytecode starts with MOVE_TO_KEY searches the record's
+// top-level object from its '{' (the trimmed record start) on level 0.  The
+// candidates such a search visits, and where it stops when no key matches,
+// depend on the record alone; only the match test depends on the key.  So
+// when a batch holds two or more such queries, one scan of the object tests
+// each candidate against all their distinct keys (the first kKeyMemo of them)
+// and keeps, per lane and key, where that key's search alone would stop, or
+// that it fails; each of those queries then starts after its first step,
+// from that outcome, instead of searching.  Outcomes as MOVE_TO_KEY: the
+// first candidate whose key matches (stop there), a candidate that is not a
+// ':' (stop there, no failure), the object's closing '}' or no progress
+// (failure).  Three keys keep the LDS of the dense-result kernels within
+// 10 KiB per wave (16 waves per CU).
+constexpr int kKeyMemo = 3;
+template <typename P> struct KeyMemo {
+  P pos[kKeyMemo][64]; // where the search ends (a stop position is > 0), -1: it fails
+};
+
+// Which batch queries take their first step from the memo (wave-uniform):
+// nibble q of `slot` is 1 + the memo key of query q (0: none), byte j of
+// `query` the query whose bytecode holds key j; `keys` distinct keys.  Off
+// (slot 0) unless at least two queries qualify.
+struct MemoPlan {
+  uint32_t slot;
+  uint32_t query;
+  int32_t keys;
+};
+__device__ __forceinline__ int32_t memo_slot(const MemoPlan &mp, int q) {
+  return (int32_t)((mp.slot >> (4 * q)) & 15u) - 1;
+}
+__device__ __forceinline__ MemoPlan memo_plan(const QueryBatch &batch) {
+  MemoPlan mp{0, 0, 0};
+  int users = 0;
+  for (int q = 0; q < batch.count; ++q) {
+    const IrReader ir = IrReader::make(batch.q[q].ir, batch.q[q].ir_len);
+    if (ir.len < 2 || ir.byte(0) != kMoveToKey) continue;
+    uint32_t p = 1;
+    const int32_t arg = ir.varint(p);
+    if (arg < 0) continue;
+    int32_t j = 0;
+    for (; j < mp.keys; ++j) { // the same key as an earlier query's?
+      const int qj = (int)((mp.query >> (8 * j)) & 0xFFu);
+      const IrReader oj = IrReader::make(batch.q[qj].ir, batch.q[qj].ir_len);
+      uint32_t pj = 1;
+      if (oj.varint(pj) != arg) continue;
+      bool same = true;
+      for (int32_t k = 0; same && k < arg; k += 8) {
+        const int32_t rest = arg - k;
+        const uint64_t mk = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
+        same = ((ir.u64(p + (uint32_t)k) ^ oj.u64(pj + (uint32_t)k)) & mk) == 0;
+      }
+      if (same) break;
+    }
+    if (j == mp.keys) {
+      if (mp.keys == kKeyMemo) continue;
+      mp.query |= (uint32_t)q << (8 * j);
+      ++mp.keys;
+    }
+    mp.slot |= (uint32_t)(j + 1) << (4 * q);
+    ++users;
+  }
+  if (users < 2) mp = MemoPlan{0, 0, 0};
+  return mp;
+}
+
+// Where query q (with a memo key) starts: the bytecode position after its
+// first MOVE_TO_KEY, and the record position after the key search's stop;
+// a failed search ends the record's walk (no element iteration is open yet).
+template <typename P>
+__device__ __forceinline__ int32_t memo_start(const KeyMemo<P> &m, const IrReader &ir, int32_t j,
+                                              bool &live, P &start_idx) {
+  uint32_t p = 1;
+  const int32_t arg = ir.varint(p); // >= 0 (memo_plan)
+  if (live) {
+    const P pos = m.pos[j][threadIdx.x & 63];
+    live = pos >= 0;
+    start_idx = pos + 1;
+  }
+  return (int32_t)p + arg;
+}
+
+// The scan (see KeyMemo): the loop of MOVE_TO_KEY (run_query) with one match
+// test per pending key at each candidate.  lane_keys: the memo keys this
+// lane's record needs (its scan ends once they are found).
+template <typename P>
+__device__ __forceinline__ void key_memo_scan(Ctx<P> &c, KeyMemo<P> &m, bool live, P line_start,
+                                              P line_end, const QueryBatch &batch,
+                                              const MemoPlan &mp, uint32_t lane_keys) {
+  const int lane = (int)(threadIdx.x & 63);
+  uint32_t pending = live ? lane_keys : 0u;
+  bool failed = false;
+  P idx = line_start;
+  if (live) {
+    const RowRef rr = row_ref(c, 0);
+    idx = next_structural(c, line_end, (P)(line_start + 1), rr);
+    uint64_t w = text8(c, (P)(idx - 7)); // bytes idx - 7 .. idx
+    for (;;) {
+      if ((w >> 56) != ':') break;
+      const bool compact =
+          idx - 2 >= line_start + 1 && ((w >> 48) & 0xFFu) == '"' && ((w >> 40) & 0xFFu) != '\\';
+      P string_end = (P)kUnset; // found on the first key that needs it
+      for (int j = 0; j < mp.keys; ++j) {
+        if (!((pending >> j) & 1u)) continue;
+        const int q = (int)((mp.query >> (8 * j)) & 0xFFu);
+        const IrReader ir = IrReader::make(batch.q[q].ir, batch.q[q].ir_len);
+        uint32_t key_pos = 1;
+        const int32_t arg = ir.varint(key_pos);
+        const uint64_t key0 = ir.u64(key_pos);
+        const uint64_t mask0 = arg >= 8 ? ~0ull : ((1ull << (8 * arg)) - 1);
+        const bool short_key =
+            arg <= 5 && (arg == 0 || ((key0 >> (8 * (arg - 1))) & 0xFFu) != '\\');
+        bool match;
+        if (compact && short_key) {
+          const uint32_t jss = (uint32_t)(5 - arg);
+          const uint64_t pattern = ((uint64_t)'"' << (8 * jss)) |
+                                   ((key0 & mask0) << (8 * (jss + 1))) | (0x22ull << 48) |
+                                   (0x3Aull << 56);
+          const uint64_t pmask = ~0ull << (8 * jss);
+          match = idx - 2 - arg >= line_start && ((w ^ pattern) & pmask) == 0;
+        } else {
+          if (string_end == (P)kUnset)
+            string_end =
+                compact ? (P)(idx - 2) : last_string_bit(c, (P)(line_start + 1), (P)(idx - 1));
+          const P ss = string_end - arg;
+          match = ss >= line_start && byte_at(c, ss) == '"';
+          if (match && arg > 0) {
+            match = ((text8(c, (P)(ss + 1)) ^ key0) & mask0) == 0;
+            for (int32_t k = 8; match && k < arg; k += 8) {
+              const int32_t rest = arg - k;
+              const uint64_t mk = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
+              match = ((text8(c, (P)(ss + 1 + k)) ^ ir.u64(key_pos + (uint32_t)k)) & mk) == 0;
+            }
+          }
+        }
+        if (match) {
+          m.pos[j][lane] = idx;
+          pending &= ~(1u << j);
+        }
+      }
+      if (!pending) break;
+      const P sep = next_structural(c, line_end, (P)(idx + 1), rr);
+      const P nxt = next_structural(c, line_end, (P)(sep + 1), rr);
+      const uint64_t wn = text8(c, (P)(nxt - 7));
+      const P off = sep - (nxt - 7);
+      const uint32_t sep_byte =
+          (off >= 0 && off < 8) ? (uint32_t)(wn >> (8 * off)) & 0xFFu : (uint32_t)byte_at(c, sep);
+      if (sep_byte == '}' || sep == idx) {
+        failed = true;
+        break;
+      }
+      idx = nxt;
+      w = wn;
+    }
+  }
+  for (int j = 0; j < mp.keys; ++j)
+    if ((pending >> j) & 1u) m.pos[j][lane] = failed ? (P)-1 : idx;
+}
+
 // Runs one query over each lane's record.  `live` marks lanes holding a
 // record (trimmed to line_start < line_end).  Fixed mode (reference): STORE i
 // writes slot i; any failure ends the record.  Variable mode (extension):
 // STORE appends (first `cap` stored, all counted) and a finished path
 // resumes the innermost MOVE_TO_EACH at its next element.  A lane whose
 // record needs the wide kernel (c.wide) stops.  Returns the lane's result
-// count.  Stored spans are file offsets (base + position).
+// count.  Stored spans are file offsets (base + position).  The walk starts
+// at bytecode position start_qp (wave-uniform) and record position start_idx:
+// 0 and line_start, or, for a query whose first step key_memo_scan ran,
+// the position after that MOVE_TO_KEY and the position it moved to.
 template <int MAXLV, typename P, int S>
 __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P, S> &w, bool live,
                                              P line_start, P line_end, const IrReader ir,
-                                             int32_t cap, bool variable, const Sink<P> &res) {
+                                             int32_t cap, bool variable, const Sink<P> &res,
+                                             P start_idx, int32_t start_qp) {
   LaneState<MAXLV, P> s(w, (int)(threadIdx.x & 63));
-  s.idx = line_start;
+  s.idx = start_idx;
   s.lvl = 0;
-  s.qp = 0;
+  s.qp = start_qp;
   s.mi = -1;
   s.nfr = 0;
   s.count = 0;
@@ -900,6 +1063,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
   const int32_t lane = (int32_t)(threadIdx.x & 63);
   const int64_t span = NARROW ? pm.span : INT64_MAX;
   __shared__ WaveArrays<MAXLV, P, (DENSE && NARROW) ? kDenseStage : 1> arrays;
+  __shared__ KeyMemo<P> memo;
+  const MemoPlan mp = memo_plan(batch);
   for (int64_t wb = wave_base; wb < lines; wb += stride) {
     const int64_t line = wb + lane;
     bool valid = lane < mode.per_wave && line < lines;
@@ -933,6 +1098,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
                            base, span);
     P line_start = (P)(ls - base), line_end = (P)(le - base);
     if (record) record = trim_record(c, line_start, line_end);
+    if (mp.slot)
+      key_memo_scan<P>(c, memo, record && !c.wide, line_start, line_end, batch, mp,
+                       (1u << mp.keys) - 1u);
     for (int q = 0; q < batch.count; ++q) {
       const QuerySlot &qs = batch.q[q];
       const bool dense = DENSE && qs.dense; // (compile-time false without DENSE)
@@ -957,8 +1125,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
           for (int i = 0; i < 2 * qs.cap; ++i) out.res[i] = -1;
       }
       const IrReader ir = IrReader::make(qs.ir, qs.ir_len);
+      // a query whose first step the memo holds starts after it (a failed
+      // first step ends the record's walk: no element iteration is open yet)
+      P start_idx = line_start;
+      const int32_t j = memo_slot(mp, q);
+      const int32_t start_qp = j >= 0 ? memo_start(memo, ir, j, live, start_idx) : 0;
       const int64_t found = run_query<MAXLV, P>(c, arrays, live, line_start, line_end, ir,
-                                                qs.cap, qs.variable != 0, out);
+                                                qs.cap, qs.variable != 0, out, start_idx,
+                                                start_qp);
       if (fix_dense) continue;
       if (dense) {
         if (NARROW)
@@ -996,6 +1170,8 @@ __global__ __launch_bounds__(64) void query_files_kernel(
   const int32_t lane = (int32_t)(threadIdx.x & 63);
   const int64_t span = NARROW ? pm.span : INT64_MAX;
   __shared__ WaveArrays<MAXLV, P> arrays;
+  __shared__ KeyMemo<P> memo;
+  const MemoPlan mp = memo_plan(batch);
   for (int64_t wb = wave_base; wb < total; wb += stride) {
     const int64_t g = wb + lane;
     bool valid = lane < fm.per_wave && g < total;
@@ -1036,6 +1212,18 @@ __global__ __launch_bounds__(64) void query_files_kernel(
     if (record) record = trim_record(c, line_start, line_end);
     if (valid)
       for (int k = 0; k < kBatchSlots; ++k) fm.counts[k][g] = 0;
+    if (mp.slot) {
+      // the memo keys of the queries this record's file runs (in a result slot)
+      uint32_t lane_keys = 0;
+      for (int q = 0; q < batch.count; ++q) {
+        const int32_t j = memo_slot(mp, q);
+        if (j >= 0 && ((bf.query_mask >> q) & 1u) &&
+            __popc(bf.query_mask & ((1u << q) - 1u)) < kBatchSlots)
+          lane_keys |= 1u << j;
+      }
+      key_memo_scan<P>(c, memo, valid && record && !c.wide, line_start, line_end, batch, mp,
+                       lane_keys);
+    }
     for (int q = 0; q < batch.count; ++q) {
       const QuerySlot &qs = batch.q[q];
       const bool uses = valid && ((bf.query_mask >> q) & 1u);
@@ -1047,9 +1235,12 @@ __global__ __launch_bounds__(64) void query_files_kernel(
       const IrReader ir = IrReader::make(qs.ir, qs.ir_len);
       const int32_t cap = qs.variable ? fm.cap : qs.cap;
       const Sink<P> out{res, cap, nullptr};
-      const int64_t found = run_query<MAXLV, P>(c, arrays, record && mine && !c.wide,
-                                                line_start, line_end, ir, cap,
-                                                qs.variable != 0, out);
+      bool live = record && mine && !c.wide;
+      P start_idx = line_start; // (first step from the memo: see query_batch_kernel)
+      const int32_t j = memo_slot(mp, q);
+      const int32_t start_qp = j >= 0 ? memo_start(memo, ir, j, live, start_idx) : 0;
+      const int64_t found = run_query<MAXLV, P>(c, arrays, live, line_start, line_end, ir, cap,
+                                                qs.variable != 0, out, start_idx, start_qp);
       if (mine) (slot == 0 ? fm.counts[0] : fm.counts[1])[g] = found;
     }
     if (NARROW && (mark || c.wide)) fm.counts[0][g] = kWideMark;

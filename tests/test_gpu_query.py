@@ -127,3 +127,54 @@ def test_key_search_ends_at_a_colon_level_end(gpu, orc):
         print(f"[query] malformed level end q={q}", flush=True)
         got, want = run_both(gpu, orc, data, q)
         np.testing.assert_array_equal(got, want)
+
+
+def batch_vs_oracle(gpj, orc, data: bytes, queries, levels: int, label: str):
+    compiled = [gpj.compile_query(q, extensions=True) for q in queries]
+    dev = gpj.DeviceBuffer.from_host(np.frombuffer(data, dtype=np.uint8))
+    idx = gpj.build_index(dev, len(data), levels)
+    oidx = orc.build_index(data, levels)
+    batch = gpj.run_query_batch(dev, len(data), idx, compiled, capacity=4)
+    for (ir, _d, slots), (spans, counts), q in zip(compiled, batch, queries):
+        print(f"[batch] {label} q={q}", flush=True)
+        if counts is None:
+            np.testing.assert_array_equal(spans, orc.query(data, oidx, ir, slots))
+        else:
+            ws, wc = orc.query_ext(data, oidx, ir, spans.shape[1])
+            np.testing.assert_array_equal(counts, wc)
+            np.testing.assert_array_equal(spans, ws)
+
+
+def test_batch_shared_first_key_edge_forms(gpu, orc):
+    # a batch whose queries start with key lookups shares one scan of each
+    # record's top-level object: every key form (compact, escaped, spaced,
+    # long, empty, absent), malformed level ends and arbitrary bytes must give
+    # each query's own results
+    lines = [
+        b'{"a":1,"b":2,"ab":3}',
+        b'{"k\\\\":1,"a":2}',
+        b'{"a\\"":1,"a":2}',
+        b'{x\\":1,"a":2}',
+        b'{"a" :1,"b"  :2}',
+        b'{"":1,"a":2}',
+        b'{"b":{"a":[1,{"a":2}]},"a":3}',
+        b'{ "a" : { "b" : 4 } }',
+        b'{"\\\\":1,"a\\\\b":2}',
+        b'{"x":"\\":","a":5}',
+        b'{"longerkey":{"b":1},"a":{"b":2},"longerkey2":7}',
+        b'{"a":{"x":1]:2}',
+        b'{"a":{"x":1,"z":[3]]:7}',
+        b'{"b":1}}"a":2}',
+        b'{"a":1,"b":2',
+        b'{}',
+        b'{"h":[{"t":"x","k":"v","id":1}],"a":[0,1]}',
+    ]
+    data = (b"\n".join(lines) + b"\n") * 3
+    queries = ["$.a", "$.b", "$.ab", "$.b.a", "$['k\\\\']", "$.x", "$.longerkey.b"]
+    batch_vs_oracle(gpu, orc, data, queries, 4, "forms")
+    queries = ["$.a.b", "$.a[1]", "$.h[*].t", '$.h[?(@.k == "v")].id', "$.longerkey2", "$.a"]
+    batch_vs_oracle(gpu, orc, data, queries, 4, "forms-ext")
+    soup = jsongen.byte_soup(22, 300_000, special_weight=0.3)
+    batch_vs_oracle(gpu, orc, soup, ["$.a", "$.a[2]", "$.b.a", '$.a.b[?(@ == "x")]'], 3, "soup")
+    # one first-key query among others: the memo stays off
+    batch_vs_oracle(gpu, orc, data, ["$.a", "$[0]", "$"], 4, "single")
