@@ -530,27 +530,38 @@ __device__ __forceinline__ bool store_result(Ctx<P> &c, LaneState<MAXLV, P> &s, 
 // from that outcome, instead of searching.  Outcomes as MOVE_TO_KEY: the
 // first candidate whose key matches (stop there), a candidate that is not a
 // ':' (stop there, no failure), the object's closing '}' or no progress
-// (failure).  Three keys keep the LDS of the dense-result kernels within
-// 10 KiB per wave (16 waves per CU).
+// (failure).  When every query of a key follows it with MOVE_DOWN, the scan
+// also takes that step: the value's end on level 0 is the next candidate
+// search's first structural (the scan needs it anyway to go on), and the
+// spaces after the stop are in the lane's text window.  (The kernels with a
+// dense-results stage keep no level ends: three keys' positions keep their
+// LDS within 10 KiB per wave, 16 waves per CU.)
 constexpr int kKeyMemo = 3;
-template <typename P> struct KeyMemo {
-  P pos[kKeyMemo][64]; // where the search ends (a stop position is > 0), -1: it fails
+template <typename P, bool END> struct KeyMemo {
+  P pos[kKeyMemo][64]; // where the query goes on (> 0), -1: the search fails
+  P end[kKeyMemo][64]; // keys followed by MOVE_DOWN: the level end it sets
+};
+template <typename P> struct KeyMemo<P, false> {
+  P pos[kKeyMemo][64];
 };
 
 // Which batch queries take their first step from the memo (wave-uniform):
 // nibble q of `slot` is 1 + the memo key of query q (0: none), byte j of
-// `query` the query whose bytecode holds key j; `keys` distinct keys.  Off
-// (slot 0) unless at least two queries qualify.
+// `query` the query whose bytecode holds key j; `keys` distinct keys; bit j
+// of `down`: every query of key j has MOVE_DOWN right after it.  Off (slot 0)
+// unless at least two queries qualify.
 struct MemoPlan {
   uint32_t slot;
   uint32_t query;
   int32_t keys;
+  uint32_t down; // (0 when the memo keeps no level ends)
 };
 __device__ __forceinline__ int32_t memo_slot(const MemoPlan &mp, int q) {
   return (int32_t)((mp.slot >> (4 * q)) & 15u) - 1;
 }
-__device__ __forceinline__ MemoPlan memo_plan(const QueryBatch &batch) {
-  MemoPlan mp{0, 0, 0};
+__device__ __forceinline__ MemoPlan memo_plan(const QueryBatch &batch, bool ends) {
+  MemoPlan mp{0, 0, 0, 0};
+  uint32_t not_down = 0;
   int users = 0;
   for (int q = 0; q < batch.count; ++q) {
     const IrReader ir = IrReader::make(batch.q[q].ir, batch.q[q].ir_len);
@@ -578,103 +589,127 @@ __device__ __forceinline__ MemoPlan memo_plan(const QueryBatch &batch) {
       ++mp.keys;
     }
     mp.slot |= (uint32_t)(j + 1) << (4 * q);
+    if (ir.byte(p + (uint32_t)arg) != kMoveDown) not_down |= 1u << j;
     ++users;
   }
-  if (users < 2) mp = MemoPlan{0, 0, 0};
+  mp.down = ends ? ((1u << mp.keys) - 1u) & ~not_down : 0u;
+  if (users < 2) mp = MemoPlan{0, 0, 0, 0};
   return mp;
 }
 
-// Where query q (with a memo key) starts: the bytecode position after its
-// first MOVE_TO_KEY, and the record position after the key search's stop;
-// a failed search ends the record's walk (no element iteration is open yet).
-template <typename P>
-__device__ __forceinline__ int32_t memo_start(const KeyMemo<P> &m, const IrReader &ir, int32_t j,
-                                              bool &live, P &start_idx) {
+// Where query q (with memo key j) starts: the bytecode position after its
+// first MOVE_TO_KEY (and the MOVE_DOWN after it when the scan took that
+// too), the record position and the level end of level 1 (kUnset: none); a
+// failed search ends the record's walk (no element iteration is open yet).
+template <typename P, bool END>
+__device__ __forceinline__ int32_t memo_start(const KeyMemo<P, END> &m, const MemoPlan &mp,
+                                              const IrReader &ir, int32_t j, bool &live,
+                                              P &start_idx, P &start_end) {
   uint32_t p = 1;
   const int32_t arg = ir.varint(p); // >= 0 (memo_plan)
+  const bool down = (mp.down >> j) & 1u;
   if (live) {
-    const P pos = m.pos[j][threadIdx.x & 63];
+    const int lane = (int)(threadIdx.x & 63);
+    const P pos = m.pos[j][lane];
     live = pos >= 0;
-    start_idx = pos + 1;
+    start_idx = pos;
+    if constexpr (END)
+      if (down) start_end = m.end[j][lane];
   }
-  return (int32_t)p + arg;
+  return (int32_t)p + arg + (down ? 1 : 0);
 }
 
 // The scan (see KeyMemo): the loop of MOVE_TO_KEY (run_query) with one match
 // test per pending key at each candidate.  lane_keys: the memo keys this
 // lane's record needs (its scan ends once they are found).
-template <typename P>
-__device__ __forceinline__ void key_memo_scan(Ctx<P> &c, KeyMemo<P> &m, bool live, P line_start,
-                                              P line_end, const QueryBatch &batch,
+template <typename P, bool END>
+__device__ __forceinline__ void key_memo_scan(Ctx<P> &c, KeyMemo<P, END> &m, bool live,
+                                              P line_start, P line_end, const QueryBatch &batch,
                                               const MemoPlan &mp, uint32_t lane_keys) {
   const int lane = (int)(threadIdx.x & 63);
-  uint32_t pending = live ? lane_keys : 0u;
-  bool failed = false;
-  P idx = line_start;
-  if (live) {
-    const RowRef rr = row_ref(c, 0);
-    idx = next_structural(c, line_end, (P)(line_start + 1), rr);
-    uint64_t w = text8(c, (P)(idx - 7)); // bytes idx - 7 .. idx
-    for (;;) {
-      if ((w >> 56) != ':') break;
-      const bool compact =
-          idx - 2 >= line_start + 1 && ((w >> 48) & 0xFFu) == '"' && ((w >> 40) & 0xFFu) != '\\';
-      P string_end = (P)kUnset; // found on the first key that needs it
-      for (int j = 0; j < mp.keys; ++j) {
-        if (!((pending >> j) & 1u)) continue;
-        const int q = (int)((mp.query >> (8 * j)) & 0xFFu);
-        const IrReader ir = IrReader::make(batch.q[q].ir, batch.q[q].ir_len);
-        uint32_t key_pos = 1;
-        const int32_t arg = ir.varint(key_pos);
-        const uint64_t key0 = ir.u64(key_pos);
-        const uint64_t mask0 = arg >= 8 ? ~0ull : ((1ull << (8 * arg)) - 1);
-        const bool short_key =
-            arg <= 5 && (arg == 0 || ((key0 >> (8 * (arg - 1))) & 0xFFu) != '\\');
-        bool match;
-        if (compact && short_key) {
-          const uint32_t jss = (uint32_t)(5 - arg);
-          const uint64_t pattern = ((uint64_t)'"' << (8 * jss)) |
-                                   ((key0 & mask0) << (8 * (jss + 1))) | (0x22ull << 48) |
-                                   (0x3Aull << 56);
-          const uint64_t pmask = ~0ull << (8 * jss);
-          match = idx - 2 - arg >= line_start && ((w ^ pattern) & pmask) == 0;
-        } else {
-          if (string_end == (P)kUnset)
-            string_end =
-                compact ? (P)(idx - 2) : last_string_bit(c, (P)(line_start + 1), (P)(idx - 1));
-          const P ss = string_end - arg;
-          match = ss >= line_start && byte_at(c, ss) == '"';
-          if (match && arg > 0) {
-            match = ((text8(c, (P)(ss + 1)) ^ key0) & mask0) == 0;
-            for (int32_t k = 8; match && k < arg; k += 8) {
-              const int32_t rest = arg - k;
-              const uint64_t mk = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
-              match = ((text8(c, (P)(ss + 1 + k)) ^ ir.u64(key_pos + (uint32_t)k)) & mk) == 0;
-            }
+  uint32_t pending = lane_keys;
+  if (!live || !pending) return;
+  const RowRef rr = row_ref(c, 0);
+  // a search that stops at idx goes on at idx + 1; a MOVE_DOWN there sets
+  // level 1's end to the first level-0 structural from idx + 2 (sep: the
+  // first one from idx + 1) and skips the spaces from idx + 1
+  auto stop = [&](uint32_t keys, P idx, P sep) {
+    if (keys & mp.down) {
+      const P end = sep == idx + 1 ? next_structural(c, line_end, (P)(idx + 2), rr) : sep;
+      P at = idx + 1;
+      while (byte_at(c, at) == ' ') ++at;
+      for (int j = 0; j < mp.keys; ++j)
+        if ((keys & mp.down) >> j & 1u) {
+          m.pos[j][lane] = at;
+          if constexpr (END) m.end[j][lane] = end;
+        }
+    }
+    for (int j = 0; j < mp.keys; ++j)
+      if ((keys & ~mp.down) >> j & 1u) m.pos[j][lane] = idx + 1;
+  };
+  P idx = next_structural(c, line_end, (P)(line_start + 1), rr);
+  uint64_t w = text8(c, (P)(idx - 7)); // bytes idx - 7 .. idx
+  for (;;) {
+    if ((w >> 56) != ':') {
+      stop(pending, idx, next_structural(c, line_end, (P)(idx + 1), rr));
+      return;
+    }
+    const bool compact =
+        idx - 2 >= line_start + 1 && ((w >> 48) & 0xFFu) == '"' && ((w >> 40) & 0xFFu) != '\\';
+    P string_end = (P)kUnset; // found on the first key that needs it
+    uint32_t matched = 0;
+    for (int j = 0; j < mp.keys; ++j) {
+      if (!((pending >> j) & 1u)) continue;
+      const int q = (int)((mp.query >> (8 * j)) & 0xFFu);
+      const IrReader ir = IrReader::make(batch.q[q].ir, batch.q[q].ir_len);
+      uint32_t key_pos = 1;
+      const int32_t arg = ir.varint(key_pos);
+      const uint64_t key0 = ir.u64(key_pos);
+      const uint64_t mask0 = arg >= 8 ? ~0ull : ((1ull << (8 * arg)) - 1);
+      const bool short_key =
+          arg <= 5 && (arg == 0 || ((key0 >> (8 * (arg - 1))) & 0xFFu) != '\\');
+      bool match;
+      if (compact && short_key) {
+        const uint32_t jss = (uint32_t)(5 - arg);
+        const uint64_t pattern = ((uint64_t)'"' << (8 * jss)) |
+                                 ((key0 & mask0) << (8 * (jss + 1))) | (0x22ull << 48) |
+                                 (0x3Aull << 56);
+        const uint64_t pmask = ~0ull << (8 * jss);
+        match = idx - 2 - arg >= line_start && ((w ^ pattern) & pmask) == 0;
+      } else {
+        if (string_end == (P)kUnset)
+          string_end =
+              compact ? (P)(idx - 2) : last_string_bit(c, (P)(line_start + 1), (P)(idx - 1));
+        const P ss = string_end - arg;
+        match = ss >= line_start && byte_at(c, ss) == '"';
+        if (match && arg > 0) {
+          match = ((text8(c, (P)(ss + 1)) ^ key0) & mask0) == 0;
+          for (int32_t k = 8; match && k < arg; k += 8) {
+            const int32_t rest = arg - k;
+            const uint64_t mk = rest >= 8 ? ~0ull : ((1ull << (8 * rest)) - 1);
+            match = ((text8(c, (P)(ss + 1 + k)) ^ ir.u64(key_pos + (uint32_t)k)) & mk) == 0;
           }
         }
-        if (match) {
-          m.pos[j][lane] = idx;
-          pending &= ~(1u << j);
-        }
       }
-      if (!pending) break;
-      const P sep = next_structural(c, line_end, (P)(idx + 1), rr);
-      const P nxt = next_structural(c, line_end, (P)(sep + 1), rr);
-      const uint64_t wn = text8(c, (P)(nxt - 7));
-      const P off = sep - (nxt - 7);
-      const uint32_t sep_byte =
-          (off >= 0 && off < 8) ? (uint32_t)(wn >> (8 * off)) & 0xFFu : (uint32_t)byte_at(c, sep);
-      if (sep_byte == '}' || sep == idx) {
-        failed = true;
-        break;
-      }
-      idx = nxt;
-      w = wn;
+      if (match) matched |= 1u << j;
     }
+    pending &= ~matched;
+    const P sep = next_structural(c, line_end, (P)(idx + 1), rr);
+    if (matched) stop(matched, idx, sep);
+    if (!pending) return;
+    const P nxt = next_structural(c, line_end, (P)(sep + 1), rr);
+    const uint64_t wn = text8(c, (P)(nxt - 7));
+    const P off = sep - (nxt - 7);
+    const uint32_t sep_byte =
+        (off >= 0 && off < 8) ? (uint32_t)(wn >> (8 * off)) & 0xFFu : (uint32_t)byte_at(c, sep);
+    if (sep_byte == '}' || sep == idx) { // the searches of the pending keys fail
+      for (int j = 0; j < mp.keys; ++j)
+        if ((pending >> j) & 1u) m.pos[j][lane] = (P)-1;
+      return;
+    }
+    idx = nxt;
+    w = wn;
   }
-  for (int j = 0; j < mp.keys; ++j)
-    if ((pending >> j) & 1u) m.pos[j][lane] = failed ? (P)-1 : idx;
 }
 
 // Runs one query over each lane's record.  `live` marks lanes holding a
@@ -685,22 +720,26 @@ __device__ __forceinline__ void key_memo_scan(Ctx<P> &c, KeyMemo<P> &m, bool liv
 // record needs the wide kernel (c.wide) stops.  Returns the lane's result
 // count.  Stored spans are file offsets (base + position).  The walk starts
 // at bytecode position start_qp (wave-uniform) and record position start_idx:
-// 0 and line_start, or, for a query whose first step key_memo_scan ran,
-// the position after that MOVE_TO_KEY and the position it moved to.
+// 0 and line_start, or, for a query whose first steps key_memo_scan took,
+// the position after them and where they left the record (on level 1 with
+// that level ending at start_end when they include a MOVE_DOWN, else
+// start_end is kUnset).
 template <int MAXLV, typename P, int S>
 __device__ __forceinline__ int64_t run_query(Ctx<P> &c, WaveArrays<MAXLV, P, S> &w, bool live,
                                              P line_start, P line_end, const IrReader ir,
                                              int32_t cap, bool variable, const Sink<P> &res,
-                                             P start_idx, int32_t start_qp) {
+                                             P start_idx, int32_t start_qp, P start_end) {
+  static_assert(MAXLV >= 2, "a first MOVE_DOWN needs level 1");
   LaneState<MAXLV, P> s(w, (int)(threadIdx.x & 63));
   s.idx = start_idx;
-  s.lvl = 0;
+  s.lvl = start_end == (P)kUnset ? 0 : 1;
   s.qp = start_qp;
   s.mi = -1;
   s.nfr = 0;
   s.count = 0;
   s.level_end.fill((P)kUnset);
   s.level_end.set(0, line_end);
+  if (s.lvl) s.level_end.set(1, start_end);
   s.curr_index.fill(-1);
 
   for (;;) {
@@ -1063,8 +1102,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
   const int32_t lane = (int32_t)(threadIdx.x & 63);
   const int64_t span = NARROW ? pm.span : INT64_MAX;
   __shared__ WaveArrays<MAXLV, P, (DENSE && NARROW) ? kDenseStage : 1> arrays;
-  __shared__ KeyMemo<P> memo;
-  const MemoPlan mp = memo_plan(batch);
+  constexpr bool kMemoEnds = !(DENSE && NARROW);
+  __shared__ KeyMemo<P, kMemoEnds> memo;
+  const MemoPlan mp = memo_plan(batch, kMemoEnds);
   for (int64_t wb = wave_base; wb < lines; wb += stride) {
     const int64_t line = wb + lane;
     bool valid = lane < mode.per_wave && line < lines;
@@ -1099,7 +1139,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
     P line_start = (P)(ls - base), line_end = (P)(le - base);
     if (record) record = trim_record(c, line_start, line_end);
     if (mp.slot)
-      key_memo_scan<P>(c, memo, record && !c.wide, line_start, line_end, batch, mp,
+      key_memo_scan(c, memo, record && !c.wide, line_start, line_end, batch, mp,
                        (1u << mp.keys) - 1u);
     for (int q = 0; q < batch.count; ++q) {
       const QuerySlot &qs = batch.q[q];
@@ -1125,14 +1165,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MAXLV == 4 ?
           for (int i = 0; i < 2 * qs.cap; ++i) out.res[i] = -1;
       }
       const IrReader ir = IrReader::make(qs.ir, qs.ir_len);
-      // a query whose first step the memo holds starts after it (a failed
-      // first step ends the record's walk: no element iteration is open yet)
-      P start_idx = line_start;
+      // a query whose first steps the memo holds starts after them (memo_start)
+      P start_idx = line_start, start_end = (P)kUnset;
       const int32_t j = memo_slot(mp, q);
-      const int32_t start_qp = j >= 0 ? memo_start(memo, ir, j, live, start_idx) : 0;
+      const int32_t start_qp =
+          j >= 0 ? memo_start(memo, mp, ir, j, live, start_idx, start_end) : 0;
       const int64_t found = run_query<MAXLV, P>(c, arrays, live, line_start, line_end, ir,
                                                 qs.cap, qs.variable != 0, out, start_idx,
-                                                start_qp);
+                                                start_qp, start_end);
       if (fix_dense) continue;
       if (dense) {
         if (NARROW)
@@ -1170,8 +1210,8 @@ __global__ __launch_bounds__(64) void query_files_kernel(
   const int32_t lane = (int32_t)(threadIdx.x & 63);
   const int64_t span = NARROW ? pm.span : INT64_MAX;
   __shared__ WaveArrays<MAXLV, P> arrays;
-  __shared__ KeyMemo<P> memo;
-  const MemoPlan mp = memo_plan(batch);
+  __shared__ KeyMemo<P, true> memo;
+  const MemoPlan mp = memo_plan(batch, true);
   for (int64_t wb = wave_base; wb < total; wb += stride) {
     const int64_t g = wb + lane;
     bool valid = lane < fm.per_wave && g < total;
@@ -1221,7 +1261,7 @@ __global__ __launch_bounds__(64) void query_files_kernel(
             __popc(bf.query_mask & ((1u << q) - 1u)) < kBatchSlots)
           lane_keys |= 1u << j;
       }
-      key_memo_scan<P>(c, memo, valid && record && !c.wide, line_start, line_end, batch, mp,
+      key_memo_scan(c, memo, valid && record && !c.wide, line_start, line_end, batch, mp,
                        lane_keys);
     }
     for (int q = 0; q < batch.count; ++q) {
@@ -1236,11 +1276,13 @@ __global__ __launch_bounds__(64) void query_files_kernel(
       const int32_t cap = qs.variable ? fm.cap : qs.cap;
       const Sink<P> out{res, cap, nullptr};
       bool live = record && mine && !c.wide;
-      P start_idx = line_start; // (first step from the memo: see query_batch_kernel)
+      P start_idx = line_start, start_end = (P)kUnset; // (memo: see query_batch_kernel)
       const int32_t j = memo_slot(mp, q);
-      const int32_t start_qp = j >= 0 ? memo_start(memo, ir, j, live, start_idx) : 0;
+      const int32_t start_qp =
+          j >= 0 ? memo_start(memo, mp, ir, j, live, start_idx, start_end) : 0;
       const int64_t found = run_query<MAXLV, P>(c, arrays, live, line_start, line_end, ir, cap,
-                                                qs.variable != 0, out, start_idx, start_qp);
+                                                qs.variable != 0, out, start_idx, start_qp,
+                                                start_end);
       if (mine) (slot == 0 ? fm.counts[0] : fm.counts[1])[g] = found;
     }
     if (NARROW && (mark || c.wide)) fm.counts[0][g] = kWideMark;
