@@ -71,10 +71,9 @@ gpj_status launch_index(const uint8_t *d_in, uint64_t n, uint64_t words, int lev
                         gpj::BatchSeg seg = gpj::BatchSeg{nullptr, nullptr, nullptr},
                         uint64_t init = gpj::index_start_state(0, 0, -1)) {
   const int aligned = (reinterpret_cast<uintptr_t>(d_in) & 15u) == 0;
-  // a workgroup per kTileHalves tiles; a single input adds one (block 0)
-  // that runs the prefix scan
+  // a workgroup per NH tiles, plus one (block 0) that runs the prefix scan
   hipLaunchKernelGGL((gpj::index_kernel<LMAX, SEG, NH>),
-                     dim3((uint32_t)(div_up(tiles, NH) + (SEG ? 0 : 1))),
+                     dim3((uint32_t)(div_up(tiles, NH) + 1)),
                      dim3(gpj::kTileWords), 0, st, d_in, n, words, levels, aligned, d_string,
                      d_leveled, d_newline, cap, d_num_lines, desc, counter, tiles, seg,
                      g_wait_limit.load(), init);
@@ -543,7 +542,7 @@ gpj_status gpj_index_build(const uint8_t *d_in, uint64_t n, int32_t levels,
             "copy index status");
     if (failed) {
       cleanup();
-      return fail(GPJ_ERR_DEVICE, "index build: tile look-back timed out");
+      return fail(GPJ_ERR_DEVICE, "index build: a tile's wait for its carry timed out");
     }
   }
   if (num_lines > cap) {

@@ -11,14 +11,13 @@
 // with v_dot4_u32_u8.  The three carried states (escape parity, in-string
 // parity, nesting depth) plus the newline count cross tiles as follows: each
 // tile publishes, before it knows its carry-in, a 4-entry transfer table
-// indexed by (escape-in, string-in).  In a single-input build one extra
-// workgroup scans those tables in tile order, 1,024 tiles per round, and
-// writes the inclusive state of every 8th tile; a tile waits for the one
-// before its group of 8.  (One atomic tile counter, or a look-back per tile,
-// each cost about 11 ns per tile in series on MI355X: the chain, not the
-// work, bounded the kernel.)  A multi-file batch resolves its prefixes with a
-// decoupled look-back over predecessors' descriptors, restarted at each file
-// (a median file has 16 tiles).
+// indexed by (escape-in, string-in).  One extra workgroup scans those tables
+// in tile order, 1,024 tiles per round, and writes the inclusive state of
+// every 8th tile; a tile waits for the one before its group of 8.  (One
+// atomic tile counter, or a look-back per tile, each cost about 11 ns per
+// tile in series on MI355X: the chain, not the work, bounded the kernel.)  In
+// a multi-file batch each file's first tile publishes its inclusive state at
+// once (files start from the initial state) and the scan restarts there.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -307,12 +306,43 @@ __device__ __forceinline__ void dpp_tbl(uint32_t a, uint32_t b, uint32_t nl, uin
   ob = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xF, false);
   onl = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)nl, CTRL, ROWMASK, 0xF, false);
 }
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ void scan_step(uint64_t &tab, uint32_t &nl) {
+// One step of the scanner's wave scan.  SEG: a composite may be absolute (a
+// file's first tile and anything after it: `abs`), which no older composite
+// changes.
+template <int CTRL, int ROWMASK, bool SEG>
+__device__ __forceinline__ void scan_step(uint64_t &tab, uint32_t &nl, uint32_t &abs) {
   uint32_t oa, ob, onl;
   dpp_tbl<CTRL, ROWMASK>((uint32_t)tab, (uint32_t)(tab >> 32), nl, oa, ob, onl);
-  tab = tbl_compose(tab, (uint64_t)oa | ((uint64_t)ob << 32));
-  nl += onl;
+  if (SEG) {
+    const uint32_t oabs =
+        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)abs, CTRL, ROWMASK, 0xF, false);
+    if (!abs) {
+      tab = tbl_compose(tab, (uint64_t)oa | ((uint64_t)ob << 32));
+      nl += onl;
+      abs = oabs;
+    }
+  } else {
+    tab = tbl_compose(tab, (uint64_t)oa | ((uint64_t)ob << 32));
+    nl += onl;
+  }
+}
+
+// A multi-file batch's scan meets inclusive descriptors (a file's first tile
+// publishes its state, from the initial one, at once): an absolute
+// composite, the table mapping every input state to that state (and depth:
+// all four delta bytes hold it, so composing later tables onto it is the
+// ordinary composition) with its newline count.
+__device__ __forceinline__ uint64_t abs_table(uint64_t d) {
+  const uint64_t st = d & 3u, depth = (d >> 2) & 0xFFu;
+  return st * 0x01010101ull | ((depth * 0x01010101ull) << 32);
+}
+__device__ __forceinline__ Carry abs_apply(uint64_t tab, uint64_t nl) {
+  Carry o;
+  o.esc = (uint32_t)tab & 1u;
+  o.str = ((uint32_t)tab >> 1) & 1u;
+  o.depth = (int32_t)(int8_t)(uint8_t)(tab >> 32);
+  o.nl = nl;
+  return o;
 }
 
 // A half-tile's transfer table (packed: output states in bits 0..7, depth
@@ -340,21 +370,23 @@ __device__ __forceinline__ uint64_t half_table(uint32_t g, uint32_t tail, uint32
   return tb;
 }
 
-// The prefix scanner of a single-input build (one workgroup): resolves tiles
-// in order.  Each round reads the aggregates of the next 1,024 tiles, takes
+// The prefix scanner (one workgroup): resolves tiles in order.  Each round reads the aggregates of the next 1,024 tiles, takes
 // the longest run of them that is published, composes each lane's share, then
 // the lanes' composites with a DPP wave scan and the waves' composites through
 // LDS, applies them to the running carry and writes the inclusive state of
 // each resolved group's last tile.  A tile waits only for its group's
 // carry-in, so the carry chain costs one descriptor round trip per round
 // instead of a look-back per tile.  Every wave takes the same branches (counts come
-// through LDS), so the barriers always match.
+// through LDS), so the barriers always match.  SEG: a multi-file batch, whose
+// files' first tiles publish inclusive descriptors; the carry restarts there.
+template <bool SEG>
 __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tiles,
                            uint32_t *error_flag, uint32_t max_spins, uint32_t lane,
                            uint32_t wv, uint64_t init) {
   __shared__ uint32_t s_ready[kScanWaves];
   __shared__ uint64_t s_tab[kScanWaves];
   __shared__ uint32_t s_nl[kScanWaves];
+  __shared__ uint32_t s_abs[kScanWaves];
   Carry carry = decode_inclusive(init);
   uint64_t next = 0;
   uint32_t spins = 0;
@@ -397,38 +429,51 @@ __device__ void scan_tiles(const uint64_t *agg, uint64_t *inc, uint64_t num_tile
     // this lane's resolved tiles: [b, b + m)
     const uint32_t m = lo >= ready ? 0u : (uint32_t)min((uint64_t)kScanPer, ready - lo);
     uint64_t tab = kIdentityTable;
-    uint32_t nl = 0;
+    uint32_t nl = 0, abs = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScanPer; ++k)
       if (k < m) {
-        tab = tbl_compose(tbl_unpack(d[k]), tab);
-        nl += (uint32_t)(d[k] >> 40) & ((1u << 22) - 1);
+        if (SEG && (d[k] >> kFlagShift) == kFlagInclusive) { // a file's first tile
+          tab = abs_table(d[k]);
+          nl = (uint32_t)(d[k] >> 10) & ((1u << 22) - 1);
+          abs = 1;
+        } else {
+          tab = tbl_compose(tbl_unpack(d[k]), tab);
+          nl += (uint32_t)(d[k] >> 40) & ((1u << 22) - 1);
+        }
       }
     // inclusive scan over the wave's lanes, older lanes applied first
-    scan_step<0x111, 0xF>(tab, nl); // row_shr:1
-    scan_step<0x112, 0xF>(tab, nl); // row_shr:2
-    scan_step<0x114, 0xF>(tab, nl); // row_shr:4
-    scan_step<0x118, 0xF>(tab, nl); // row_shr:8
-    scan_step<0x142, 0xA>(tab, nl); // row_bcast:15
-    scan_step<0x143, 0xC>(tab, nl); // row_bcast:31
+    scan_step<0x111, 0xF, SEG>(tab, nl, abs); // row_shr:1
+    scan_step<0x112, 0xF, SEG>(tab, nl, abs); // row_shr:2
+    scan_step<0x114, 0xF, SEG>(tab, nl, abs); // row_shr:4
+    scan_step<0x118, 0xF, SEG>(tab, nl, abs); // row_shr:8
+    scan_step<0x142, 0xA, SEG>(tab, nl, abs); // row_bcast:15
+    scan_step<0x143, 0xC, SEG>(tab, nl, abs); // row_bcast:31
     if (lane == 63) {
       s_tab[wv] = tab;
       s_nl[wv] = nl;
+      s_abs[wv] = abs;
     }
     lds_barrier();
     // carry before this wave, and after the whole round
     Carry before = carry;
     for (uint32_t w = 0; w < kScanWaves; ++w) {
       if (w == wv) before = carry;
-      carry = tbl_apply(s_tab[w], s_nl[w], carry);
+      carry = (SEG && s_abs[w]) ? abs_apply(s_tab[w], s_nl[w])
+                                : tbl_apply(s_tab[w], s_nl[w], carry);
     }
     uint32_t ea, eb, enl; // this lane's exclusive prefix (wave_shr:1)
     dpp_tbl<0x138, 0xF>((uint32_t)tab, (uint32_t)(tab >> 32), nl, ea, eb, enl);
-    Carry c = tbl_apply((uint64_t)ea | ((uint64_t)eb << 32), enl, before);
+    const uint64_t etab = (uint64_t)ea | ((uint64_t)eb << 32);
+    Carry c = tbl_apply(etab, enl, before);
+    if (SEG && __builtin_amdgcn_update_dpp(0, (int)abs, 0x138, 0xF, 0xF, false))
+      c = abs_apply(etab, enl);
 #pragma unroll
     for (uint32_t k = 0; k < kScanPer; ++k)
       if (k < m) {
-        c = tbl_apply(tbl_unpack(d[k]), (d[k] >> 40) & ((1ull << 22) - 1), c);
+        c = (SEG && (d[k] >> kFlagShift) == kFlagInclusive)
+                ? decode_inclusive(d[k])
+                : tbl_apply(tbl_unpack(d[k]), (d[k] >> 40) & ((1ull << 22) - 1), c);
         if (((b + k) & (kIncGroup - 1)) == kIncGroup - 1 || b + k + 1 == num_tiles)
           desc_store(&inc[b + k], encode_inclusive(c));
       }
@@ -447,14 +492,14 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
   for (uint64_t k = i; k < count; k += stride) desc[k] = 0;
   if (i == 0) {
     counter[0] = 0; // spare
-    counter[1] = 0; // look-back failure flag
+    counter[1] = 0; // failure flag of the bounded waits
   }
 }
 
-// The fused single-pass index kernel, plus the scanner workgroup (block 0)
-// for a single input.  A workgroup of 256 lanes takes whole 16 KiB tiles of
-// kTileWords words, one word per lane.  Single input: kTileHalves
-// consecutive tiles per workgroup (lane t holds word t of each); their
+// The fused single-pass index kernel, plus the scanner workgroup (block 0).
+// A workgroup of 256 lanes takes whole 16 KiB tiles of kTileWords words, one
+// word per lane: NH (1 or kTileHalves) consecutive tiles per workgroup (lane
+// t holds word t of each); their
 // carry-free work (classification, escapes, in-tile string parity, depth and
 // newline sums) runs one tile after the other, each tile publishing its
 // aggregate as soon as that is done, then the workgroup waits once, for its
@@ -464,9 +509,8 @@ __global__ void index_clear_kernel(uint64_t *desc, uint64_t count,
 // trips under load), so the bytes a wave slot holds across that wait, not
 // the work per byte, set the throughput: one tile per workgroup at 8 waves
 // per SIMD ran the 8 GiB input in 4.3 ms.  SEG: multi-file batch (see
-// BatchSeg): one tile per workgroup (files start on tiles), whose carry
-// comes from a decoupled look-back that never reads before its file's first
-// tile; that tile starts from the initial state.
+// BatchSeg): files start on tiles, a file's first tile starts from the
+// initial state, and a workgroup's two tiles may belong to different files.
 template <int LMAX, bool SEG, int NH>
 __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMAX <= 4) ? 8 : 4))) void index_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint64_t words, int levels,
@@ -510,39 +554,49 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMA
 
   // 0. tile id from the block: workgroups are dispatched in block order, so
   //    the tile a wait depends on has been dispatched or is next in line.
-  //    Single input: block 0 is the prefix scanner, block b takes tiles
-  //    NH (b - 1) ..  Multi-file: block b takes tiles NH b ..  (An atomic
-  //    tile counter costs ~11 ns per tile in series on MI355X.)
-  if (!SEG) {
-    if (blockIdx.x == 0) {
-      scan_tiles(desc, desc + num_tiles, num_tiles, error_flag, max_spins, lane, wv, init);
-      return;
-    }
-    if (t == 0) s_tile = NH * (blockIdx.x - 1);
-  } else if (t == 0) {
-    s_tile = NH * blockIdx.x;
+  //    Block 0 is the prefix scanner, block b takes tiles NH (b - 1) ..  (An
+  //    atomic tile counter costs ~11 ns per tile in series on MI355X.)
+  if (blockIdx.x == 0) {
+    scan_tiles<SEG>(desc, desc + num_tiles, num_tiles, error_flag, max_spins, lane, wv, init);
+    return;
   }
-  lds_barrier();
-  // tile and word base are wave-uniform (scalar registers); per-lane offsets
-  // are formed from them where needed rather than kept alive in VGPRs
-  const uint64_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_tile);
-  const uint64_t wbase = tile * (uint64_t)kTileWords;
-  // The file tile tt belongs to (the whole input when !SEG; wave-uniform):
-  // its first and last tiles, byte range (bytes past its end read as 0; a
-  // tile past the last one reads none), newline slice and line count.
+  // The file of the workgroup's tile h (the whole input when !SEG;
+  // wave-uniform): its first and last tiles, byte range (bytes past its end
+  // read as 0; a tile past the last one reads none), newline slice and line
+  // count.  SEG: read once into LDS.
   struct TileFile {
     uint64_t first, last, base, end;
     int64_t *nl_out;
     int64_t nl_cap;
     int64_t *lines_out;
   };
-  auto file_of = [&](uint64_t tt) -> TileFile {
-    if (!SEG) return TileFile{0, num_tiles - 1, 0, n, newline_out, newline_cap, num_lines_out};
+  __shared__ TileFile s_file[SEG ? NH : 1];
+  if (t == 0) s_tile = NH * (blockIdx.x - 1);
+  if (SEG && t < NH) {
+    const uint64_t tt = NH * (uint64_t)(blockIdx.x - 1) + t;
     const uint32_t fi = seg.tile_file[tt < num_tiles ? tt : num_tiles - 1];
     const BatchFile bf = seg.files[fi];
-    return TileFile{bf.offset / kTileBytes, (bf.offset + bf.bytes - 1) / kTileBytes, bf.offset,
-                    tt < num_tiles ? bf.offset + bf.bytes : 0, newline_out + bf.nl_offset,
-                    (int64_t)bf.nl_capacity, seg.line_counts + fi};
+    s_file[SEG ? t : 0] =
+        TileFile{bf.offset / kTileBytes, (bf.offset + bf.bytes - 1) / kTileBytes, bf.offset,
+                 tt < num_tiles ? bf.offset + bf.bytes : 0, newline_out + bf.nl_offset,
+                 (int64_t)bf.nl_capacity, seg.line_counts + fi};
+  }
+  lds_barrier();
+  // tile and word base are wave-uniform (scalar registers); per-lane offsets
+  // are formed from them where needed rather than kept alive in VGPRs
+  const uint64_t tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_tile);
+  const uint64_t wbase = tile * (uint64_t)kTileWords;
+  auto uni = [](uint64_t x) -> uint64_t { // (a wave-uniform LDS value into scalar registers)
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  };
+  auto file_of = [&](int h) -> TileFile {
+    if (!SEG) return TileFile{0, num_tiles - 1, 0, n, newline_out, newline_cap, num_lines_out};
+    const TileFile &f = s_file[SEG ? h : 0];
+    return TileFile{uni(f.first), uni(f.last), uni(f.base), uni(f.end),
+                    reinterpret_cast<int64_t *>(uni(reinterpret_cast<uint64_t>(f.nl_out))),
+                    (int64_t)uni((uint64_t)f.nl_cap),
+                    reinterpret_cast<int64_t *>(uni(reinterpret_cast<uint64_t>(f.lines_out)))};
   };
 
   // What each half keeps from its carry-free work until the carry arrives:
@@ -560,7 +614,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMA
   for (int h = 0; h < NH; ++h) {
     const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the tile's first word
     const uint64_t pos = (hb + t) * 64;
-    const uint64_t file_end = file_of(tile + h).end;
+    const uint64_t file_end = file_of(h).end;
 
     // 1. classify this lane's 64 bytes
     WordClasses m = {0, 0, 0, 0, 0, 0};
@@ -723,7 +777,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMA
       // tile's work overlaps the wait for the first's carry); a file's
       // first tile of a batch knows its inclusive state already
       if (lane == 0 && tile + h < num_tiles) {
-        if (SEG && tile + h == file_of(tile + h).first)
+        if (SEG && tile + h == file_of(h).first)
           desc_store(&desc[tile + h],
                      encode_inclusive(tbl_apply(tbl_unpack(htbl[h]), NL, decode_inclusive(init))));
         else
@@ -735,39 +789,47 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMA
 
   // 5. resolve the exclusive prefix of the workgroup's first tile (wave 0)
   if (wv == 0) {
-    const uint64_t first_tile = file_of(tile).first;
+    const uint64_t first_tile = file_of(0).first;
     Carry excl = decode_inclusive(init); // the first tile's carry-in
-    if (!SEG && tile != first_tile) {
+    if (tile != first_tile) {
       // wait for the inclusive state before this tile's group (the initial
-      // state for group 0) and for the aggregates of the group's earlier
+      // state for group 0) and for the descriptors of the group's earlier
       // tiles, all in one load per poll (lanes 0 .. need-1 and 63); stop when
-      // a failure is flagged
+      // a failure is flagged.  SEG: when the tile's file starts inside the
+      // group, its first tile's inclusive descriptor is the start instead.
       const uint64_t *inc = desc + num_tiles;
       const uint64_t g0 = tile & ~(kIncGroup - 1);
       const uint32_t need = (uint32_t)(tile - g0);
       uint32_t spins = 0;
-      uint64_t d = 0;
-      uint64_t acc = kIdentityTable, acc_nl = 0;
+      uint64_t start = 0;
+      uint32_t from = 0; // the group's first tile composed onto start
+      uint64_t v = 0;
       for (;;) {
         if (spins >= max_spins) { // bounded wait: give up, flag the failure
           if (lane == 0)
             __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          d = kFlagInclusive << kFlagShift;
+          start = kInitialInclusive;
+          from = need;
           break;
         }
-        uint64_t v = 0;
+        v = 0;
         if (lane < need) v = desc_load(&desc[g0 + lane]);
         else if (lane == 63) v = g0 > 0 ? desc_load(&inc[g0 - 1]) : init;
         const uint64_t missing = __ballot(lane < need && (v >> kFlagShift) == 0);
-        d = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63) << 32) |
-            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+        const uint64_t restart = SEG ? __ballot(lane < need && (v >> kFlagShift) == kFlagInclusive)
+                                     : 0ull;
+        if (missing == 0 && restart) {
+          const int k = 63 - __builtin_clzll(restart);
+          start = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), k) << 32) |
+                  (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k);
+          from = (uint32_t)k + 1;
+          break;
+        }
+        const uint64_t d = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
         if (missing == 0 && (d >> kFlagShift) == kFlagInclusive) {
-          for (uint32_t i = 0; i < need; ++i) {
-            const uint64_t a = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)i) << 32) |
-                               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)i);
-            acc = tbl_compose(tbl_unpack(a), acc);
-            acc_nl += (a >> 40) & ((1ull << 22) - 1);
-          }
+          start = d;
+          from = 0;
           break;
         }
         // another tile's failure ends this wait as well
@@ -776,82 +838,23 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMA
           spins = max_spins;
         __builtin_amdgcn_s_sleep(1);
       }
-      excl = tbl_apply(acc, acc_nl, decode_inclusive(d));
-
-    } else if (SEG && tile != first_tile) {
-      // Look back 64 predecessors at a time: lane i holds tile-1-i.  Compose
-      // the transfer tables newer-first up to the nearest inclusive prefix.
-      uint64_t acc = kIdentityTable;
-      uint64_t acc_nl = 0;
-      int64_t base = (int64_t)tile - 1;
-      uint32_t spins = 0;
-      // Wait for the nearest predecessor's aggregate with one load per poll:
-      // descriptor polls cross the XCDs' L2 caches, and a full window per
-      // poll from every waiting tile slows them all (3 % of kernel time).
-      if (base >= (int64_t)first_tile) {
-        for (;;) {
-          if (spins >= max_spins) break; // the window loop flags the failure
-          const uint64_t d0 = __builtin_amdgcn_readfirstlane((int)(uint32_t)(desc_load(&desc[base]) >> 32));
-          if ((d0 >> (kFlagShift - 32)) != 0) break;
-          ++spins;
-          __builtin_amdgcn_s_sleep(1);
-        }
+      uint64_t acc = kIdentityTable, acc_nl = 0;
+      for (uint32_t i = from; i < need; ++i) {
+        const uint64_t a = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)i) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)i);
+        acc = tbl_compose(tbl_unpack(a), acc);
+        acc_nl += (a >> 40) & ((1ull << 22) - 1);
       }
-      for (;;) {
-        const int64_t p = base - (int64_t)lane;
-        uint64_t d = 0;
-        uint32_t first_incl = 64;
-        for (;;) {
-          if (spins >= max_spins) { // bounded wait: give up, flag the failure
-            if (lane == 0)
-              __hip_atomic_store(error_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            d = kFlagInclusive << kFlagShift;
-            first_incl = 0;
-            break;
-          }
-          d = (p >= (int64_t)first_tile) ? desc_load(&desc[p]) : kInitialInclusive;
-          const uint64_t incl = __ballot((d >> kFlagShift) == kFlagInclusive);
-          const uint64_t zero = __ballot((d >> kFlagShift) == 0);
-          first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
-          const uint64_t needed = first_incl >= 63 ? ~0ull : ((2ull << first_incl) - 1ull);
-          if ((zero & needed) == 0) break;
-          ++spins;
-          __builtin_amdgcn_s_sleep(1);
-        }
-        // tables of lanes [0, first_incl), identity beyond; only as many
-        // reduction steps as the window needs (none when the nearest
-        // predecessor is already inclusive)
-        if (first_incl > 0) {
-          uint64_t t = (lane < first_incl) ? tbl_unpack(d) : kIdentityTable;
-          uint64_t nlv = (lane < first_incl) ? ((d >> 40) & ((1ull << 22) - 1)) : 0ull;
-          for (uint32_t off = 1; off < first_incl; off <<= 1) {
-            const uint64_t ot = __shfl_down(t, off, 64);
-            const uint64_t on = __shfl_down(nlv, off, 64);
-            if ((lane & (2 * off - 1)) == 0 && lane + off < 64) {
-              t = tbl_compose(t, ot);
-              nlv += on;
-            }
-          }
-          acc = tbl_compose(acc, __shfl(t, 0, 64));
-          acc_nl += __shfl(nlv, 0, 64);
-        }
-        if (first_incl < 64) {
-          const uint64_t inc = __shfl(d, (int)first_incl, 64);
-          excl = tbl_apply(acc, acc_nl, decode_inclusive(inc));
-          break;
-        }
-        base -= 64;
-      }
+      excl = tbl_apply(acc, acc_nl, decode_inclusive(start));
     }
     if (lane == 0) {
       Carry ch = excl;
 #pragma unroll
       for (int h = 0; h < NH; ++h) {
-        const TileFile f = file_of(tile + h);
+        const TileFile f = file_of(h);
         if (SEG && h > 0 && tile + h == f.first) ch = decode_inclusive(init); // a new file
         s_carry[h] = ch;
         ch = tbl_apply(tbl_unpack(htbl[h]), uNL[h], ch);
-        if (SEG && tile + h < num_tiles) desc_store(&desc[tile + h], encode_inclusive(ch));
         if (tile + h == f.last && tile + h < num_tiles) *f.lines_out = (int64_t)(1 + ch.nl);
       }
     }
@@ -862,7 +865,7 @@ __global__ __launch_bounds__(kTileWords) __attribute__((amdgpu_waves_per_eu((LMA
   for (int h = 0; h < NH; ++h) {
     const uint64_t hb = wbase + (uint64_t)h * kTileWords; // the tile's first word
     const Carry c = s_carry[h];
-    const TileFile f = file_of(tile + h);
+    const TileFile f = file_of(h);
     if (STASH && h == 0) { // (each lane reads back only its own slots)
       const uint32_t *const st = &s_work[t];
       kO[0] = (uint64_t)st[0 * kTileWords] | ((uint64_t)st[1 * kTileWords] << 32);
