@@ -178,3 +178,25 @@ def test_batch_shared_first_key_edge_forms(gpu, orc):
     batch_vs_oracle(gpu, orc, soup, ["$.a", "$.a[2]", "$.b.a", '$.a.b[?(@ == "x")]'], 3, "soup")
     # one first-key query among others: the memo stays off
     batch_vs_oracle(gpu, orc, data, ["$.a", "$[0]", "$"], 4, "single")
+
+
+def test_batch_shared_first_key_without_move_down(gpu, orc):
+    # bytecode the compiler does not emit but the interpreter runs: MOVE_TO_KEY
+    # "a" followed directly by STORE_RESULT + END, in a batch with `$.a`
+    # (MOVE_TO_KEY "a", MOVE_DOWN, ...) and `$.b.a`: the shared search of "a"
+    # must not take the MOVE_DOWN for both, and every result equals the
+    # oracle's
+    lines = [b'{"a":1,"b":{"a":2}}', b'{ "a" : [1, 2] ,"b":3}', b'{"b":{},"a":{"x":1}}',
+             b'{"x":1}', b'{"a":1', b'{"b":{"a":"q"},"a" :  "s"}']
+    data = (b"\n".join(lines) + b"\n") * 40
+    raw = bytes([4, 1]) + b"a" + bytes([1, 0])  # MOVE_TO_KEY "a", STORE_RESULT, END
+    compiled = [(raw, 1, 1), gpu.compile_query("$.a", extensions=True),
+                gpu.compile_query("$.b.a", extensions=True)]
+    dev = gpu.DeviceBuffer.from_host(np.frombuffer(data, dtype=np.uint8))
+    idx = gpu.build_index(dev, len(data), 3)
+    oidx = orc.build_index(data, 3)
+    batch = gpu.run_query_batch(dev, len(data), idx, compiled, capacity=4)
+    for (ir, _d, slots), (spans, counts) in zip(compiled, batch):
+        print(f"[batch] raw-ir {ir!r}", flush=True)
+        assert counts is None
+        np.testing.assert_array_equal(spans, orc.query(data, oidx, ir, slots))
